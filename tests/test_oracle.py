@@ -1,0 +1,322 @@
+"""Pins the CPU oracle (oracle/dtb_oracle.c) before it is trusted as the checker.
+
+- Philox4x32-10 against the Random123 known-answer vectors.
+- The block step against golden vectors written by the reference
+  models/block.py (tests/golden/make_golden.py) and against closed forms.
+- BOLD against the reference models/bold_model_pytorch.py golden vectors and
+  the fixed point of the Balloon-Windkessel ODE.
+- diffusion EnKF against the textbook perturbed-observation EnKF (np.cov),
+  identical-member / sigma->inf invariants, voxel-replication invariance and
+  linearity in rho.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+
+def load_case(golden_dir, name):
+    z = np.load(os.path.join(golden_dir, f"block_{name}.npz"))
+    return {k: z[k] for k in z.files}
+
+
+def case_csr(c):
+    return dict(rowptr=c["rowptr"], post=c["post"], kind=c["kind"], w0=c["w0"], w1=c["w1"], delay=None)
+
+
+# ---------------------------------------------------------------- Philox
+def test_philox_known_answers():
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert list(orc.philox4x32_10(ctr, key)) == want
+
+
+def test_noise_uniforms_range_and_moments():
+    u = orc.noise_uniforms(123, 7, 200000)
+    assert u.dtype == np.float32 and u.min() >= 0.0 and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.005 and abs(u.var() - 1 / 12) < 0.002
+    # multiples of 2^-24 exactly
+    assert np.all(np.floor(u.astype(np.float64) * 2 ** 24) == u.astype(np.float64) * 2 ** 24)
+
+
+# ------------------------------------------------------- generator (CSR)
+def test_generator_fixed_indegree_properties():
+    n, ne, K, D = 300, 240, 17, 5
+    g = orc.gen_fixed_indegree(n, ne, K, D, 0.25, 0.75, True, seed=99)
+    rp = g["rowptr"]
+    assert rp[0] == 0 and rp[-1] == n * K and np.all(np.diff(rp) >= 0)
+    pre = np.repeat(np.arange(n), np.diff(rp))
+    # fixed in-degree, no self-connections
+    assert np.all(np.bincount(g["post"], minlength=n) == K)
+    assert not np.any(pre == g["post"])
+    # receptor pair follows the presynaptic type
+    assert np.array_equal(g["kind"], (pre >= ne).astype(np.uint8))
+    assert g["delay"].min() >= 1 and g["delay"].max() <= D
+    assert set(np.unique(g["delay"])) == set(range(1, D + 1))
+    for w in (g["w0"], g["w1"]):
+        assert w.min() >= 0.25 and w.max() < 0.75
+    # rows sorted by postsynaptic index
+    for j in range(n):
+        seg = g["post"][rp[j]:rp[j + 1]]
+        assert np.all(np.diff(seg) >= 0)
+
+
+# ------------------------------------------------- block step vs golden
+@pytest.mark.parametrize("name,exact", [("dense64", True), ("isolated64", True), ("sparse400", False)])
+@pytest.mark.parametrize("pull", [False, True])
+def test_oracle_block_matches_reference_golden(golden_dir, name, exact, pull):
+    c = load_case(golden_dir, name)
+    n, steps, keep = int(c["n"]), int(c["steps"]), int(c["keep_every"])
+    ob = orc.OracleBlock(c["props"], case_csr(c), float(c["dt"]), pull=pull)
+    act = np.unpackbits(c["active"], axis=1)[:, :n]
+    spk = np.unpackbits(c["spike"], axis=1)[:, :n]
+    snaps = set(int(s) for s in c["snap_steps"])
+    # dyadic weights make the reference's float32 W.s exact: FP64 agreement to
+    # rounding; otherwise the reference sums W.s in float32 (models/block.py:72)
+    rtol = 1e-11 if exact else 2e-6
+    kv = 0
+    for s in range(1, steps + 1):
+        sa, mv = ob.run(noise_rate=float(c["noise_rate"]), isolated=bool(c["isolated"]), seed=int(c["seed"]))
+        assert np.array_equal(ob.get("active"), act[s - 1]), f"threshold raster differs at step {s}"
+        assert np.array_equal(ob.get("spike"), spk[s - 1]), f"delivered raster differs at step {s}"
+        assert np.array_equal(sa, c["sum_act"][s - 1])
+        np.testing.assert_allclose(mv, c["mean_v"][s - 1], rtol=rtol, atol=0)
+        if s % keep == 0:
+            np.testing.assert_allclose(ob.get("V"), c["V"][kv], rtol=rtol, atol=0)
+            kv += 1
+        if s in snaps:
+            scale = np.abs(c[f"J_{s}"]).max() + 1e-30
+            np.testing.assert_allclose(ob.get("J"), c[f"J_{s}"], rtol=rtol, atol=rtol * scale)
+            scale = np.abs(c[f"I_ui_{s}"]).max() + 1e-30
+            np.testing.assert_allclose(ob.get("I_ui"), c[f"I_ui_{s}"], rtol=rtol, atol=rtol * scale)
+    assert abs(ob.t - float(c["t_final"])) == 0.0
+
+
+# --------------------------------------------------- block closed forms
+def _props(n, **kw):
+    p = np.zeros((n, 22), dtype=np.float32)
+    d = dict(noise=0.0, blocked=0, I_ext=0.0, sub=0, C=1.0, T_ref=2.0, g_L=0.03, V_L=-75.0, V_th=-50.0,
+             V_reset=-65.0, g=(0, 0, 0, 0), Vu=(0, 0, -70, -100), tau=(2, 40, 10, 50))
+    d.update(kw)
+    p[:, 0] = d["noise"]; p[:, 1] = d["blocked"]; p[:, 2] = d["I_ext"]; p[:, 3] = d["sub"]
+    p[:, 4] = d["C"]; p[:, 5] = d["T_ref"]; p[:, 6] = d["g_L"]; p[:, 7] = d["V_L"]
+    p[:, 8] = d["V_th"]; p[:, 9] = d["V_reset"]; p[:, 10:14] = d["g"]; p[:, 14:18] = d["Vu"]
+    p[:, 18:22] = d["tau"]
+    return p
+
+
+def _empty_csr(n):
+    return dict(rowptr=np.zeros(n + 1, np.int64), post=np.zeros(0, np.int32), kind=np.zeros(0, np.uint8),
+                w0=np.zeros(0, np.float32), w1=None, delay=None)
+
+
+def _hold_steps(dt, T_ref):
+    """Steps held at V_reset after t_last=0: IEEE-double time accumulation."""
+    t, k = 0.0, 0
+    while True:
+        t += dt
+        k += 1
+        if t >= 0.0 + T_ref:
+            return k - 1
+
+
+def test_closed_form_leak_relaxation_and_initial_refractory_hold():
+    dt, gL, C = 0.1, 0.03, 1.0
+    ob = orc.OracleBlock(_props(1, g_L=gL, C=C), _empty_csr(1), dt)
+    hold = _hold_steps(dt, 2.0)
+    V = []
+    for _ in range(200):
+        ob.run(noise_rate=0.0)
+        V.append(ob.get("V")[0])
+    V = np.array(V)
+    assert np.all(V[:hold] == -65.0)
+    m = np.arange(1, 200 - hold + 1)
+    gL32 = np.float64(np.float32(gL))  # properties are float32 columns
+    want = -75.0 + (-65.0 + 75.0) * (1 - dt * gL32 / np.float64(np.float32(C))) ** m
+    np.testing.assert_allclose(V[hold:], want, rtol=1e-12)
+
+
+def test_closed_form_periodic_firing_under_constant_current():
+    # V* = V_L + I/g_L above threshold: spikes at a fixed interval
+    dt, gL, I = 0.1, 0.05, 2.0
+    ob = orc.OracleBlock(_props(1, g_L=gL, I_ext=I, T_ref=1.0), _empty_csr(1), dt)
+    spikes = []
+    for s in range(1, 3001):
+        ob.run(noise_rate=0.0)
+        if ob.get("active")[0]:
+            spikes.append(s)
+    gL32 = np.float64(np.float32(gL))
+    vstar = -75.0 + I / gL32
+    a = 1 - dt * gL32
+    # steps from V_reset to crossing V_th: smallest m with V* + (Vr-V*) a^m >= V_th
+    m = math.ceil(math.log((-50.0 - vstar) / (-65.0 - vstar)) / math.log(a))
+    isi = np.diff(spikes)
+    # after a spike at step s the neuron is held at V_reset while
+    # t < t_s + T_ref: T_ref/dt - 1 or T_ref/dt steps depending on how the
+    # accumulated time rounds; then it climbs m Euler steps to V_th
+    r = round(1.0 / dt)
+    assert set(isi.tolist()) <= {r - 1 + m, r + m}, (set(isi.tolist()), m)
+    assert len(set(isi.tolist())) == 2  # both roundings occur over 3000 steps
+    # the first spike: held from t_last = 0 exactly, then m steps
+    assert spikes[0] == _hold_steps(dt, 1.0) + m
+
+
+def test_closed_form_single_spike_delivery_and_exponential_decay():
+    # neuron 0 spikes (noise) exactly once; neuron 1 receives w on channel pair E
+    dt = 0.1
+    tau = (2.0, 40.0, 10.0, 50.0)
+    p = _props(2, g=(0, 0, 0, 0), tau=tau)
+    csr = dict(rowptr=np.array([0, 1, 1], np.int64), post=np.array([1], np.int32), kind=np.array([0], np.uint8),
+               w0=np.array([0.375], np.float32), w1=np.array([0.625], np.float32), delay=None)
+    ob = orc.OracleBlock(p, csr, dt)
+    seen = None
+    for s in range(1, 60):
+        u = np.ones(2, np.float32)
+        if s == 5:
+            u[0] = 0.0
+        ob.run(noise_rate=0.5, uniforms=u)
+        J = ob.get("J")[:, 1]
+        if s == 5:
+            assert J[0] == 0.375 and J[1] == 0.625 and J[2] == 0 and J[3] == 0
+            seen = s
+        elif seen:
+            k = s - seen
+            np.testing.assert_allclose(J[:2], [0.375 * math.exp(-dt / tau[0]) ** k,
+                                               0.625 * math.exp(-dt / tau[1]) ** k], rtol=1e-12)
+        else:
+            assert np.all(J == 0)
+
+
+def test_closed_form_conductance_geometric_decay_uses_current_V():
+    # g_L=0, one channel g=1, V_u=0, constant J (tau -> inf): V_{n+1} = V_n (1 - dt g J)
+    dt, w = 0.1, 0.5
+    p = _props(2, g_L=0.0, g=(1.0, 0, 0, 0), Vu=(0, 0, 0, 0), tau=(1e30, 1e30, 1e30, 1e30), T_ref=0.0,
+               V_th=100.0, V_reset=-65.0)
+    csr = dict(rowptr=np.array([0, 1, 1], np.int64), post=np.array([1], np.int32), kind=np.array([0], np.uint8),
+               w0=np.array([w], np.float32), w1=None, delay=None)
+    ob = orc.OracleBlock(p, csr, dt)
+    u = np.ones(2, np.float32)
+    u0 = u.copy(); u0[0] = 0.0
+    ob.run(noise_rate=0.5, uniforms=u0)   # step 1: neuron 0 fires once, J_1 = w
+    V1 = ob.get("V")[1]
+    assert V1 == (100.0 - 65.0) / 2  # (V_th+V_reset)/2: no current flows before the spike
+    for m in range(1, 50):
+        ob.run(noise_rate=0.5, uniforms=u)
+        np.testing.assert_allclose(ob.get("V")[1], V1 * (1 - dt * w) ** m, rtol=1e-12)
+
+
+def test_noise_spikes_do_not_trigger_refractoriness():
+    dt = 0.1
+    ob = orc.OracleBlock(_props(1, g_L=0.03), _empty_csr(1), dt)
+    ref = orc.OracleBlock(_props(1, g_L=0.03), _empty_csr(1), dt)
+    for s in range(100):
+        ob.run(noise_rate=0.5, uniforms=np.zeros(1, np.float32))  # noise every step
+        ref.run(noise_rate=0.0)
+        assert ob.get("spike")[0] == 1 and ob.get("active")[0] == 0
+        assert ob.get("V")[0] == ref.get("V")[0]
+
+
+def test_dyadic_weights_push_equals_pull_bitwise():
+    n, ne = 200, 160
+    g = orc.gen_fixed_indegree(n, ne, 30, 4, 0.0, 1.0, True, seed=3)
+    for k in ("w0", "w1"):
+        g[k] = (np.floor(g[k] * 256) / 256).astype(np.float32)
+    p = _props(n, g=(0.02, 0.002, 0.1, 0.02), T_ref=2.0)
+    p[:, 3] = np.arange(n) // 50
+    a = orc.OracleBlock(p, g, 0.1, pull=False)
+    b = orc.OracleBlock(p, g, 0.1, pull=True)
+    for s in range(300):
+        a.run(noise_rate=0.02, seed=4)
+        b.run(noise_rate=0.02, seed=4)
+        assert np.array_equal(a.get("J"), b.get("J"))
+        assert np.array_equal(a.get("V"), b.get("V"))
+
+
+def test_compaction_is_ascending_positions():
+    rng = np.random.default_rng(0)
+    f = (rng.random(5000) < 0.01).astype(np.uint8)
+    assert np.array_equal(orc.compact(f), np.nonzero(f)[0].astype(np.int32))
+    assert orc.compact(np.zeros(0, np.uint8)).shape == (0,)
+
+
+# ------------------------------------------------------------------ BOLD
+def test_oracle_bold_matches_reference_golden(golden_dir):
+    z = np.load(os.path.join(golden_dir, "bold_v18.npz"))
+    nv = z["u"].shape[1]
+    b = orc.OracleBOLD(nv)
+    k = 0
+    for s in range(z["u"].shape[0]):
+        out = b.run(z["u"][s])
+        np.testing.assert_allclose(out, z["out"][s], rtol=1e-12, atol=1e-15)
+        if (s + 1) % 10 == 0:
+            np.testing.assert_allclose(np.stack([b.s, b.q, b.v, b.f_in]), z["state"][k], rtol=1e-12, atol=1e-15)
+            k += 1
+
+
+def test_bold_fixed_point_and_steady_state():
+    b = orc.OracleBOLD(3, s=np.zeros(3), q=np.ones(3), v=np.ones(3), f_in=np.ones(3))
+    for _ in range(100):
+        out = b.run(np.zeros(3))
+    assert np.all(np.abs(out) < 1e-15)
+    eps, tf, alpha, E0 = 200.0, 0.4, 0.2, 0.8
+    u = np.array([0.001, 0.005, 0.01])
+    b = orc.OracleBOLD(3)
+    for _ in range(200000):
+        b.run(u)
+    f = 1 + eps * tf * u
+    v = f ** alpha
+    q = f * (1 - (1 - E0) ** (1 / f)) / E0 * v ** (1 - 1 / alpha)
+    np.testing.assert_allclose(b.f_in, f, rtol=1e-6)
+    np.testing.assert_allclose(b.v, v, rtol=1e-6)
+    np.testing.assert_allclose(b.q, q, rtol=1e-6)
+    assert np.all(np.abs(b.s) < 1e-8)
+
+
+# ------------------------------------------------------------------ EnKF
+def test_enkf_reduces_to_textbook_perturbed_observation_update():
+    rng = np.random.default_rng(1)
+    E, sigma = 40, 0.03
+    w = rng.normal(size=(E, 1, 3))
+    y = np.array([0.7])
+    xi = rng.normal(size=(E, 1))
+    out = orc.diffusion_enkf(w, sigma, y, 1.0, xi)
+    h = w[:, 0, -1]
+    ye = y[0] + math.sqrt(sigma) * xi[:, 0]
+    for s in range(3):
+        x = w[:, 0, s]
+        k = np.cov(x, h)[0, 1] / (np.var(h, ddof=1) + sigma)
+        np.testing.assert_allclose(out[:, 0, s], x + k * (ye - h), rtol=1e-12, atol=1e-12)
+    # one voxel: the global term equals the local term, so rho does not matter
+    np.testing.assert_allclose(orc.diffusion_enkf(w, sigma, y, 0.0, xi), out, rtol=1e-12, atol=1e-13)
+
+
+def test_enkf_invariants():
+    rng = np.random.default_rng(2)
+    E, V, S = 16, 5, 4
+    w = rng.normal(size=(E, V, S))
+    y = rng.normal(size=V)
+    xi = rng.normal(size=(E, V))
+    same = np.broadcast_to(w[:1], w.shape).copy()
+    np.testing.assert_array_equal(orc.diffusion_enkf(same, 0.1, y, 0.3, xi), same)
+    np.testing.assert_allclose(orc.diffusion_enkf(w, 1e300, y, 0.3, xi), w, rtol=0, atol=1e-200)
+    w0 = orc.diffusion_enkf(w, 0.1, y, 0.0, xi)
+    w1 = orc.diffusion_enkf(w, 0.1, y, 1.0, xi)
+    for rho in (0.25, 0.8):
+        np.testing.assert_allclose(orc.diffusion_enkf(w, 0.1, y, rho, xi), w0 + rho * (w1 - w0), rtol=1e-10,
+                                   atol=1e-12)
+    # identical voxels: the global coupling averages identical terms -> rho-free
+    rep = np.repeat(w[:, :1], V, axis=1)
+    xr = np.repeat(xi[:, :1], V, axis=1)
+    yr = np.repeat(y[:1], V)
+    np.testing.assert_allclose(orc.diffusion_enkf(rep, 0.1, yr, 0.0, xr), orc.diffusion_enkf(rep, 0.1, yr, 1.0, xr),
+                               rtol=1e-12, atol=1e-12)
+    # the global term couples voxels: it differs from the local one for V > 1
+    assert np.abs(w0 - w1).max() > 1e-6
