@@ -1,0 +1,1340 @@
+// dtb_block.hip -- the per-timestep network step of the Digital Twin Brain
+// on MI355X (gfx950), and its C ABI (include/dtb.h).
+//
+// One step n (t_n = t_{n-1} + dt, accumulated in double) is two kernels:
+//
+//   neuron_step_kernel  (one lane per neuron, coalesced SoA)
+//     J_{n-1} = Jp_{n-1} + ring[(n-1) mod D]      ring slot cleared
+//     I_syn   = sum_k g_k (V_uk - V_{n-1}) J_k     models/block.py:83-87
+//     V_n     = LIF Euler + refractory + clamp     models/block.py:89-111
+//     s_n     = noise | threshold                  models/block.py:136-138
+//     Jp_n    = J_{n-1} * exp(-dt/tau)             models/block.py:71
+//     spiking ids appended to the step's list (ballot + LDS prefix, one
+//     global atomic per workgroup); threshold counts and V sums per
+//     sub-population (wavefront reductions)
+//
+//   deliver_kernel  (one wavefront per spiking row, 64 synapses per pass)
+//     ring[(n + d - 1) mod D][post] += w          models/block.py:72-73,
+//     the event-driven form of W.s with per-synapse delays d (d = 1 is the
+//     reference's same-step delivery).
+//
+// So J_n = Jp_n + ring[n mod D] once the step's delivery has run; the next
+// neuron kernel folds the slot in and clears it.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "dtb_block.hpp"
+
+namespace dtb {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int block_alloc(Block* b, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? DTB_OUT_OF_MEMORY : DTB_RUNTIME_ERROR;
+    }
+    b->allocs.push_back(*p);
+    return DTB_OK;
+}
+
+template <typename T>
+static int alloc_n(Block* b, T** p, int64_t count)
+{
+    return block_alloc(b, reinterpret_cast<void**>(p), sizeof(T) * (size_t)std::max<int64_t>(count, 1));
+}
+
+__device__ inline float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// ---------------------------------------------------------------- kernels
+struct NeuronArgs {
+    int64_t n;
+    int32_t D, S;
+    const float4* __restrict__ g;
+    const float4* __restrict__ vu;
+    const float4* __restrict__ decay;
+    const float* __restrict__ gL;
+    const float* __restrict__ vL;
+    const float* __restrict__ dtC;
+    const float* __restrict__ vth;
+    const float* __restrict__ vrst;
+    const float* __restrict__ iext;
+    const float* __restrict__ tref;
+    const float* __restrict__ noisep;
+    const int32_t* __restrict__ pop;
+    float* __restrict__ V;
+    float4* __restrict__ Jp;
+    int32_t* __restrict__ ref_end;
+    int32_t* __restrict__ last_step;
+    double* __restrict__ t_last;
+    uint8_t* __restrict__ flags;
+    void* __restrict__ ring;
+    int32_t* __restrict__ spk;
+    uint32_t* __restrict__ spk_cnt;
+    const StepCtl* __restrict__ ctl;
+    const double* __restrict__ ttab;
+    double dt;
+    uint64_t seed;
+    int32_t noise_mode;
+    float noise_rate;
+    const float* __restrict__ uniforms;
+    int32_t isolated;
+    uint32_t* __restrict__ rec_freq;
+    float* __restrict__ rec_vsum;
+};
+
+// Steps until the accumulated time reaches t_spike + T_ref: the same double
+// additions the host uses for t, so `step >= ref_end` equals block.py's
+// `t >= t_last + T_ref` (models/block.py:100).
+__device__ inline int32_t refractory_end(int64_t step, double t_spike, float tref, double dt)
+{
+    const double lim = t_spike + (double)tref;
+    double tt = t_spike;
+    int64_t c = 0;
+    while (tt < lim && c < (1 << 22)) {
+        tt += dt;
+        ++c;
+    }
+    int64_t e = step + c;
+    return (int32_t)(e > 0x7fffffff ? 0x7fffffff : e);
+}
+
+template <bool PAIR>
+__global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t k)
+{
+    __shared__ uint32_t s_wcnt[4];
+    __shared__ uint32_t s_base;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t step = a.ctl->nbase + k + 1;
+    const double t = a.ttab[k];
+    const int par = (int)(step & 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.spk_cnt[par ^ 1] = 0; // list of step+1
+
+    bool act = false, spike = false;
+    float vout = 0.0f;
+    int32_t p = -1;
+    if (i < a.n) {
+        const uint32_t slot = (uint32_t)((step - 1) % a.D);
+        const float V = a.V[i];
+        float4 J = a.Jp[i];
+        if (PAIR) {
+            float4* r = reinterpret_cast<float4*>(a.ring) + (int64_t)slot * a.n + i;
+            const float4 d = *r;
+            *r = make_float4(0.f, 0.f, 0.f, 0.f);
+            J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
+        } else {
+            float2* r = reinterpret_cast<float2*>(a.ring) + (int64_t)slot * a.n + i;
+            const float2 d = *r;
+            *r = make_float2(0.f, 0.f);
+            J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
+        }
+        const float4 g = a.g[i];
+        const float4 vu = a.vu[i];
+        float isyn = g.x * (vu.x - V) * J.x;
+        isyn += g.y * (vu.y - V) * J.y;
+        isyn += g.z * (vu.z - V) * J.z;
+        isyn += g.w * (vu.w - V) * J.w;
+        const float main_part = -a.gL[i] * (V - a.vL[i]);
+        const float vn = V + a.dtC[i] * (main_part + isyn + a.iext[i]);
+        const bool not_ref = step >= (int64_t)a.ref_end[i];
+        const float vi = not_ref ? vn : a.vrst[i];
+        const float vth = a.vth[i];
+        act = vi >= vth;
+        vout = vi > vth ? vth : vi;
+        a.V[i] = vout;
+        if (act) {
+            a.ref_end[i] = refractory_end(step, t, a.tref[i], a.dt);
+            a.last_step[i] = (int32_t)step;
+            a.t_last[i] = t;
+        }
+        float u;
+        if (a.noise_mode == DTB_NOISE_UNIFORMS)
+            u = a.uniforms[(int64_t)k * a.n + i];
+        else
+            u = noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
+        const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
+        spike = (u < pr) || (act && !a.isolated);
+        const float4 dc = a.decay[i];
+        a.Jp[i] = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
+        a.flags[i] = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
+        p = a.pop[i];
+    }
+
+    // spike compaction: ballot + per-wave prefix, one global atomic per workgroup
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = threadIdx.x >> 6;
+    const uint64_t m = __ballot(spike);
+    const uint32_t pre = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wcnt[wid] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+        s_base = tot ? atomicAdd(&a.spk_cnt[par], tot) : 0u;
+    }
+    __syncthreads();
+    if (spike) {
+        uint32_t off = s_base;
+        for (uint32_t w = 0; w < wid; ++w) off += s_wcnt[w];
+        a.spk[(int64_t)par * a.n + off + pre] = (int32_t)i;
+    }
+
+    // per sub-population statistics (threshold spikes, V sum)
+    if (a.rec_freq || a.rec_vsum) {
+        const int32_t p0 = __shfl(p, 0);
+        if (__all(p == p0)) {
+            if (p0 >= 0) {
+                if (a.rec_freq) {
+                    const uint32_t c = (uint32_t)__popcll(__ballot(act));
+                    if (lane == 0 && c) atomicAdd(&a.rec_freq[(int64_t)k * a.S + p0], c);
+                }
+                if (a.rec_vsum) {
+                    const float v = wave_sum(vout);
+                    if (lane == 0) unsafeAtomicAdd(&a.rec_vsum[(int64_t)k * a.S + p0], v);
+                }
+            }
+        } else if (p >= 0) {
+            if (a.rec_freq && act) atomicAdd(&a.rec_freq[(int64_t)k * a.S + p], 1u);
+            if (a.rec_vsum) unsafeAtomicAdd(&a.rec_vsum[(int64_t)k * a.S + p], vout);
+        }
+    }
+}
+
+struct DeliverArgs {
+    int64_t n;
+    int32_t D;
+    SynPack pk;
+    const int64_t* __restrict__ rowptr;
+    const uint32_t* __restrict__ col;
+    const float* __restrict__ w;
+    const float2* __restrict__ w2;
+    void* __restrict__ ring;
+    const int32_t* __restrict__ spk;
+    const uint32_t* __restrict__ spk_cnt;
+    const StepCtl* __restrict__ ctl;
+    uint64_t* __restrict__ rec_events;
+    uint32_t* __restrict__ rec_nspk;
+};
+
+template <bool PAIR>
+__global__ __launch_bounds__(256) void deliver_kernel(DeliverArgs a, int32_t k)
+{
+    const int64_t step = a.ctl->nbase + k + 1;
+    const int par = (int)(step & 1);
+    const uint32_t cnt = min(a.spk_cnt[par], (uint32_t)a.n);
+    const uint32_t base_slot = (uint32_t)(step % a.D);
+    const int32_t* list = a.spk + (int64_t)par * a.n;
+    const uint32_t lane = __lane_id();
+    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    if (gw == 0 && lane == 0 && a.rec_nspk) a.rec_nspk[k] = cnt;
+    uint64_t ev = 0;
+    for (uint32_t r = gw; r < cnt; r += nw) {
+        const int32_t j = list[r];
+        const int64_t beg = a.rowptr[j];
+        const int64_t end = a.rowptr[j + 1];
+        ev += (uint64_t)(end - beg);
+        for (int64_t e = beg + lane; e < end; e += kWave) {
+            const uint32_t c = a.col[e];
+            const uint32_t post = c & a.pk.idx_mask;
+            uint32_t slot = base_slot + ((c >> a.pk.idx_bits) & a.pk.dmask);
+            if (slot >= (uint32_t)a.D) slot -= (uint32_t)a.D;
+            const uint32_t kind = c >> 31;
+            if (PAIR) {
+                const float2 wv = a.w2[e];
+                float* dst = reinterpret_cast<float*>(a.ring) + (((int64_t)slot * a.n + post) << 2) + (kind << 1);
+                unsafeAtomicAdd(dst, wv.x);
+                unsafeAtomicAdd(dst + 1, wv.y);
+            } else {
+                float* dst = reinterpret_cast<float*>(a.ring) + (((int64_t)slot * a.n + post) << 1) + kind;
+                unsafeAtomicAdd(dst, a.w[e]);
+            }
+        }
+    }
+    if (lane == 0 && ev && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev);
+}
+
+// Batch prologue: times of the K steps, step counters, record clearing.
+__global__ void prologue_kernel(StepCtl* ctl, double* ttab, int32_t K, double dt, uint32_t* rec_freq,
+                                float* rec_vsum, float* rec_isum, int64_t nrec, uint64_t* rec_events,
+                                uint32_t* rec_nspk)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (tid == 0) {
+        const int64_t n0 = ctl->nsteps;
+        double t = ctl->t;
+        for (int32_t k = 0; k < K; ++k) {
+            t += dt;
+            ttab[k] = t;
+        }
+        ctl->nbase = n0;
+        ctl->nsteps = n0 + K;
+        ctl->t = t;
+    }
+    for (int64_t x = tid; x < nrec; x += stride) {
+        if (rec_freq) rec_freq[x] = 0u;
+        if (rec_vsum) rec_vsum[x] = 0.0f;
+        if (rec_isum) rec_isum[x] = 0.0f;
+    }
+    for (int64_t x = tid; x < K; x += stride) {
+        rec_events[x] = 0ull;
+        rec_nspk[x] = 0u;
+    }
+}
+
+// Mean I_syn per sub-population at the end of a step (after delivery).
+template <bool PAIR>
+__global__ __launch_bounds__(256) void isum_kernel(NeuronArgs a, int32_t k, float* rec_isum)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t step = a.ctl->nbase + k + 1;
+    float isyn = 0.0f;
+    int32_t p = -1;
+    if (i < a.n) {
+        const uint32_t slot = (uint32_t)(step % a.D);
+        const float V = a.V[i];
+        float4 J = a.Jp[i];
+        if (PAIR) {
+            const float4 d = reinterpret_cast<const float4*>(a.ring)[(int64_t)slot * a.n + i];
+            J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
+        } else {
+            const float2 d = reinterpret_cast<const float2*>(a.ring)[(int64_t)slot * a.n + i];
+            J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
+        }
+        const float4 g = a.g[i];
+        const float4 vu = a.vu[i];
+        isyn = g.x * (vu.x - V) * J.x;
+        isyn += g.y * (vu.y - V) * J.y;
+        isyn += g.z * (vu.z - V) * J.z;
+        isyn += g.w * (vu.w - V) * J.w;
+        p = a.pop[i];
+    }
+    const int32_t p0 = __shfl(p, 0);
+    if (__all(p == p0)) {
+        if (p0 >= 0) {
+            const float v = wave_sum(isyn);
+            if (__lane_id() == 0) unsafeAtomicAdd(&rec_isum[(int64_t)k * a.S + p0], v);
+        }
+    } else if (p >= 0) {
+        unsafeAtomicAdd(&rec_isum[(int64_t)k * a.S + p], isyn);
+    }
+}
+
+__global__ void sample_kernel(int64_t ns, const uint32_t* __restrict__ idx, const float* __restrict__ V,
+                              const uint8_t* __restrict__ flags, int32_t k, uint8_t* __restrict__ sspk,
+                              float* __restrict__ sv)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const uint32_t j = idx[s];
+    sv[(int64_t)k * ns + s] = V[j];
+    sspk[(int64_t)k * ns + s] = (uint8_t)((flags[j] >> 1) & 1);
+}
+
+__global__ void scale_kernel(float* x, const float* inv, int64_t K, int32_t S)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < K * S) x[t] *= inv[t % S];
+}
+
+// Derived per-neuron arrays from the [n][22] property rows.
+__global__ void derive_kernel(int64_t n, const float* __restrict__ props, double dt, int32_t excl, float4* g,
+                              float4* vu, float4* decay, float* gL, float* vL, float* dtC, float* vth, float* vrst,
+                              float* iext, float* tref, float* noisep, int32_t* pop, int32_t* sub,
+                              int32_t* ref_end, const int32_t* last_step, const double* t_last)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* p = props + i * DTB_NPROP;
+    noisep[i] = p[0];
+    iext[i] = p[2];
+    const int32_t s = (int32_t)p[3];
+    sub[i] = s;
+    pop[i] = (excl && p[1] != 0.0f) ? -1 : s;
+    dtC[i] = (float)(dt / (double)p[4]);
+    tref[i] = p[5];
+    gL[i] = p[6];
+    vL[i] = p[7];
+    vth[i] = p[8];
+    vrst[i] = p[9];
+    g[i] = make_float4(p[10], p[11], p[12], p[13]);
+    vu[i] = make_float4(p[14], p[15], p[16], p[17]);
+    decay[i] = make_float4((float)exp(-dt / (double)p[18]), (float)exp(-dt / (double)p[19]),
+                           (float)exp(-dt / (double)p[20]), (float)exp(-dt / (double)p[21]));
+    ref_end[i] = refractory_end(last_step[i], t_last[i], p[5], dt);
+}
+
+__global__ void init_state_kernel(int64_t n, const float* __restrict__ props, float* V)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V[i] = (props[i * DTB_NPROP + 8] + props[i * DTB_NPROP + 9]) / 2.0f; // models/block.py:56
+}
+
+// CSR packing with validation; err[0] counts bad entries.
+__global__ void pack_kernel(int64_t nnz, int64_t n, const int32_t* __restrict__ post, const uint8_t* __restrict__ kind,
+                            const uint8_t* __restrict__ delay, SynPack pk, uint32_t* col, uint32_t* err)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const int32_t q = post[e];
+    const uint32_t d = delay ? (uint32_t)delay[e] : 1u;
+    const uint32_t kd = kind[e];
+    if (q < 0 || q >= n || d < 1 || d > (pk.dmask + 1u) || kd > 1u) {
+        atomicAdd(err, 1u);
+        col[e] = 0;
+        return;
+    }
+    col[e] = (uint32_t)q | ((d - 1u) << pk.idx_bits) | (kd << 31);
+}
+
+__global__ void interleave_kernel(int64_t nnz, const float* w0, const float* w1, float2* w2)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz) w2[e] = make_float2(w0[e], w1[e]);
+}
+
+__global__ void unpack_kernel(int64_t nnz, const uint32_t* __restrict__ col, SynPack pk, int32_t field, void* out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const uint32_t c = col[e];
+    if (field == DTB_FIELD_POST) reinterpret_cast<int32_t*>(out)[e] = (int32_t)(c & pk.idx_mask);
+    if (field == DTB_FIELD_KIND) reinterpret_cast<uint8_t*>(out)[e] = (uint8_t)(c >> 31);
+    if (field == DTB_FIELD_DELAY) reinterpret_cast<uint8_t*>(out)[e] = (uint8_t)(((c >> pk.idx_bits) & pk.dmask) + 1);
+}
+
+template <bool PAIR>
+__global__ void state_kernel(int64_t n, uint32_t slot, const float* __restrict__ V, const float4* __restrict__ Jp,
+                             const void* __restrict__ ring, const float4* __restrict__ g,
+                             const float4* __restrict__ vu, float* J_out, float* I_out, float* Isyn_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 J = Jp[i];
+    if (PAIR) {
+        const float4 d = reinterpret_cast<const float4*>(ring)[(int64_t)slot * n + i];
+        J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
+    } else {
+        const float2 d = reinterpret_cast<const float2*>(ring)[(int64_t)slot * n + i];
+        J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
+    }
+    const float v = V[i];
+    const float4 gg = g[i];
+    const float4 uu = vu[i];
+    const float i0 = gg.x * (uu.x - v) * J.x, i1 = gg.y * (uu.y - v) * J.y;
+    const float i2 = gg.z * (uu.z - v) * J.z, i3 = gg.w * (uu.w - v) * J.w;
+    if (J_out) {
+        J_out[i] = J.x; J_out[n + i] = J.y; J_out[2 * n + i] = J.z; J_out[3 * n + i] = J.w;
+    }
+    if (I_out) {
+        I_out[i] = i0; I_out[n + i] = i1; I_out[2 * n + i] = i2; I_out[3 * n + i] = i3;
+    }
+    if (Isyn_out) Isyn_out[i] = ((i0 + i1) + i2) + i3;
+}
+
+__global__ void flag_bit_kernel(int64_t n, const uint8_t* __restrict__ flags, int bit, uint8_t* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint8_t)((flags[i] >> bit) & 1);
+}
+
+// Property edits.  Requests are grouped by column into per-sub-population
+// tables (NaN: no request) so one pass over the neurons applies them all.
+__global__ void mul_props_kernel(int64_t n, int32_t nsub, const int32_t* __restrict__ sub,
+                                 const float* __restrict__ table, const int32_t* __restrict__ cols, int32_t ncols,
+                                 float* props)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = sub[i];
+    if (s < 0 || s >= nsub) return;
+    for (int32_t c = 0; c < ncols; ++c) {
+        const float f = table[(int64_t)c * nsub + s];
+        if (f == f) props[i * DTB_NPROP + cols[c]] *= f;
+    }
+}
+
+// Marsaglia & Tsang (2000), "A simple method for generating gamma
+// variables", ACM TOMS 26(3); shape < 1 boosted by U^(1/shape).  Normals by
+// Box-Muller.  All draws come from Philox4x32-10 keyed by (seed) with
+// counter (neuron, draw, GAMMA), 4 words per call.
+struct PhiloxStream {
+    uint32_t k0, k1;
+    uint64_t idx;
+    uint32_t ctr;
+    u32x4 buf;
+    int pos;
+    __device__ uint32_t next()
+    {
+        if (pos == 4) {
+            buf = philox4x32_10(u32x4{(uint32_t)idx, (uint32_t)(idx >> 32), ctr++, kStreamGamma}, k0, k1);
+            pos = 0;
+        }
+        const uint32_t v = pos == 0 ? buf.x : pos == 1 ? buf.y : pos == 2 ? buf.z : buf.w;
+        ++pos;
+        return v;
+    }
+    __device__ float uniform_open() { return ((float)(next() >> 8) + 0.5f) * (1.0f / 16777216.0f); } // (0,1)
+    __device__ float normal()
+    {
+        const float u1 = uniform_open(), u2 = uniform_open();
+        return sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+    }
+};
+
+__device__ float gamma_draw(float alpha, PhiloxStream& rs)
+{
+    if (!(alpha > 0.0f)) return 0.0f;
+    float scale = 1.0f;
+    if (alpha < 1.0f) {
+        scale = powf(rs.uniform_open(), 1.0f / alpha);
+        alpha += 1.0f;
+    }
+    const float d = alpha - 1.0f / 3.0f;
+    const float c = 1.0f / sqrtf(9.0f * d);
+    for (int it = 0; it < 1000; ++it) {
+        float x = 0.0f, v = 0.0f;
+        for (int tries = 0; tries < 64; ++tries) {
+            x = rs.normal();
+            v = 1.0f + c * x;
+            if (v > 0.0f) break;
+        }
+        if (!(v > 0.0f)) continue;
+        v = v * v * v;
+        const float u = rs.uniform_open();
+        const float x2 = x * x;
+        if (u < 1.0f - 0.0331f * x2 * x2) return scale * d * v;
+        if (logf(u) < 0.5f * x2 + d * (1.0f - v + logf(v))) return scale * d * v;
+    }
+    return scale * d;
+}
+
+__global__ void gamma_props_kernel(int64_t n, int32_t nsub, const int32_t* __restrict__ sub,
+                                   const float2* __restrict__ table, const int32_t* __restrict__ cols, int32_t ncols,
+                                   uint64_t seed, float* props)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = sub[i];
+    if (s < 0 || s >= nsub) return;
+    for (int32_t c = 0; c < ncols; ++c) {
+        const float2 ab = table[(int64_t)c * nsub + s];
+        if (!(ab.x == ab.x)) continue;
+        PhiloxStream rs{(uint32_t)seed, (uint32_t)(seed >> 32), (uint64_t)i, (uint32_t)cols[c] << 24, {}, 4};
+        props[i * DTB_NPROP + cols[c]] = gamma_draw(ab.x, rs) / ab.y;
+    }
+}
+
+__global__ void set_props_kernel(int64_t m, const uint32_t* __restrict__ neuron, const uint32_t* __restrict__ col,
+                                 const float* __restrict__ val, float* props)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) props[(int64_t)neuron[r] * DTB_NPROP + col[r]] = val[r];
+}
+
+// ------------------------------------------------------------- host side
+static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+
+static int upload(Block* b, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return DTB_OK;
+    DTB_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, b->stream));
+    return DTB_OK;
+}
+
+int derive_props(Block* b)
+{
+    hipLaunchKernelGGL(derive_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->props, b->dt,
+                       b->stats_excl, b->g, b->vu, b->decay, b->gL, b->vL, b->dtC, b->vth, b->vrst, b->iext, b->tref,
+                       b->noisep, b->pop, b->sub, b->ref_end, b->last_step, b->t_last);
+    DTB_TRY_HIP(hipGetLastError());
+    // statistics populations and their sizes (host copy of sub_idx / pop)
+    std::vector<int32_t> pop((size_t)b->n);
+    DTB_TRY_HIP(hipMemcpyAsync(pop.data(), b->pop, sizeof(int32_t) * (size_t)b->n, hipMemcpyDeviceToHost, b->stream));
+    std::vector<int32_t> sub((size_t)b->n);
+    DTB_TRY_HIP(hipMemcpyAsync(sub.data(), b->sub, sizeof(int32_t) * (size_t)b->n, hipMemcpyDeviceToHost, b->stream));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    int32_t S = 0;
+    for (int32_t s : sub) {
+        DTB_REQUIRE(s >= 0, "property column 3 (sub_idx) must be >= 0");
+        S = std::max(S, s + 1);
+    }
+    DTB_REQUIRE(b->S == 0 || S <= b->S, "sub_idx range grew from %d to %d after creation", b->S, S);
+    if (b->S == 0) b->S = S;
+    b->pop_size.assign((size_t)b->S, 0);
+    for (int32_t p : pop)
+        if (p >= 0) b->pop_size[(size_t)p]++;
+    std::vector<float> inv((size_t)b->S);
+    for (int32_t s = 0; s < b->S; ++s) inv[(size_t)s] = b->pop_size[(size_t)s] ? 1.0f / (float)b->pop_size[(size_t)s] : NAN;
+    if (!b->pop_inv) DTB_TRY(alloc_n(b, &b->pop_inv, b->S));
+    DTB_TRY_HIP(hipMemcpyAsync(b->pop_inv, inv.data(), sizeof(float) * (size_t)b->S, hipMemcpyHostToDevice, b->stream));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    return DTB_OK;
+}
+
+static int alloc_state(Block* b)
+{
+    const int64_t n = b->n;
+    DTB_TRY(alloc_n(b, &b->props, n * DTB_NPROP));
+    DTB_TRY(alloc_n(b, &b->g, n));
+    DTB_TRY(alloc_n(b, &b->vu, n));
+    DTB_TRY(alloc_n(b, &b->decay, n));
+    DTB_TRY(alloc_n(b, &b->gL, n));
+    DTB_TRY(alloc_n(b, &b->vL, n));
+    DTB_TRY(alloc_n(b, &b->dtC, n));
+    DTB_TRY(alloc_n(b, &b->vth, n));
+    DTB_TRY(alloc_n(b, &b->vrst, n));
+    DTB_TRY(alloc_n(b, &b->iext, n));
+    DTB_TRY(alloc_n(b, &b->tref, n));
+    DTB_TRY(alloc_n(b, &b->noisep, n));
+    DTB_TRY(alloc_n(b, &b->pop, n));
+    DTB_TRY(alloc_n(b, &b->sub, n));
+    DTB_TRY(alloc_n(b, &b->V, n));
+    DTB_TRY(alloc_n(b, &b->Jp, n));
+    DTB_TRY(alloc_n(b, &b->ref_end, n));
+    DTB_TRY(alloc_n(b, &b->last_step, n));
+    DTB_TRY(alloc_n(b, &b->t_last, n));
+    DTB_TRY(alloc_n(b, &b->flags, n));
+    DTB_TRY(alloc_n(b, &b->spk, 2 * n));
+    DTB_TRY(alloc_n(b, &b->spk_cnt, 2));
+    DTB_TRY(alloc_n(b, &b->ctl, 1));
+    b->kmax = 1000;
+    DTB_TRY(alloc_n(b, &b->ttab, b->kmax));
+    DTB_TRY(alloc_n(b, &b->rec_events, b->kmax));
+    DTB_TRY(alloc_n(b, &b->rec_nspk, b->kmax));
+    DTB_TRY_HIP(hipMemsetAsync(b->Jp, 0, sizeof(float4) * (size_t)n, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->last_step, 0, sizeof(int32_t) * (size_t)n, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->t_last, 0, sizeof(double) * (size_t)n, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->flags, 0, (size_t)n, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->spk_cnt, 0, 2 * sizeof(uint32_t), b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->ctl, 0, sizeof(StepCtl), b->stream));
+    return DTB_OK;
+}
+
+static int alloc_ring_and_records(Block* b)
+{
+    const size_t esz = b->pair ? sizeof(float4) : sizeof(float2);
+    DTB_TRY(block_alloc(b, &b->ring, esz * (size_t)b->D * (size_t)b->n));
+    DTB_TRY_HIP(hipMemsetAsync(b->ring, 0, esz * (size_t)b->D * (size_t)b->n, b->stream));
+    DTB_TRY(alloc_n(b, &b->rec_freq, b->kmax * b->S));
+    DTB_TRY(alloc_n(b, &b->rec_vsum, b->kmax * b->S));
+    DTB_TRY(alloc_n(b, &b->rec_isum, b->kmax * b->S));
+    // delivery grid: enough wavefronts for the expected spiking rows
+    const double expect = std::max(8.0, (double)b->n * 4e-3);
+    b->deliver_grid = (int32_t)std::min<double>(2048.0, std::max(4.0, std::ceil(expect / 4.0)));
+    return DTB_OK;
+}
+
+static int finish_create(Block* b, const float* props)
+{
+    DTB_TRY(upload(b, b->props, props, sizeof(float) * (size_t)b->n * DTB_NPROP));
+    hipLaunchKernelGGL(init_state_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->props, b->V);
+    DTB_TRY_HIP(hipGetLastError());
+    DTB_TRY(derive_props(b));
+    DTB_TRY(alloc_ring_and_records(b));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    return DTB_OK;
+}
+
+static void destroy(Block* b)
+{
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    for (void* p : b->allocs) hipFree(p);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+static int begin(Block** out, int device, int64_t n)
+{
+    DTB_REQUIRE(out != nullptr, "out handle pointer is NULL");
+    *out = nullptr;
+    DTB_REQUIRE(n >= 1, "n must be >= 1");
+    DTB_REQUIRE(n < (1ll << 30), "n = %lld exceeds the 2^30 neurons of one block", (long long)n);
+    int count = 0;
+    DTB_TRY_HIP(hipGetDeviceCount(&count));
+    DTB_REQUIRE(device >= 0 && device < count, "device %d not present (%d devices)", device, count);
+    DTB_TRY_HIP(hipSetDevice(device));
+    Block* b = new Block();
+    b->device = device;
+    b->n = n;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+        return DTB_RUNTIME_ERROR;
+    }
+    *out = b;
+    return DTB_OK;
+}
+
+static int make_pack(Block* b, int32_t D)
+{
+    b->D = D;
+    b->pk.idx_bits = std::max<uint32_t>(1, bits_for((uint64_t)b->n));
+    b->pk.dbits = bits_for((uint64_t)D);
+    DTB_REQUIRE(b->pk.idx_bits + b->pk.dbits <= 31, "n=%lld with max delay %d does not fit the 32-bit synapse word",
+                (long long)b->n, D);
+    b->pk.idx_mask = (uint32_t)((1ull << b->pk.idx_bits) - 1);
+    b->pk.dmask = (uint32_t)((1ull << b->pk.dbits) - 1);
+    return DTB_OK;
+}
+
+static int create_from_csr(Block* b, const dtb_block_desc* d)
+{
+    const int64_t n = d->n, nnz = d->nnz;
+    DTB_REQUIRE(nnz >= 0, "nnz must be >= 0");
+    DTB_REQUIRE(d->props && d->rowptr, "props and rowptr are required");
+    DTB_REQUIRE(nnz == 0 || (d->post && d->kind && d->w0), "post, kind and w0 are required when nnz > 0");
+    DTB_REQUIRE(d->dt > 0.0, "dt must be > 0");
+    b->nnz = nnz;
+    b->dt = d->dt;
+    b->seed = d->noise_seed;
+    b->stats_excl = d->stats_exclude_blocked;
+    b->pair = d->w1 != nullptr;
+    // rowptr check and max delay on the host
+    std::vector<int64_t> rp((size_t)n + 1);
+    DTB_TRY_HIP(hipMemcpy(rp.data(), d->rowptr, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyDefault));
+    DTB_REQUIRE(rp[0] == 0 && rp[(size_t)n] == nnz, "rowptr must start at 0 and end at nnz");
+    for (int64_t j = 0; j < n; ++j)
+        DTB_REQUIRE(rp[(size_t)j + 1] >= rp[(size_t)j], "rowptr must be non-decreasing (row %lld)", (long long)j);
+    int32_t D = 1;
+    if (d->delay && nnz > 0) {
+        std::vector<uint8_t> dl((size_t)nnz);
+        DTB_TRY_HIP(hipMemcpy(dl.data(), d->delay, (size_t)nnz, hipMemcpyDefault));
+        for (uint8_t x : dl) D = std::max<int32_t>(D, x);
+        DTB_REQUIRE(D <= DTB_MAX_DELAY, "delay %d exceeds DTB_MAX_DELAY", D);
+    }
+    DTB_TRY(make_pack(b, D));
+    DTB_TRY(alloc_state(b));
+    DTB_TRY(alloc_n(b, &b->rowptr, n + 1));
+    DTB_TRY(upload(b, b->rowptr, rp.data(), sizeof(int64_t) * (size_t)(n + 1)));
+    DTB_TRY(alloc_n(b, &b->col, nnz));
+    if (nnz > 0) {
+        // stage the raw arrays on the device, pack and validate there
+        void *dpost, *dkind, *ddelay = nullptr, *derr;
+        DTB_TRY_HIP(hipMalloc(&dpost, sizeof(int32_t) * (size_t)nnz));
+        DTB_TRY_HIP(hipMalloc(&dkind, (size_t)nnz));
+        if (d->delay) DTB_TRY_HIP(hipMalloc(&ddelay, (size_t)nnz));
+        DTB_TRY_HIP(hipMalloc(&derr, sizeof(uint32_t)));
+        int st = DTB_OK;
+        uint32_t nerr = 0;
+        do {
+            if ((st = upload(b, dpost, d->post, sizeof(int32_t) * (size_t)nnz))) break;
+            if ((st = upload(b, dkind, d->kind, (size_t)nnz))) break;
+            if (d->delay && (st = upload(b, ddelay, d->delay, (size_t)nnz))) break;
+            if (hipMemsetAsync(derr, 0, sizeof(uint32_t), b->stream) != hipSuccess) { st = DTB_RUNTIME_ERROR; break; }
+            hipLaunchKernelGGL(pack_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, b->stream, nnz, n,
+                               (const int32_t*)dpost, (const uint8_t*)dkind, (const uint8_t*)ddelay, b->pk, b->col,
+                               (uint32_t*)derr);
+            if (hipMemcpyAsync(&nerr, derr, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+                hipStreamSynchronize(b->stream) != hipSuccess) {
+                st = DTB_RUNTIME_ERROR;
+                set_error("CSR packing failed");
+            }
+        } while (0);
+        hipFree(dpost);
+        hipFree(dkind);
+        if (ddelay) hipFree(ddelay);
+        hipFree(derr);
+        if (st) return st;
+        DTB_REQUIRE(nerr == 0, "%u synapses have post outside [0,n), kind not in {0,1} or delay outside 1..64", nerr);
+        if (b->pair) {
+            DTB_TRY(alloc_n(b, &b->w2, nnz));
+            float *d0, *d1;
+            DTB_TRY_HIP(hipMalloc(&d0, sizeof(float) * (size_t)nnz));
+            DTB_TRY_HIP(hipMalloc(&d1, sizeof(float) * (size_t)nnz));
+            int s2 = upload(b, d0, d->w0, sizeof(float) * (size_t)nnz);
+            if (!s2) s2 = upload(b, d1, d->w1, sizeof(float) * (size_t)nnz);
+            if (!s2) {
+                hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, b->stream, nnz, d0, d1,
+                                   b->w2);
+                if (hipStreamSynchronize(b->stream) != hipSuccess) s2 = DTB_RUNTIME_ERROR;
+            }
+            hipFree(d0);
+            hipFree(d1);
+            if (s2) return s2;
+        } else {
+            DTB_TRY(alloc_n(b, &b->w, nnz));
+            DTB_TRY(upload(b, b->w, d->w0, sizeof(float) * (size_t)nnz));
+        }
+    } else {
+        if (b->pair) DTB_TRY(alloc_n(b, &b->w2, 1));
+        else DTB_TRY(alloc_n(b, &b->w, 1));
+    }
+    return finish_create(b, d->props);
+}
+
+// ------------------------------------------------------------- the run loop
+static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* uniforms, bool freq, bool vsum)
+{
+    NeuronArgs x;
+    x.n = b->n; x.D = b->D; x.S = b->S;
+    x.g = b->g; x.vu = b->vu; x.decay = b->decay; x.gL = b->gL; x.vL = b->vL; x.dtC = b->dtC; x.vth = b->vth;
+    x.vrst = b->vrst; x.iext = b->iext; x.tref = b->tref; x.noisep = b->noisep; x.pop = b->pop;
+    x.V = b->V; x.Jp = b->Jp; x.ref_end = b->ref_end; x.last_step = b->last_step; x.t_last = b->t_last;
+    x.flags = b->flags; x.ring = b->ring; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.ctl = b->ctl; x.ttab = b->ttab;
+    x.dt = b->dt; x.seed = b->seed;
+    x.noise_mode = a->noise_mode; x.noise_rate = a->noise_rate; x.uniforms = uniforms; x.isolated = a->isolated;
+    x.rec_freq = freq ? b->rec_freq : nullptr;
+    x.rec_vsum = vsum ? b->rec_vsum : nullptr;
+    return x;
+}
+
+static DeliverArgs deliver_args(Block* b)
+{
+    DeliverArgs x;
+    x.n = b->n; x.D = b->D; x.pk = b->pk; x.rowptr = b->rowptr; x.col = b->col; x.w = b->w; x.w2 = b->w2;
+    x.ring = b->ring; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.ctl = b->ctl; x.rec_events = b->rec_events;
+    x.rec_nspk = b->rec_nspk;
+    return x;
+}
+
+struct Want {
+    bool freq, vsum, isum, sample;
+};
+
+static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float* uniforms, Want w)
+{
+    const int64_t nrec = (int64_t)K * b->S;
+    hipLaunchKernelGGL(prologue_kernel, dim3(grid_for(std::max<int64_t>(nrec, K), 256 * 4)), dim3(256), 0,
+                       b->stream, b->ctl, b->ttab, K, b->dt, w.freq ? b->rec_freq : nullptr,
+                       w.vsum ? b->rec_vsum : nullptr, w.isum ? b->rec_isum : nullptr, nrec, b->rec_events,
+                       b->rec_nspk);
+    const NeuronArgs na = neuron_args(b, a, uniforms, w.freq, w.vsum);
+    const DeliverArgs da = deliver_args(b);
+    const unsigned ng = grid_for(b->n, 256);
+    for (int32_t k = 0; k < K; ++k) {
+        if (b->pair) {
+            hipLaunchKernelGGL(neuron_step_kernel<true>, dim3(ng), dim3(256), 0, b->stream, na, k);
+            hipLaunchKernelGGL(deliver_kernel<true>, dim3(b->deliver_grid), dim3(256), 0, b->stream, da, k);
+            if (w.isum) hipLaunchKernelGGL(isum_kernel<true>, dim3(ng), dim3(256), 0, b->stream, na, k, b->rec_isum);
+        } else {
+            hipLaunchKernelGGL(neuron_step_kernel<false>, dim3(ng), dim3(256), 0, b->stream, na, k);
+            hipLaunchKernelGGL(deliver_kernel<false>, dim3(b->deliver_grid), dim3(256), 0, b->stream, da, k);
+            if (w.isum) hipLaunchKernelGGL(isum_kernel<false>, dim3(ng), dim3(256), 0, b->stream, na, k, b->rec_isum);
+        }
+        if (w.sample)
+            hipLaunchKernelGGL(sample_kernel, dim3(grid_for(b->ns, 256)), dim3(256), 0, b->stream, b->ns, b->samples,
+                               b->V, b->flags, k, b->rec_sspk, b->rec_sv);
+    }
+    if (w.vsum)
+        hipLaunchKernelGGL(scale_kernel, dim3(grid_for(nrec, 256)), dim3(256), 0, b->stream, b->rec_vsum, b->pop_inv,
+                           (int64_t)K, b->S);
+    if (w.isum)
+        hipLaunchKernelGGL(scale_kernel, dim3(grid_for(nrec, 256)), dim3(256), 0, b->stream, b->rec_isum, b->pop_inv,
+                           (int64_t)K, b->S);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+static int copy_out(Block* b, void* dst, const void* src, size_t bytes)
+{
+    if (!dst || bytes == 0) return DTB_OK;
+    DTB_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, b->stream));
+    return DTB_OK;
+}
+
+static bool is_device_ptr(const void* p)
+{
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice;
+}
+
+static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
+{
+    DTB_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+    DTB_REQUIRE(a != nullptr, "run args are NULL");
+    DTB_REQUIRE(a->noise_mode >= 0 && a->noise_mode <= 2, "noise_mode must be 0, 1 or 2");
+    DTB_REQUIRE(a->noise_mode != DTB_NOISE_UNIFORMS || a->uniforms, "noise_mode 2 needs uniforms");
+    DTB_REQUIRE(b->steps_done + nsteps < 0x7fff0000ll, "step counter limit (2^31) reached");
+    const Want w{a->freqs != nullptr, a->vmeans != nullptr, a->imeans != nullptr,
+                 a->sample_spike != nullptr || a->sample_v != nullptr};
+    if (w.sample && b->ns == 0) {
+        set_error("sample outputs requested but no samples are set");
+        return DTB_UNSET_SAMPLE_FAILURE;
+    }
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    float* ustage = nullptr;
+    const bool uhost = a->noise_mode == DTB_NOISE_UNIFORMS && !is_device_ptr(a->uniforms);
+    if (uhost) DTB_TRY_HIP(hipMalloc(&ustage, sizeof(float) * (size_t)b->n * (size_t)std::min<int64_t>(nsteps, b->kmax)));
+    int st = DTB_OK;
+    int64_t done = 0;
+    while (done < nsteps && st == DTB_OK) {
+        const int32_t K = (int32_t)std::min<int64_t>(nsteps - done, b->kmax);
+        const float* u = nullptr;
+        if (a->noise_mode == DTB_NOISE_UNIFORMS) {
+            const float* src = a->uniforms + (size_t)done * (size_t)b->n;
+            if (uhost) {
+                if ((st = upload(b, ustage, src, sizeof(float) * (size_t)b->n * (size_t)K))) break;
+                u = ustage;
+            } else {
+                u = src;
+            }
+        }
+        const bool graph = a->use_graph && a->noise_mode != DTB_NOISE_UNIFORMS;
+        if (graph) {
+            uint32_t nr;
+            std::memcpy(&nr, &a->noise_rate, 4);
+            std::vector<uint64_t> key{(uint64_t)K, (uint64_t)a->noise_mode, (uint64_t)a->isolated, nr,
+                                      (uint64_t)w.freq, (uint64_t)w.vsum, (uint64_t)w.isum, (uint64_t)w.sample,
+                                      (uint64_t)b->ns};
+            auto it = b->graphs.find(key);
+            if (it == b->graphs.end()) {
+                hipGraph_t g;
+                if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    set_error("hipStreamBeginCapture failed");
+                    st = DTB_RUNTIME_ERROR;
+                    break;
+                }
+                int s2 = enqueue_batch(b, K, a, nullptr, w);
+                hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+                if (s2 || e2 != hipSuccess) {
+                    if (!s2) set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+                    st = s2 ? s2 : DTB_RUNTIME_ERROR;
+                    break;
+                }
+                hipGraphExec_t ge;
+                e2 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (e2 != hipSuccess) {
+                    set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e2));
+                    st = DTB_RUNTIME_ERROR;
+                    break;
+                }
+                it = b->graphs.emplace(key, ge).first;
+            }
+            if (hipGraphLaunch(it->second, b->stream) != hipSuccess) {
+                set_error("hipGraphLaunch failed");
+                st = DTB_RUNTIME_ERROR;
+                break;
+            }
+        } else {
+            if ((st = enqueue_batch(b, K, a, u, w))) break;
+        }
+        const size_t S = (size_t)b->S;
+        if ((st = copy_out(b, a->freqs ? a->freqs + done * S : nullptr, b->rec_freq, sizeof(uint32_t) * S * K))) break;
+        if ((st = copy_out(b, a->vmeans ? a->vmeans + done * S : nullptr, b->rec_vsum, sizeof(float) * S * K))) break;
+        if ((st = copy_out(b, a->imeans ? a->imeans + done * S : nullptr, b->rec_isum, sizeof(float) * S * K))) break;
+        if ((st = copy_out(b, a->sample_spike ? a->sample_spike + done * b->ns : nullptr, b->rec_sspk,
+                           (size_t)b->ns * K)))
+            break;
+        if ((st = copy_out(b, a->sample_v ? a->sample_v + done * b->ns : nullptr, b->rec_sv,
+                           sizeof(float) * (size_t)b->ns * K)))
+            break;
+        if ((st = copy_out(b, a->events ? a->events + done : nullptr, b->rec_events, sizeof(uint64_t) * K))) break;
+        if ((st = copy_out(b, a->spikes ? a->spikes + done : nullptr, b->rec_nspk, sizeof(uint32_t) * K))) break;
+        if (uhost || K < nsteps - done) {
+            // the staging buffer and records are reused by the next batch
+            if (hipStreamSynchronize(b->stream) != hipSuccess) {
+                set_error("step batch failed: %s", hipGetErrorString(hipGetLastError()));
+                st = DTB_RUNTIME_ERROR;
+                break;
+            }
+        }
+        for (int32_t k = 0; k < K; ++k) b->t_host += b->dt;
+        b->steps_done += K;
+        done += K;
+    }
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (ustage) hipFree(ustage);
+    if (st) return st;
+    if (e != hipSuccess) {
+        set_error("step batch failed: %s", hipGetErrorString(e));
+        return DTB_RUNTIME_ERROR;
+    }
+    return DTB_OK;
+}
+
+static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* bytes_out)
+{
+    DTB_REQUIRE(dst != nullptr || field == DTB_FIELD_SPIKE_LIST, "dst is NULL");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    const int64_t n = b->n, nnz = b->nnz;
+    auto need = [&](int64_t want) -> int {
+        if (bytes_out) *bytes_out = want;
+        if (bytes < want) {
+            set_error("field %d needs %lld bytes, got %lld", field, (long long)want, (long long)bytes);
+            return DTB_INVALID_PARAMETER;
+        }
+        return DTB_OK;
+    };
+    void* tmp = nullptr;
+    int st = DTB_OK;
+    switch (field) {
+    case DTB_FIELD_V:
+        DTB_TRY(need(4 * n));
+        DTB_TRY(copy_out(b, dst, b->V, 4 * (size_t)n));
+        break;
+    case DTB_FIELD_J:
+    case DTB_FIELD_I_UI:
+    case DTB_FIELD_I_SYN: {
+        const int64_t sz = field == DTB_FIELD_I_SYN ? 4 * n : 16 * n;
+        DTB_TRY(need(sz));
+        DTB_TRY_HIP(hipMalloc(&tmp, (size_t)sz));
+        float* J = field == DTB_FIELD_J ? (float*)tmp : nullptr;
+        float* I = field == DTB_FIELD_I_UI ? (float*)tmp : nullptr;
+        float* Is = field == DTB_FIELD_I_SYN ? (float*)tmp : nullptr;
+        const uint32_t slot = (uint32_t)(b->steps_done % b->D);
+        if (b->pair)
+            hipLaunchKernelGGL(state_kernel<true>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, slot, b->V,
+                               b->Jp, b->ring, b->g, b->vu, J, I, Is);
+        else
+            hipLaunchKernelGGL(state_kernel<false>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, slot, b->V,
+                               b->Jp, b->ring, b->g, b->vu, J, I, Is);
+        st = copy_out(b, dst, tmp, (size_t)sz);
+        break;
+    }
+    case DTB_FIELD_ACTIVE:
+    case DTB_FIELD_SPIKE:
+        DTB_TRY(need(n));
+        DTB_TRY_HIP(hipMalloc(&tmp, (size_t)n));
+        hipLaunchKernelGGL(flag_bit_kernel, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, b->flags,
+                           field == DTB_FIELD_ACTIVE ? 0 : 1, (uint8_t*)tmp);
+        st = copy_out(b, dst, tmp, (size_t)n);
+        break;
+    case DTB_FIELD_SPIKE_LIST: {
+        const int par = (int)(b->steps_done & 1);
+        uint32_t cnt = 0;
+        if (b->steps_done > 0) {
+            DTB_TRY_HIP(hipMemcpyAsync(&cnt, b->spk_cnt + par, 4, hipMemcpyDeviceToHost, b->stream));
+            DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+        }
+        DTB_TRY(need(4 * (int64_t)cnt));
+        if (cnt) {
+            std::vector<int32_t> l(cnt);
+            DTB_TRY_HIP(hipMemcpyAsync(l.data(), b->spk + (int64_t)par * n, 4 * (size_t)cnt, hipMemcpyDeviceToHost,
+                                       b->stream));
+            DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+            std::sort(l.begin(), l.end());
+            DTB_TRY_HIP(hipMemcpy(dst, l.data(), 4 * (size_t)cnt, hipMemcpyDefault));
+        }
+        break;
+    }
+    case DTB_FIELD_PROPS:
+        DTB_TRY(need(4 * n * DTB_NPROP));
+        DTB_TRY(copy_out(b, dst, b->props, 4 * (size_t)n * DTB_NPROP));
+        break;
+    case DTB_FIELD_ROWPTR:
+        DTB_TRY(need(8 * (n + 1)));
+        DTB_TRY(copy_out(b, dst, b->rowptr, 8 * (size_t)(n + 1)));
+        break;
+    case DTB_FIELD_POST:
+    case DTB_FIELD_KIND:
+    case DTB_FIELD_DELAY: {
+        const int64_t sz = field == DTB_FIELD_POST ? 4 * nnz : nnz;
+        DTB_TRY(need(sz));
+        if (nnz == 0) break;
+        DTB_TRY_HIP(hipMalloc(&tmp, (size_t)sz));
+        hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, b->stream, nnz, b->col, b->pk, field,
+                           tmp);
+        st = copy_out(b, dst, tmp, (size_t)sz);
+        break;
+    }
+    case DTB_FIELD_W0:
+    case DTB_FIELD_W1:
+        DTB_TRY(need(4 * nnz));
+        if (nnz == 0) break;
+        if (b->pair) {
+            DTB_TRY_HIP(hipMemcpy2DAsync(dst, 4, reinterpret_cast<const float*>(b->w2) + (field == DTB_FIELD_W1), 8, 4,
+                                         (size_t)nnz, hipMemcpyDefault, b->stream));
+        } else {
+            DTB_TRY(copy_out(b, dst, b->w, 4 * (size_t)nnz));
+        }
+        break;
+    case DTB_FIELD_T_LAST:
+        DTB_TRY(need(8 * n));
+        DTB_TRY(copy_out(b, dst, b->t_last, 8 * (size_t)n));
+        break;
+    case DTB_FIELD_POP_SIZES:
+        DTB_TRY(need(8 * (int64_t)b->S));
+        DTB_TRY_HIP(hipMemcpy(dst, b->pop_size.data(), 8 * (size_t)b->S, hipMemcpyDefault));
+        break;
+    default:
+        set_error("unknown field %d", field);
+        return DTB_INVALID_PARAMETER;
+    }
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (tmp) hipFree(tmp);
+    if (st) return st;
+    DTB_TRY_HIP(e);
+    return DTB_OK;
+}
+
+// Property requests grouped by column into [ncols][S] tables.
+template <typename T>
+static int group_requests(Block* b, int64_t m, const uint32_t* pop, const uint32_t* col, const T* val, T none,
+                          std::vector<int32_t>& cols, std::vector<T>& table)
+{
+    std::vector<uint32_t> hp((size_t)m), hc((size_t)m);
+    std::vector<T> hv((size_t)m);
+    DTB_TRY_HIP(hipMemcpy(hp.data(), pop, 4 * (size_t)m, hipMemcpyDefault));
+    DTB_TRY_HIP(hipMemcpy(hc.data(), col, 4 * (size_t)m, hipMemcpyDefault));
+    DTB_TRY_HIP(hipMemcpy(hv.data(), val, sizeof(T) * (size_t)m, hipMemcpyDefault));
+    int32_t colpos[DTB_NPROP];
+    std::fill(colpos, colpos + DTB_NPROP, -1);
+    cols.clear();
+    for (int64_t r = 0; r < m; ++r) {
+        DTB_REQUIRE(hc[(size_t)r] < DTB_NPROP, "property column %u out of range", hc[(size_t)r]);
+        DTB_REQUIRE(hc[(size_t)r] != 3, "property column 3 (sub_idx) cannot be changed");
+        DTB_REQUIRE(hp[(size_t)r] < (uint32_t)b->S, "sub-population %u out of range [0,%d)", hp[(size_t)r], b->S);
+        if (colpos[hc[(size_t)r]] < 0) {
+            colpos[hc[(size_t)r]] = (int32_t)cols.size();
+            cols.push_back((int32_t)hc[(size_t)r]);
+        }
+    }
+    table.assign(cols.size() * (size_t)b->S, none);
+    for (int64_t r = 0; r < m; ++r) {
+        T& slot = table[(size_t)colpos[hc[(size_t)r]] * b->S + hp[(size_t)r]];
+        slot = hv[(size_t)r];
+    }
+    return DTB_OK;
+}
+
+} // namespace dtb
+
+using namespace dtb;
+
+struct dtb_block : public Block {};
+
+extern "C" {
+
+const char* dtb_last_error(void) { return g_err; }
+
+int32_t dtb_abi_version(void) { return 1; }
+
+int dtb_block_create(dtb_block** out, const dtb_block_desc* desc)
+{
+    DTB_REQUIRE(desc != nullptr, "desc is NULL");
+    Block* b = nullptr;
+    DTB_TRY(begin(&b, desc->device, desc->n));
+    int st = create_from_csr(b, desc);
+    if (st) {
+        destroy(b);
+        return st;
+    }
+    *out = static_cast<dtb_block*>(b);
+    return DTB_OK;
+}
+
+int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_spec* spec, const float* props,
+                                    double dt, uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device)
+{
+    DTB_REQUIRE(spec != nullptr && props != nullptr, "spec and props are required");
+    DTB_REQUIRE(spec->n >= 2, "fixed in-degree networks need n >= 2");
+    DTB_REQUIRE(spec->in_degree >= 0, "in_degree must be >= 0");
+    DTB_REQUIRE(spec->max_delay >= 1 && spec->max_delay <= DTB_MAX_DELAY, "max_delay must be in 1..64");
+    DTB_REQUIRE(spec->n_exc >= 0 && spec->n_exc <= spec->n, "n_exc must be in [0, n]");
+    DTB_REQUIRE(dt > 0.0, "dt must be > 0");
+    Block* b = nullptr;
+    DTB_TRY(begin(&b, device, spec->n));
+    int st = DTB_OK;
+    do {
+        b->dt = dt;
+        b->seed = noise_seed;
+        b->stats_excl = stats_exclude_blocked;
+        b->pair = spec->pair_weights != 0;
+        b->nnz = spec->n * (int64_t)spec->in_degree;
+        if ((st = make_pack(b, spec->max_delay))) break;
+        if ((st = alloc_state(b))) break;
+        if ((st = gen_fixed_indegree(b, spec))) break;
+        st = finish_create(b, props);
+    } while (0);
+    if (st) {
+        destroy(b);
+        return st;
+    }
+    *out = static_cast<dtb_block*>(b);
+    return DTB_OK;
+}
+
+int dtb_block_destroy(dtb_block* b)
+{
+    destroy(b);
+    return DTB_OK;
+}
+
+int dtb_block_run(dtb_block* b, int64_t nsteps, const dtb_run_args* args)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    return run_impl(b, nsteps, args);
+}
+
+int dtb_block_get(dtb_block* b, int32_t field, void* dst, int64_t bytes, int64_t* bytes_out)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    return get_impl(b, field, dst, bytes, bytes_out);
+}
+
+int dtb_block_info(dtb_block* b, int64_t* info)
+{
+    DTB_REQUIRE(b != nullptr && info != nullptr, "block or info is NULL");
+    info[0] = b->n;
+    info[1] = b->nnz;
+    info[2] = b->S;
+    info[3] = b->D;
+    info[4] = b->steps_done;
+    info[5] = b->ns;
+    info[6] = b->pair ? 1 : 0;
+    info[7] = b->device;
+    return DTB_OK;
+}
+
+int dtb_block_time(dtb_block* b, double* t)
+{
+    DTB_REQUIRE(b != nullptr && t != nullptr, "block or t is NULL");
+    *t = b->t_host;
+    return DTB_OK;
+}
+
+int dtb_block_set_props(dtb_block* b, int64_t m, const uint32_t* neuron, const uint32_t* col, const float* val)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    DTB_REQUIRE(m >= 0, "m must be >= 0");
+    if (m == 0) return DTB_OK;
+    DTB_REQUIRE(neuron && col && val, "neuron, col and val are required");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    std::vector<uint32_t> hn((size_t)m), hc((size_t)m);
+    DTB_TRY_HIP(hipMemcpy(hn.data(), neuron, 4 * (size_t)m, hipMemcpyDefault));
+    DTB_TRY_HIP(hipMemcpy(hc.data(), col, 4 * (size_t)m, hipMemcpyDefault));
+    for (int64_t r = 0; r < m; ++r) {
+        DTB_REQUIRE(hn[(size_t)r] < (uint64_t)b->n, "neuron %u out of range", hn[(size_t)r]);
+        DTB_REQUIRE(hc[(size_t)r] < DTB_NPROP && hc[(size_t)r] != 3, "property column %u not settable", hc[(size_t)r]);
+    }
+    void *dn, *dc, *dv;
+    DTB_TRY_HIP(hipMalloc(&dn, 4 * (size_t)m));
+    DTB_TRY_HIP(hipMalloc(&dc, 4 * (size_t)m));
+    DTB_TRY_HIP(hipMalloc(&dv, 4 * (size_t)m));
+    int st = upload(b, dn, hn.data(), 4 * (size_t)m);
+    if (!st) st = upload(b, dc, hc.data(), 4 * (size_t)m);
+    if (!st) st = upload(b, dv, val, 4 * (size_t)m);
+    if (!st) {
+        hipLaunchKernelGGL(set_props_kernel, dim3(grid_for(m, 256)), dim3(256), 0, b->stream, m, (const uint32_t*)dn,
+                           (const uint32_t*)dc, (const float*)dv, b->props);
+        st = derive_props(b);
+    }
+    hipStreamSynchronize(b->stream);
+    hipFree(dn);
+    hipFree(dc);
+    hipFree(dv);
+    return st;
+}
+
+int dtb_block_mul_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, const uint32_t* col, const float* val)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    DTB_REQUIRE(m >= 0, "m must be >= 0");
+    if (m == 0) return DTB_OK;
+    DTB_REQUIRE(pop && col && val, "pop, col and val are required");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    std::vector<int32_t> cols;
+    std::vector<float> table;
+    DTB_TRY(group_requests<float>(b, m, pop, col, val, NAN, cols, table));
+    void *dt, *dc;
+    DTB_TRY_HIP(hipMalloc(&dt, sizeof(float) * table.size()));
+    DTB_TRY_HIP(hipMalloc(&dc, sizeof(int32_t) * cols.size()));
+    int st = upload(b, dt, table.data(), sizeof(float) * table.size());
+    if (!st) st = upload(b, dc, cols.data(), sizeof(int32_t) * cols.size());
+    if (!st) {
+        hipLaunchKernelGGL(mul_props_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->S, b->sub,
+                           (const float*)dt, (const int32_t*)dc, (int32_t)cols.size(), b->props);
+        st = derive_props(b);
+    }
+    hipStreamSynchronize(b->stream);
+    hipFree(dt);
+    hipFree(dc);
+    return st;
+}
+
+int dtb_block_gamma_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, const uint32_t* col,
+                                 const float* alpha, const float* beta, uint64_t seed)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    DTB_REQUIRE(m >= 0, "m must be >= 0");
+    if (m == 0) return DTB_OK;
+    DTB_REQUIRE(pop && col && alpha && beta, "pop, col, alpha and beta are required");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    std::vector<float> ha((size_t)m), hb((size_t)m);
+    DTB_TRY_HIP(hipMemcpy(ha.data(), alpha, 4 * (size_t)m, hipMemcpyDefault));
+    DTB_TRY_HIP(hipMemcpy(hb.data(), beta, 4 * (size_t)m, hipMemcpyDefault));
+    std::vector<float2> ab((size_t)m);
+    for (int64_t r = 0; r < m; ++r) {
+        DTB_REQUIRE(ha[(size_t)r] >= 0.0f && hb[(size_t)r] > 0.0f, "gamma needs alpha >= 0 and beta > 0");
+        ab[(size_t)r] = make_float2(ha[(size_t)r], hb[(size_t)r]);
+    }
+    std::vector<int32_t> cols;
+    std::vector<float2> table;
+    DTB_TRY(group_requests<float2>(b, m, pop, col, ab.data(), make_float2(NAN, NAN), cols, table));
+    void *dt, *dc;
+    DTB_TRY_HIP(hipMalloc(&dt, sizeof(float2) * table.size()));
+    DTB_TRY_HIP(hipMalloc(&dc, sizeof(int32_t) * cols.size()));
+    int st = upload(b, dt, table.data(), sizeof(float2) * table.size());
+    if (!st) st = upload(b, dc, cols.data(), sizeof(int32_t) * cols.size());
+    if (!st) {
+        hipLaunchKernelGGL(gamma_props_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->S, b->sub,
+                           (const float2*)dt, (const int32_t*)dc, (int32_t)cols.size(), seed, b->props);
+        st = derive_props(b);
+    }
+    hipStreamSynchronize(b->stream);
+    hipFree(dt);
+    hipFree(dc);
+    return st;
+}
+
+int dtb_block_replace_props(dtb_block* b, const float* props)
+{
+    DTB_REQUIRE(b != nullptr && props != nullptr, "block or props is NULL");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    DTB_TRY(upload(b, b->props, props, sizeof(float) * (size_t)b->n * DTB_NPROP));
+    return derive_props(b);
+}
+
+int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    DTB_REQUIRE(m >= 0 && (m == 0 || idx), "bad sample list");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    std::vector<uint32_t> h((size_t)m);
+    if (m) DTB_TRY_HIP(hipMemcpy(h.data(), idx, 4 * (size_t)m, hipMemcpyDefault));
+    for (uint32_t x : h) DTB_REQUIRE(x < (uint64_t)b->n, "sample index %u out of range", x);
+    if (m > b->ns) {
+        // new buffers; the old ones stay owned by the block until destroy
+        DTB_TRY(alloc_n(b, &b->samples, m));
+        DTB_TRY(alloc_n(b, &b->rec_sspk, m * b->kmax));
+        DTB_TRY(alloc_n(b, &b->rec_sv, m * b->kmax));
+        for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+    }
+    b->ns = m;
+    if (m) DTB_TRY(upload(b, b->samples, h.data(), 4 * (size_t)m));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    return DTB_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,90 @@
+// Block state shared by the engine (dtb_block.hip) and the network
+// generator (dtb_gen.hip).
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "dtb_internal.hpp"
+
+namespace dtb {
+
+// Step bookkeeping kept on the device so captured graphs replay without
+// host arguments: nbase = steps done before the current batch.
+struct StepCtl {
+    int64_t nsteps;
+    int64_t nbase;
+    double t;
+    double pad;
+};
+
+struct Block {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0, nnz = 0;
+    int32_t S = 0;    // statistics sub-populations: max(sub_idx)+1
+    int32_t D = 1;    // ring depth = largest delay
+    bool pair = false; // float2 weights (else one weight per synapse)
+    SynPack pk{};
+    double dt = 0.1;
+    uint64_t seed = 0;
+    int stats_excl = 0;
+
+    // properties: master copy [n][22] and the derived per-neuron arrays
+    float* props = nullptr;
+    float4 *g = nullptr, *vu = nullptr, *decay = nullptr;
+    float *gL = nullptr, *vL = nullptr, *dtC = nullptr, *vth = nullptr, *vrst = nullptr, *iext = nullptr;
+    float *tref = nullptr, *noisep = nullptr;
+    int32_t* pop = nullptr; // statistics sub-population (-1: excluded)
+    int32_t* sub = nullptr; // raw sub_idx (property column 3)
+
+    // dynamic state
+    float* V = nullptr;
+    float4* Jp = nullptr;          // J decayed to the current step, arrivals of the current step not added
+    int32_t* ref_end = nullptr;    // first step that is not refractory
+    int32_t* last_step = nullptr;  // step of the last threshold spike (0: none)
+    double* t_last = nullptr;      // time of the last threshold spike
+    uint8_t* flags = nullptr;      // bit0 threshold spike, bit1 delivered spike (last step)
+    void* ring = nullptr;          // [D][n] float2 (E,I) or float4 (AMPA,NMDA,GABAa,GABAb)
+
+    // presynaptic-major CSR
+    int64_t* rowptr = nullptr;
+    uint32_t* col = nullptr; // packed (post, delay-1, kind)
+    float* w = nullptr;      // compact weights
+    float2* w2 = nullptr;    // pair weights
+
+    // spike lists, double-buffered by step parity
+    int32_t* spk = nullptr;     // [2][n]
+    uint32_t* spk_cnt = nullptr; // [2]
+
+    StepCtl* ctl = nullptr;
+    double* ttab = nullptr; // [kmax] time of each step of the batch
+    int64_t kmax = 0;
+
+    // per-step records of the current batch
+    uint32_t* rec_freq = nullptr; // [kmax][S]
+    float* rec_vsum = nullptr;    // [kmax][S]
+    float* rec_isum = nullptr;    // [kmax][S]
+    uint64_t* rec_events = nullptr; // [kmax]
+    uint32_t* rec_nspk = nullptr;   // [kmax]
+    uint8_t* rec_sspk = nullptr;    // [kmax][ns]
+    float* rec_sv = nullptr;        // [kmax][ns]
+    float* pop_inv = nullptr;       // [S] 1/size (NaN for empty)
+
+    uint32_t* samples = nullptr;
+    int64_t ns = 0;
+
+    std::vector<int64_t> pop_size;
+    int64_t steps_done = 0;
+    double t_host = 0.0;
+    int32_t deliver_grid = 64;
+
+    std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
+    std::vector<void*> allocs;
+};
+
+int block_alloc(Block* b, void** p, size_t bytes);
+int derive_props(Block* b);
+int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec);
+
+} // namespace dtb

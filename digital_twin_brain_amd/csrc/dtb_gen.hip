@@ -1,0 +1,282 @@
+// dtb_gen.hip -- seeded fixed in-degree networks built on the GPU, directly
+// in the presynaptic-major packed CSR the step kernels read.
+//
+// For post i and slot k, one Philox4x32-10 call (ctr = (k, i_lo, i_hi,
+// CONN), key = seed) gives r0..r3:
+//   pre   = floor(r0 (n-1) / 2^32), shifted past i (no self-connection)
+//   w0    = fmaf(w_hi - w_lo, u24(r1), w_lo)
+//   delay = 1 + floor(r2 D / 2^32)
+//   w1    = fmaf(w_hi - w_lo, u24(r3), w_lo)      (pair weights)
+//   kind  = pre >= n_exc
+// Passes: (1) out-degree histogram, (2) exclusive scan -> rowptr,
+// (3) scatter (post, k) keys into their rows, (4) per-row LDS bitonic sort of
+// the keys, then re-draw each synapse from its key and write the packed word
+// and weight.  Sorting makes the CSR independent of atomic ordering, so it is
+// bit-identical to the sequential generator of the oracle.
+#include <algorithm>
+#include <vector>
+
+#include "dtb_block.hpp"
+
+namespace dtb {
+
+struct GenSpec {
+    int64_t n, n_exc;
+    int32_t K, D;
+    float w_lo, span;
+    uint32_t k0, k1;
+};
+
+__device__ inline u32x4 conn_draw(const GenSpec& s, int64_t i, int32_t k)
+{
+    return philox4x32_10(u32x4{(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), kStreamConn}, s.k0, s.k1);
+}
+
+__device__ inline uint32_t conn_pre(const GenSpec& s, int64_t i, uint32_t r0)
+{
+    uint32_t pre = (uint32_t)(((uint64_t)r0 * (uint64_t)(s.n - 1)) >> 32);
+    if ((int64_t)pre >= i) pre += 1;
+    return pre;
+}
+
+__global__ void gen_count_kernel(GenSpec s, uint32_t* __restrict__ outdeg)
+{
+    const int64_t total = s.n * (int64_t)s.K;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+        const int64_t i = x / s.K;
+        const int32_t k = (int32_t)(x - i * s.K);
+        const u32x4 r = conn_draw(s, i, k);
+        atomicAdd(&outdeg[conn_pre(s, i, r.x)], 1u);
+    }
+}
+
+// Exclusive scan of uint32 counts into int64 offsets: per-tile sums, a
+// single-workgroup scan of the tile sums, then per-tile scans.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ inline int64_t block_exclusive_scan(int64_t v, int64_t* lds, int64_t* total)
+{
+    // Hillis-Steele over 256 lanes in LDS
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int64_t add = t >= off ? lds[t - off] : 0;
+        __syncthreads();
+        lds[t] += add;
+        __syncthreads();
+    }
+    const int64_t incl = lds[t];
+    *total = lds[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_sums_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                       int64_t* __restrict__ tile_sums)
+{
+    __shared__ int64_t lds[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    int64_t v = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t x = base + (int64_t)j * kScanThreads + threadIdx.x;
+        if (x < n) v += in[x];
+    }
+    int64_t tot;
+    block_exclusive_scan(v, lds, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_offsets_kernel(int64_t* __restrict__ tile_sums,
+                                                                          int64_t ntiles)
+{
+    __shared__ int64_t lds[kScanThreads];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < ntiles; base += kScanThreads) {
+        const int64_t x = base + threadIdx.x;
+        const int64_t v = x < ntiles ? tile_sums[x] : 0;
+        int64_t tot;
+        const int64_t ex = block_exclusive_scan(v, lds, &tot);
+        if (x < ntiles) tile_sums[x] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tiles_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                   const int64_t* __restrict__ tile_off,
+                                                                   int64_t* __restrict__ out)
+{
+    __shared__ int64_t lds[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    int64_t sum = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t x = base + j;
+        v[j] = x < n ? in[x] : 0u;
+        sum += v[j];
+    }
+    int64_t tot;
+    int64_t run = tile_off[blockIdx.x] + block_exclusive_scan(sum, lds, &tot);
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t x = base + j;
+        if (x < n) out[x] = run;
+        run += v[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tile_off[blockIdx.x] + tot;
+}
+
+__global__ void gen_fill_kernel(GenSpec s, unsigned long long* __restrict__ cursor, uint64_t* __restrict__ keys)
+{
+    const int64_t total = s.n * (int64_t)s.K;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+        const int64_t i = x / s.K;
+        const int32_t k = (int32_t)(x - i * s.K);
+        const u32x4 r = conn_draw(s, i, k);
+        const uint32_t pre = conn_pre(s, i, r.x);
+        const unsigned long long pos = atomicAdd(&cursor[pre], 1ull);
+        keys[pos] = ((uint64_t)i << 32) | (uint32_t)k;
+    }
+}
+
+// One workgroup per row: bitonic sort of the row's keys in LDS (rows up to
+// kMaxRow synapses), then each synapse is re-drawn from its key.
+constexpr int kSortThreads = 256;
+constexpr int kMaxRow = 8192;
+
+template <bool PAIR>
+__global__ __launch_bounds__(kSortThreads) void gen_sort_finalize_kernel(GenSpec s, SynPack pk,
+                                                                          const int64_t* __restrict__ rowptr,
+                                                                          const uint64_t* __restrict__ keys,
+                                                                          uint32_t* __restrict__ col,
+                                                                          float* __restrict__ w,
+                                                                          float2* __restrict__ w2,
+                                                                          uint32_t* __restrict__ too_long)
+{
+    __shared__ uint64_t lds[kMaxRow];
+    for (int64_t j = blockIdx.x; j < s.n; j += gridDim.x) {
+        const int64_t beg = rowptr[j];
+        const int32_t len = (int32_t)(rowptr[j + 1] - beg);
+        if (len > kMaxRow) {
+            if (threadIdx.x == 0) atomicAdd(too_long, 1u);
+            continue;
+        }
+        if (len == 0) continue;
+        int32_t P = 1;
+        while (P < len) P <<= 1;
+        for (int32_t x = threadIdx.x; x < P; x += kSortThreads) lds[x] = x < len ? keys[beg + x] : ~0ull;
+        __syncthreads();
+        for (int32_t size = 2; size <= P; size <<= 1) {
+            for (int32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int32_t t = threadIdx.x; t < (P >> 1); t += kSortThreads) {
+                    const int32_t lo = 2 * t - (t & (stride - 1));
+                    const int32_t hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    const uint64_t a = lds[lo], b = lds[hi];
+                    if ((a > b) == up) {
+                        lds[lo] = b;
+                        lds[hi] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const uint32_t kind = (j >= s.n_exc) ? 1u : 0u;
+        for (int32_t x = threadIdx.x; x < len; x += kSortThreads) {
+            const uint64_t key = lds[x];
+            const int64_t i = (int64_t)(key >> 32);
+            const int32_t k = (int32_t)(uint32_t)key;
+            const u32x4 r = conn_draw(s, i, k);
+            const uint32_t d = (uint32_t)(((uint64_t)r.z * (uint64_t)s.D) >> 32); // delay - 1
+            col[beg + x] = (uint32_t)i | (d << pk.idx_bits) | (kind << 31);
+            const float a0 = fmaf(s.span, u24(r.y), s.w_lo);
+            if (PAIR)
+                w2[beg + x] = make_float2(a0, fmaf(s.span, u24(r.w), s.w_lo));
+            else
+                w[beg + x] = a0;
+        }
+        __syncthreads();
+    }
+}
+
+static inline unsigned grid_of(int64_t work, int bs, int64_t cap)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, (work + bs - 1) / bs));
+}
+
+int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
+{
+    const int64_t n = spec->n;
+    const int64_t nnz = b->nnz;
+    GenSpec s{n, spec->n_exc, spec->in_degree, spec->max_delay, spec->w_lo, spec->w_hi - spec->w_lo,
+              (uint32_t)spec->seed, (uint32_t)(spec->seed >> 32)};
+    void* p = nullptr;
+    DTB_TRY(block_alloc(b, &p, sizeof(int64_t) * (size_t)(n + 1)));
+    b->rowptr = (int64_t*)p;
+    DTB_TRY(block_alloc(b, &p, sizeof(uint32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    b->col = (uint32_t*)p;
+    if (b->pair) {
+        DTB_TRY(block_alloc(b, &p, sizeof(float2) * (size_t)std::max<int64_t>(nnz, 1)));
+        b->w2 = (float2*)p;
+    } else {
+        DTB_TRY(block_alloc(b, &p, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)));
+        b->w = (float*)p;
+    }
+    uint32_t* outdeg = nullptr;
+    int64_t* tiles = nullptr;
+    unsigned long long* cursor = nullptr;
+    uint64_t* keys = nullptr;
+    uint32_t* flag = nullptr;
+    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    int st = DTB_OK;
+    auto fail = [&](hipError_t e, const char* what) {
+        set_error("%s failed: %s", what, hipGetErrorString(e));
+        st = e == hipErrorOutOfMemory ? DTB_OUT_OF_MEMORY : DTB_RUNTIME_ERROR;
+    };
+    hipError_t e;
+    do {
+        if ((e = hipMalloc(&outdeg, sizeof(uint32_t) * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+        if ((e = hipMalloc(&tiles, sizeof(int64_t) * (size_t)std::max<int64_t>(ntiles, 1))) != hipSuccess) { fail(e, "hipMalloc"); break; }
+        if ((e = hipMalloc(&cursor, sizeof(unsigned long long) * (size_t)n)) != hipSuccess) { fail(e, "hipMalloc"); break; }
+        if ((e = hipMalloc(&keys, sizeof(uint64_t) * (size_t)std::max<int64_t>(nnz, 1))) != hipSuccess) { fail(e, "hipMalloc(keys)"); break; }
+        if ((e = hipMalloc(&flag, sizeof(uint32_t))) != hipSuccess) { fail(e, "hipMalloc"); break; }
+        hipMemsetAsync(outdeg, 0, sizeof(uint32_t) * (size_t)n, b->stream);
+        hipMemsetAsync(flag, 0, sizeof(uint32_t), b->stream);
+        const unsigned gg = grid_of(nnz, 256, 65536);
+        hipLaunchKernelGGL(gen_count_kernel, dim3(gg), dim3(256), 0, b->stream, s, outdeg);
+        hipLaunchKernelGGL(scan_tile_sums_kernel, dim3((unsigned)ntiles), dim3(kScanThreads), 0, b->stream, outdeg, n,
+                           tiles);
+        hipLaunchKernelGGL(scan_tile_offsets_kernel, dim3(1), dim3(kScanThreads), 0, b->stream, tiles, ntiles);
+        hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)ntiles), dim3(kScanThreads), 0, b->stream, outdeg, n,
+                           tiles, b->rowptr);
+        hipMemcpyAsync(cursor, b->rowptr, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToDevice, b->stream);
+        hipLaunchKernelGGL(gen_fill_kernel, dim3(gg), dim3(256), 0, b->stream, s, cursor, keys);
+        const unsigned rg = grid_of(n, 1, 1 << 20);
+        if (b->pair)
+            hipLaunchKernelGGL(gen_sort_finalize_kernel<true>, dim3(rg), dim3(kSortThreads), 0, b->stream, s, b->pk,
+                               b->rowptr, keys, b->col, b->w, b->w2, flag);
+        else
+            hipLaunchKernelGGL(gen_sort_finalize_kernel<false>, dim3(rg), dim3(kSortThreads), 0, b->stream, s, b->pk,
+                               b->rowptr, keys, b->col, b->w, b->w2, flag);
+        if ((e = hipGetLastError()) != hipSuccess) { fail(e, "generator launch"); break; }
+        uint32_t nlong = 0;
+        hipMemcpyAsync(&nlong, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream);
+        if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) { fail(e, "generator"); break; }
+        if (nlong) {
+            set_error("%u presynaptic rows exceed %d synapses (generator row sort limit)", nlong, kMaxRow);
+            st = DTB_UNSUPPORTED;
+        }
+    } while (0);
+    if (outdeg) hipFree(outdeg);
+    if (tiles) hipFree(tiles);
+    if (cursor) hipFree(cursor);
+    if (keys) hipFree(keys);
+    if (flag) hipFree(flag);
+    return st;
+}
+
+} // namespace dtb

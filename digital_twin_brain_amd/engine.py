@@ -1,0 +1,178 @@
+"""NativeBlock: a Python handle over one libdtb block (include/dtb.h).
+
+Arrays passed in may be numpy arrays (host) or torch tensors (host or
+device); outputs are written into numpy arrays or torch tensors the caller
+provides, or into freshly allocated ones.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+
+
+def _as_np(x, dtype):
+    if x is None:
+        return None
+    if hasattr(x, "detach"):
+        x = x.detach().cpu().numpy()
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+class NativeBlock:
+    """One simulated network on one GPU (handle of dtb_block_*)."""
+
+    def __init__(self, handle, keep=None):
+        self._h = ctypes.c_void_p(handle)
+        self._keep = keep
+        info = self.info()
+        self.n, self.nnz, self.S, self.max_delay = info[0], info[1], info[2], info[3]
+        self.pair = bool(info[6])
+        self.device = int(info[7])
+
+    # ------------------------------------------------------------ creation
+    @classmethod
+    def from_csr(cls, props, rowptr, post, kind, w0, w1=None, delay=None, dt=0.1, noise_seed=0,
+                 stats_exclude_blocked=False, device=0):
+        """Network from presynaptic-major CSR (cuda/src/configuration.cpp:747-828)."""
+        nat.require_gpu()
+        props = _as_np(props, np.float32)
+        if props.ndim != 2 or props.shape[1] < nat.NPROP:
+            raise ValueError(f"props must be [n, {nat.NPROP}], got {props.shape}")
+        props = np.ascontiguousarray(props[:, :nat.NPROP])
+        n = props.shape[0]
+        rowptr = _as_np(rowptr, np.int64)
+        post = _as_np(post, np.int32)
+        kind = _as_np(kind, np.uint8)
+        w0 = _as_np(w0, np.float32)
+        w1 = _as_np(w1, np.float32)
+        delay = _as_np(delay, np.uint8)
+        if rowptr.shape != (n + 1,):
+            raise ValueError("rowptr must have n+1 entries")
+        nnz = int(post.shape[0])
+        for name, a in (("kind", kind), ("w0", w0), ("w1", w1), ("delay", delay)):
+            if a is not None and a.shape[0] != nnz:
+                raise ValueError(f"{name} must have nnz={nnz} entries")
+        d = nat.BlockDesc(n=n, nnz=nnz, props=nat.ptr(props), rowptr=nat.ptr(rowptr), post=nat.ptr(post),
+                          kind=nat.ptr(kind), delay=nat.ptr(delay), w0=nat.ptr(w0), w1=nat.ptr(w1), dt=float(dt),
+                          noise_seed=int(noise_seed), stats_exclude_blocked=int(bool(stats_exclude_blocked)),
+                          device=int(device))
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().dtb_block_create(ctypes.byref(h), ctypes.byref(d)))
+        return cls(h.value)
+
+    @classmethod
+    def fixed_indegree(cls, props, n_exc, in_degree, max_delay=1, w_lo=0.0, w_hi=1.0, pair_weights=False,
+                       conn_seed=1, dt=0.1, noise_seed=3, stats_exclude_blocked=False, device=0):
+        """Seeded fixed in-degree network generated on the GPU."""
+        nat.require_gpu()
+        props = _as_np(props, np.float32)
+        props = np.ascontiguousarray(props[:, :nat.NPROP])
+        spec = nat.FixedIndegreeSpec(n=props.shape[0], n_exc=int(n_exc), in_degree=int(in_degree),
+                                     max_delay=int(max_delay), w_lo=float(w_lo), w_hi=float(w_hi),
+                                     pair_weights=int(bool(pair_weights)), seed=int(conn_seed))
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().dtb_block_create_fixed_indegree(ctypes.byref(h), ctypes.byref(spec), nat.ptr(props),
+                                                            float(dt), int(noise_seed),
+                                                            int(bool(stats_exclude_blocked)), int(device)))
+        return cls(h.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            nat.lib().dtb_block_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- info
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        nat.check(nat.lib().dtb_block_info(self._h, nat.ptr(out)))
+        return [int(x) for x in out]
+
+    @property
+    def steps(self):
+        return self.info()[4]
+
+    @property
+    def t(self):
+        out = np.zeros(1, dtype=np.float64)
+        nat.check(nat.lib().dtb_block_time(self._h, nat.ptr(out)))
+        return float(out[0])
+
+    # ----------------------------------------------------------------- run
+    def run(self, nsteps, noise_mode=nat.NOISE_SCALAR, noise_rate=0.0, uniforms=None, isolated=False, graph=True,
+            freqs=None, vmeans=None, imeans=None, sample_spike=None, sample_v=None, events=None, spikes=None):
+        """Advance nsteps; output buffers (numpy or torch, host or device) are filled in place."""
+        if uniforms is not None and isinstance(uniforms, np.ndarray):
+            uniforms = np.ascontiguousarray(uniforms, dtype=np.float32)
+        a = nat.RunArgs(noise_mode=int(noise_mode), noise_rate=float(noise_rate), uniforms=nat.ptr(uniforms),
+                        isolated=int(bool(isolated)), use_graph=int(bool(graph)), freqs=nat.ptr(freqs),
+                        vmeans=nat.ptr(vmeans), imeans=nat.ptr(imeans), sample_spike=nat.ptr(sample_spike),
+                        sample_v=nat.ptr(sample_v), events=nat.ptr(events), spikes=nat.ptr(spikes))
+        nat.check(nat.lib().dtb_block_run(self._h, int(nsteps), ctypes.byref(a)))
+
+    # ----------------------------------------------------------------- get
+    def get(self, name, out=None):
+        f = nat.FIELD[name]
+        n, nnz, S = self.n, self.nnz, self.S
+        shapes = dict(V=((n,), np.float32), J=((4, n), np.float32), I_ui=((4, n), np.float32),
+                      I_syn=((n,), np.float32), active=((n,), np.uint8), spike=((n,), np.uint8),
+                      props=((n, nat.NPROP), np.float32), rowptr=((n + 1,), np.int64), post=((nnz,), np.int32),
+                      kind=((nnz,), np.uint8), delay=((nnz,), np.uint8), w0=((nnz,), np.float32),
+                      w1=((nnz,), np.float32), t_last=((n,), np.float64), pop_sizes=((S,), np.int64))
+        if name == "spike_list":
+            nb = ctypes.c_int64(0)
+            st = nat.lib().dtb_block_get(self._h, f, None, 0, ctypes.byref(nb))
+            if st not in (nat.DTB_OK, nat.DTB_INVALID_PARAMETER):
+                nat.check(st)
+            buf = np.zeros(max(nb.value // 4, 0), dtype=np.int32)
+            if buf.size:
+                nat.check(nat.lib().dtb_block_get(self._h, f, nat.ptr(buf), buf.nbytes, None))
+            return buf
+        shape, dt = shapes[name]
+        if out is None:
+            out = np.zeros(shape, dtype=dt)
+        nbytes = out.nbytes if isinstance(out, np.ndarray) else out.numel() * out.element_size()
+        nat.check(nat.lib().dtb_block_get(self._h, f, nat.ptr(out), int(nbytes), None))
+        return out
+
+    # ---------------------------------------------------------- properties
+    def set_props(self, neuron, col, val):
+        neuron = _as_np(neuron, np.uint32)
+        col = _as_np(col, np.uint32)
+        val = _as_np(val, np.float32)
+        nat.check(nat.lib().dtb_block_set_props(self._h, int(neuron.shape[0]), nat.ptr(neuron), nat.ptr(col),
+                                                nat.ptr(val)))
+
+    def mul_props_by_pop(self, pop, col, val):
+        pop = _as_np(pop, np.uint32)
+        col = _as_np(col, np.uint32)
+        val = _as_np(val, np.float32)
+        nat.check(nat.lib().dtb_block_mul_props_by_pop(self._h, int(pop.shape[0]), nat.ptr(pop), nat.ptr(col),
+                                                       nat.ptr(val)))
+
+    def gamma_props_by_pop(self, pop, col, alpha, beta, seed=0):
+        pop = _as_np(pop, np.uint32)
+        col = _as_np(col, np.uint32)
+        alpha = _as_np(alpha, np.float32)
+        beta = _as_np(beta, np.float32)
+        nat.check(nat.lib().dtb_block_gamma_props_by_pop(self._h, int(pop.shape[0]), nat.ptr(pop), nat.ptr(col),
+                                                         nat.ptr(alpha), nat.ptr(beta), int(seed)))
+
+    def replace_props(self, props):
+        props = np.ascontiguousarray(_as_np(props, np.float32)[:, :nat.NPROP])
+        if props.shape != (self.n, nat.NPROP):
+            raise ValueError(f"props must be [{self.n}, {nat.NPROP}]")
+        nat.check(nat.lib().dtb_block_replace_props(self._h, nat.ptr(props)))
+
+    def set_samples(self, idx):
+        idx = _as_np(idx, np.uint32)
+        nat.check(nat.lib().dtb_block_set_samples(self._h, int(idx.shape[0]), nat.ptr(idx)))
+        self.n_samples = int(idx.shape[0])

@@ -1,0 +1,189 @@
+/*
+ * dtb.h -- C ABI of the MI355X Digital Twin Brain network-step library
+ * (libdtb.so, built from digital_twin_brain_amd/csrc/*.hip for gfx950).
+ *
+ * Plain C types only: pointers, sizes and POD structs.  Every buffer
+ * argument may be host memory or device memory (the library copies with
+ * hipMemcpyDefault); device pointers avoid the PCIe copy.  Calls on one
+ * block handle are not re-entrant; separate handles are independent.
+ * No entry point exits the process: each returns a status code and leaves a
+ * thread-local message for dtb_last_error().
+ */
+#ifndef DTB_H_
+#define DTB_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  0..3 keep the values of SnnStatus
+ * (cuda/protos/snn.proto:36-41); 4 and 5 are new. */
+#define DTB_OK 0
+#define DTB_INVALID_PARAMETER 1
+#define DTB_UNSET_SAMPLE_FAILURE 2
+#define DTB_UNSUPPORTED 3
+#define DTB_RUNTIME_ERROR 4
+#define DTB_OUT_OF_MEMORY 5
+
+/* Number of property columns per neuron: noise_rate, blocked_in_stat, I_ext,
+ * sub_idx, C, T_ref, g_L, V_L, V_th, V_reset, g_ui[4], V_ui[4], tau_ui[4]
+ * (generation/make_block.py:394-434, models/block.py:156). */
+#define DTB_NPROP 22
+
+/* Largest synaptic delay in steps. */
+#define DTB_MAX_DELAY 64
+
+/* Thread-local message of the last failing call (replaces the reference's
+ * exit() on error, cuda/src/common.cpp:199-209). */
+const char* dtb_last_error(void);
+int32_t dtb_abi_version(void);
+
+typedef struct dtb_block dtb_block;
+
+/* A network given as presynaptic-major CSR: row j lists the synapses of
+ * presynaptic neuron j (cuda/src/configuration.cpp:747-828).  kind 0 feeds
+ * the AMPA/NMDA pair with (w0, w1), kind 1 the GABAa/GABAb pair
+ * (generation/read_block.py:68-74).  w1 == NULL stores one weight per
+ * synapse shared by the pair ("compact", 8 B/synapse). */
+typedef struct {
+    int64_t n;                     /* neurons */
+    int64_t nnz;                   /* synapses */
+    const float* props;            /* [n][22] */
+    const int64_t* rowptr;         /* [n+1] */
+    const int32_t* post;           /* [nnz] postsynaptic neuron */
+    const uint8_t* kind;           /* [nnz] 0 or 1 */
+    const uint8_t* delay;          /* [nnz] steps in 1..DTB_MAX_DELAY, NULL = all 1 */
+    const float* w0;               /* [nnz] */
+    const float* w1;               /* [nnz] or NULL */
+    double dt;                     /* ms */
+    uint64_t noise_seed;           /* Philox key of the background noise */
+    int32_t stats_exclude_blocked; /* 1: property column 1 != 0 excludes a neuron from statistics */
+    int32_t device;
+} dtb_block_desc;
+
+/* block(node_property, w_uij, delta_t) -- models/block.py:30-60, with the
+ * connectivity already converted to CSR; also the engine init of
+ * cuda/src/block.cpp:37-193 (parse_conn_table + parse_params). */
+int dtb_block_create(dtb_block** out, const dtb_block_desc* desc);
+
+/* Seeded fixed in-degree network generated on the GPU (the synthetic
+ * counterpart of generation/make_block.py:206-323): for post i and slot k
+ * one Philox4x32-10 call keyed by seed gives the presynaptic neuron
+ * (uniform, no self-connection), w0, the delay (uniform 1..max_delay) and
+ * w1.  kind = (pre >= n_exc).  Rows are sorted by (post, k). */
+typedef struct {
+    int64_t n;
+    int64_t n_exc;
+    int32_t in_degree;
+    int32_t max_delay;
+    float w_lo, w_hi;
+    int32_t pair_weights; /* 0: compact (one weight per synapse) */
+    uint64_t seed;
+} dtb_fixed_indegree_spec;
+
+int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_spec* spec, const float* props,
+                                    double dt, uint64_t noise_seed, int32_t stats_exclude_blocked,
+                                    int32_t device);
+
+int dtb_block_destroy(dtb_block* b);
+
+/* Noise modes of dtb_block_run. */
+#define DTB_NOISE_SCALAR 0    /* Bernoulli(noise_rate) per neuron (block.run, models/block.py:136) */
+#define DTB_NOISE_PER_NEURON 1 /* Bernoulli(property column 0) (cuda/src/stage/spike.cpp:156-157) */
+#define DTB_NOISE_UNIFORMS 2  /* spike iff uniforms[step][i] < noise_rate (injected noise) */
+
+typedef struct {
+    int32_t noise_mode;
+    float noise_rate;
+    const float* uniforms; /* DTB_NOISE_UNIFORMS: [nsteps][n] */
+    int32_t isolated;      /* 1: only noise spikes reach synapses (models/block.py:137-138) */
+    int32_t use_graph;     /* 1: replay captured hipGraphs of whole step batches */
+    /* per-step outputs, NULL = not wanted (host or device memory) */
+    uint32_t* freqs;       /* [nsteps][S] threshold spikes per sub-population */
+    float* vmeans;         /* [nsteps][S] mean V per sub-population */
+    float* imeans;         /* [nsteps][S] mean I_syn per sub-population */
+    uint8_t* sample_spike; /* [nsteps][n_samples] delivered spike of sampled neurons */
+    float* sample_v;       /* [nsteps][n_samples] V of sampled neurons */
+    uint64_t* events;      /* [nsteps] synaptic events delivered (sum of out-degrees of spiking rows) */
+    uint32_t* spikes;      /* [nsteps] spiking rows delivered */
+} dtb_run_args;
+
+/* nsteps of block.run (models/block.py:116-150) / BrainBlock's step
+ * sequence (cuda/tools/dist_simulator.cpp:3822-4114); returns when the
+ * outputs are written. */
+int dtb_block_run(dtb_block* b, int64_t nsteps, const dtb_run_args* args);
+
+/* State fields for dtb_block_get. */
+#define DTB_FIELD_V 0          /* float[n] */
+#define DTB_FIELD_J 1          /* float[4][n] J_ui */
+#define DTB_FIELD_I_UI 2       /* float[4][n] */
+#define DTB_FIELD_I_SYN 3      /* float[n] */
+#define DTB_FIELD_ACTIVE 4     /* uint8[n] threshold spikes of the last step */
+#define DTB_FIELD_SPIKE 5      /* uint8[n] delivered spikes of the last step */
+#define DTB_FIELD_SPIKE_LIST 6 /* int32[count] delivered spike list of the last step, ascending */
+#define DTB_FIELD_PROPS 7      /* float[n][22] */
+#define DTB_FIELD_ROWPTR 8     /* int64[n+1] */
+#define DTB_FIELD_POST 9       /* int32[nnz] */
+#define DTB_FIELD_KIND 10      /* uint8[nnz] */
+#define DTB_FIELD_DELAY 11     /* uint8[nnz] */
+#define DTB_FIELD_W0 12        /* float[nnz] */
+#define DTB_FIELD_W1 13        /* float[nnz] (== W0 for compact weights) */
+#define DTB_FIELD_T_LAST 14    /* double[n] time of the last threshold spike */
+#define DTB_FIELD_POP_SIZES 15 /* int64[S] neurons per sub-population counted in statistics */
+
+/* Copies a field (block.V_i, .J_ui, .I_ui, .I_syn, .active, models/block.py)
+ * into dst; bytes must be at least the field size.  For SPIKE_LIST the
+ * byte count of the list is returned through *bytes_out. */
+int dtb_block_get(dtb_block* b, int32_t field, void* dst, int64_t bytes, int64_t* bytes_out);
+
+/* info[0..7] = n, nnz, S (sub-populations), max delay, steps done,
+ * n_samples, pair weights (0/1), device. */
+int dtb_block_info(dtb_block* b, int64_t* info);
+/* Simulated time t (models/block.py:58,133), accumulated in double. */
+int dtb_block_time(dtb_block* b, double* t);
+
+/* Point-wise property assignment, cuda/src/stage/property.cpp:17-120:
+ * props[neuron[i]][col[i]] = val[i] (col is a property column 0..21). */
+int dtb_block_set_props(dtb_block* b, int64_t m, const uint32_t* neuron, const uint32_t* col, const float* val);
+
+/* Multiply a property column over every neuron of a sub-population,
+ * cuda/src/stage/property.cpp:145-256 (mul_property_by_subblk). */
+int dtb_block_mul_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, const uint32_t* col,
+                               const float* val);
+
+/* Draw a property column from Gamma(alpha, beta) (shape, rate) for every
+ * neuron of a sub-population, cuda/src/stage/property.cpp:347-471
+ * (Marsaglia-Tsang on Philox4x32-10 keyed by (seed, neuron)). */
+int dtb_block_gamma_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, const uint32_t* col,
+                                 const float* alpha, const float* beta, uint64_t seed);
+
+/* Replace every property (block.update_property, models/block.py:152-170). */
+int dtb_block_replace_props(dtb_block* b, const float* props);
+
+/* Sampled neurons for sample_spike / sample_v (BlockWrapper.set_samples,
+ * cuda/python/dist_blockwrapper_pytorch.py:505-528). */
+int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx);
+
+/* Balloon-Windkessel forward-Euler step on device arrays of n voxels,
+ * models/bold_model_pytorch.py:43-78.  par = {epsilon, tao_s, tao_f, tao_0,
+ * alpha, E_0, V_0, delta_t}.  stream: hipStream_t or NULL. */
+int dtb_bold_step_f32(int64_t n, const double* par, float* s, float* q, float* v, float* f, const float* u,
+                      float* out, void* stream);
+int dtb_bold_step_f64(int64_t n, const double* par, double* s, double* q, double* v, double* f,
+                      const double* u, double* out, void* stream);
+
+/* Diffusion EnKF analysis in FP64 on device arrays,
+ * data_assimilation/DataAssimilation.py:40-60.  w: [E][V][S] updated in
+ * place (S = hyper-parameters + 6 hemodynamic states, last = BOLD);
+ * y: [V] observation; xi: [E][V] standard-normal draws. */
+int dtb_enkf_analysis_f64(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y, double rho,
+                          const double* xi, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTB_H_ */

@@ -136,11 +136,12 @@ def run_bold_case(ref_bold, name, nv, steps, seed):
     # state_update from an ensemble-shaped hidden state (DataAssimilation.py:400)
     w = rng.uniform(-0.5, 1.5, size=(3, nv // 3, 4))
     w[0, 0, 1] = -1.0
+    w_saved = w.copy()  # state_update keeps a view of w[:, :, 0] as s, which run() then updates in place
     bold.state_update(torch.from_numpy(w))
     su = np.stack([bold.s.numpy(), bold.q.numpy(), bold.v.numpy(), bold.f_in.numpy()])
     o2 = bold.run(torch.from_numpy(u[0][: (nv // 3) * 3]))
     np.savez_compressed(os.path.join(HERE, f"bold_{name}.npz"), u=u, out=np.stack(outs), state=np.stack(states),
-                        su_w=w, su_state=su, su_out=o2.numpy(), **{k: float(v) for k, v in params.items()})
+                        su_w=w_saved, su_state=su, su_out=o2.numpy(), **{k: float(v) for k, v in params.items()})
     print(f"bold {name}: nv={nv} steps={steps}")
 
 

@@ -1,0 +1,317 @@
+"""GPU parity of the block step (libdtb through the C ABI) against the
+reference golden vectors and the CPU oracle.
+
+Tolerances (FP32 GPU state against an FP64 reference):
+- threshold/delivered rasters identical until the first threshold-crossing
+  divergence, which is reported; the first divergence must come late;
+- V within 1e-5 relative and J_ui within 1e-5 of max|J| before it;
+- CSR indexing, spike compaction and spike counts bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _load(golden_dir, name):
+    z = np.load(os.path.join(golden_dir, f"block_{name}.npz"))
+    return {k: z[k] for k in z.files}
+
+
+def _native_from_case(c, seed):
+    from digital_twin_brain_amd import NativeBlock
+
+    w1 = c["w1"] if not np.array_equal(c["w0"], c["w1"]) else None
+    return NativeBlock.from_csr(c["props"], c["rowptr"], c["post"], c["kind"], c["w0"], w1, dt=float(c["dt"]),
+                                noise_seed=seed)
+
+
+@pytest.mark.parametrize("name,min_agree", [("dense64", 150), ("isolated64", 200), ("sparse400", 500)])
+def test_block_matches_reference_golden(golden_dir, name, min_agree):
+    c = _load(golden_dir, name)
+    n, steps, keep = int(c["n"]), int(c["steps"]), int(c["keep_every"])
+    nb = _native_from_case(c, int(c["seed"]))
+    act = np.unpackbits(c["active"], axis=1)[:, :n]
+    spk = np.unpackbits(c["spike"], axis=1)[:, :n]
+    snaps = set(int(s) for s in c["snap_steps"])
+    freqs = np.zeros(nb.S, np.uint32)
+    vmeans = np.zeros(nb.S, np.float32)
+    diverged = None
+    kv = 0
+    for s in range(1, steps + 1):
+        nb.run(1, noise_mode=0, noise_rate=float(c["noise_rate"]), isolated=bool(c["isolated"]), graph=False,
+               freqs=freqs, vmeans=vmeans)
+        a = nb.get("active")
+        if not np.array_equal(a, act[s - 1]):
+            diverged = s
+            break
+        assert np.array_equal(nb.get("spike"), spk[s - 1]), f"delivered raster differs at step {s}"
+        assert np.array_equal(freqs.astype(np.float64), c["sum_act"][s - 1])
+        np.testing.assert_allclose(vmeans, c["mean_v"][s - 1], rtol=RTOL)
+        if s % keep == 0:
+            np.testing.assert_allclose(nb.get("V"), c["V"][kv], rtol=RTOL, atol=0)
+            kv += 1
+        if s in snaps:
+            Jg = c[f"J_{s}"]
+            np.testing.assert_allclose(nb.get("J"), Jg, rtol=RTOL, atol=RTOL * (np.abs(Jg).max() + 1e-30))
+            Ig = c[f"I_ui_{s}"]
+            np.testing.assert_allclose(nb.get("I_ui"), Ig, rtol=RTOL, atol=RTOL * (np.abs(Ig).max() + 1e-30))
+            Isg = c[f"I_syn_{s}"]
+            np.testing.assert_allclose(nb.get("I_syn"), Isg, rtol=RTOL, atol=RTOL * (np.abs(Isg).max() + 1e-30))
+    print(f"{name}: first threshold-raster divergence at step {diverged} of {steps}")
+    assert diverged is None or diverged > min_agree, f"raster diverged at step {diverged}"
+    assert abs(nb.t - float(c["t_final"])) == 0.0 or diverged is not None
+
+
+def test_dropin_block_class_matches_golden(golden_dir):
+    """digital_twin_brain_amd.block with the reference constructor and run()."""
+    from digital_twin_brain_amd import block
+    from tests.golden.make_golden import csr_to_torch_sparse
+
+    c = _load(golden_dir, "dense64")
+    n = int(c["n"])
+    W = csr_to_torch_sparse(n, dict(rowptr=c["rowptr"], post=c["post"], kind=c["kind"], w0=c["w0"], w1=c["w1"]))
+    B = block(node_property=torch.from_numpy(c["props"]), w_uij=W.to_dense(), delta_t=float(c["dt"]),
+              seed=int(c["seed"]))
+    act = np.unpackbits(c["active"], axis=1)[:, :n]
+    for s in range(1, 101):
+        sa, mv = B.run(noise_rate=float(c["noise_rate"]), isolated=False)
+        assert np.array_equal(B.active.numpy().astype(np.uint8), act[s - 1])
+        assert np.array_equal(sa.numpy().astype(np.float64), c["sum_act"][s - 1])
+        np.testing.assert_allclose(B.V_i.numpy(), c["V"][s - 1], rtol=RTOL)
+    J = B.J_ui.numpy()
+    assert J.shape == (4, n) and B.I_ui.shape == (4, n) and B.I_syn.shape == (n,)
+    assert B.sub_idx.shape == (n,) and float(B.t) == pytest.approx(100 * float(c["dt"]))
+
+
+@pytest.mark.parametrize("pair,D", [(False, 1), (True, 1), (False, 5), (True, 7)])
+def test_gpu_generator_csr_bit_exact(pair, D):
+    from digital_twin_brain_amd import NativeBlock
+
+    n, ne, K = 3000, 2400, 60
+    props = np.zeros((n, 22), np.float32)
+    props[:, 4] = 1; props[:, 5] = 2; props[:, 6] = 0.03; props[:, 7] = -75; props[:, 8] = -50; props[:, 9] = -65
+    props[:, 18:22] = (2, 40, 10, 50)
+    nb = NativeBlock.fixed_indegree(props, ne, K, max_delay=D, w_lo=0.0, w_hi=1.0, pair_weights=pair, conn_seed=77)
+    ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, pair, seed=77)
+    assert np.array_equal(nb.get("rowptr"), ref["rowptr"])
+    assert np.array_equal(nb.get("post"), ref["post"])
+    assert np.array_equal(nb.get("kind"), ref["kind"])
+    assert np.array_equal(nb.get("delay"), ref["delay"])
+    assert np.array_equal(nb.get("w0"), ref["w0"])
+    assert np.array_equal(nb.get("w1"), ref["w1"] if pair else ref["w0"])
+
+
+def _c1_like_props(n, n_exc, rng, g_scale=3.0):
+    p = np.zeros((n, 22), np.float32)
+    p[:, 3] = (np.arange(n) >= n_exc).astype(np.float32)
+    p[:, 4] = 1; p[:, 5] = 2; p[:, 6] = 0.03; p[:, 7] = -75; p[:, 8] = -50; p[:, 9] = -65
+    g_cal = np.array([2.13037975e-02, 1.39240506e-04, 1.58227848e-01, 1.89873418e-02], np.float32)
+    p[:, 10:14] = g_cal * g_scale * rng.uniform(0.8, 1.2, size=(n, 4)).astype(np.float32)
+    p[:, 14:18] = (0, 0, -70, -100)
+    p[:, 18:22] = (2, 40, 10, 50)
+    return p
+
+
+def test_compaction_and_delivery_bit_exact_on_fixed_spike_vector():
+    """Isolated step with injected uniforms: the spiking set is fixed, so the
+    compacted list and (dyadic weights) the delivered J are exact."""
+    from digital_twin_brain_amd import NativeBlock
+
+    n, ne, K = 5000, 4000, 40
+    rng = np.random.default_rng(5)
+    ref = orc.gen_fixed_indegree(n, ne, K, 1, 0.0, 1.0, True, seed=12)
+    for k in ("w0", "w1"):
+        ref[k] = (np.floor(ref[k] * 256) / 256).astype(np.float32)
+    props = _c1_like_props(n, ne, rng)
+    nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], ref["w1"], dt=0.1)
+    ob = orc.OracleBlock(props, ref, 0.1)
+    for s in range(1, 6):
+        u = np.ones(n, np.float32)
+        chosen = rng.choice(n, size=50 + 100 * s, replace=False)
+        u[chosen] = 0.0
+        nb.run(1, noise_mode=2, noise_rate=0.5, uniforms=u, isolated=True, graph=False)
+        ob.run(noise_rate=0.5, uniforms=u, isolated=True)
+        got = nb.get("spike_list")
+        assert np.array_equal(got, orc.compact(ob.get("spike")))
+        assert np.array_equal(np.sort(chosen).astype(np.int32), got)
+    # same-step delivery of dyadic weights: the ring sum is exact; J differs
+    # from the FP64 oracle only by the FP32 decay of earlier steps
+    Jg, Jo = nb.get("J"), ob.get("J")
+    np.testing.assert_allclose(Jg, Jo, rtol=1e-6, atol=1e-6 * np.abs(Jo).max())
+
+
+@pytest.mark.parametrize("pair,D,graph", [(False, 1, True), (True, 1, False), (False, 6, True), (True, 4, True)])
+def test_network_run_matches_oracle(pair, D, graph):
+    """C1-like recurrent network (fixed in-degree, delays 1..D) against the
+    FP64 push oracle on the same CSR and the same Philox noise."""
+    from digital_twin_brain_amd import NativeBlock
+
+    n, ne, K, steps = 4000, 3200, 100, 600
+    rng = np.random.default_rng(11)
+    props = _c1_like_props(n, ne, rng)
+    props[:, 3] = np.arange(n) // 500  # 8 sub-populations
+    nb = NativeBlock.fixed_indegree(props, ne, K, max_delay=D, pair_weights=pair, conn_seed=3, noise_seed=9)
+    csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
+               w0=nb.get("w0"), w1=nb.get("w1") if pair else None)
+    ob = orc.OracleBlock(props, csr, 0.1)
+    rate = 2.0 ** -7
+    B = 50
+    freqs = np.zeros((B, nb.S), np.uint32)
+    vmeans = np.zeros((B, nb.S), np.float32)
+    events = np.zeros(B, np.uint64)
+    nspk = np.zeros(B, np.uint32)
+    diverged = None
+    total_spikes = 0
+    for b0 in range(0, steps, B):
+        nb.run(B, noise_mode=0, noise_rate=rate, graph=graph, freqs=freqs, vmeans=vmeans, events=events,
+               spikes=nspk)
+        oa = []
+        for k in range(B):
+            sa, mv = ob.run(noise_rate=rate, seed=9)
+            oa.append((sa, mv))
+            if k < B - 1:
+                continue
+        # per-step records of the batch against the oracle (bit-exact counts)
+        for k in range(B):
+            if diverged is None and not np.array_equal(freqs[k].astype(np.float64), oa[k][0]):
+                diverged = b0 + k + 1
+        if diverged is not None:
+            break
+        for k in range(B):
+            np.testing.assert_allclose(vmeans[k], oa[k][1], rtol=RTOL)
+        np.testing.assert_allclose(nb.get("V"), ob.get("V"), rtol=RTOL)
+        Jo = ob.get("J")
+        np.testing.assert_allclose(nb.get("J"), Jo, rtol=RTOL, atol=RTOL * np.abs(Jo).max())
+        sl = nb.get("spike_list")
+        assert np.array_equal(sl, orc.compact(ob.get("spike")))
+        # events = sum of out-degrees of the delivered rows
+        rp = csr["rowptr"]
+        assert int(events[-1]) == int((rp[sl + 1] - rp[sl]).sum())
+        assert int(nspk[-1]) == sl.shape[0]
+        total_spikes += int(nspk.sum())
+    print(f"pair={pair} D={D}: first divergence {diverged}, delivered spikes {total_spikes}")
+    assert total_spikes > 0
+    assert diverged is None or diverged > 300
+
+
+def test_graph_and_direct_launch_agree_bitwise():
+    from digital_twin_brain_amd import NativeBlock
+
+    n, ne, K = 3000, 2400, 50
+    rng = np.random.default_rng(2)
+    props = _c1_like_props(n, ne, rng)
+    ref = orc.gen_fixed_indegree(n, ne, K, 3, 0.0, 1.0, False, seed=4)
+    ref["w0"] = (np.floor(ref["w0"] * 256) / 256).astype(np.float32)
+    a = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"], dt=0.1,
+                             noise_seed=1)
+    b = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"], dt=0.1,
+                             noise_seed=1)
+    fa = np.zeros((400, a.S), np.uint32)
+    fb = np.zeros((400, b.S), np.uint32)
+    a.run(400, noise_rate=0.01, graph=True, freqs=fa)
+    for k in range(4):
+        b.run(100, noise_rate=0.01, graph=False, freqs=fb[k * 100:(k + 1) * 100])
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(a.get("V"), b.get("V"))
+    assert np.array_equal(a.get("J"), b.get("J"))
+    assert fa.sum() > 0
+
+
+def test_property_edits_match_reference_semantics():
+    from digital_twin_brain_amd import NativeBlock
+
+    n = 1000
+    rng = np.random.default_rng(3)
+    props = _c1_like_props(n, 800, rng)
+    props[:, 3] = np.arange(n) // 250
+    csr = orc.gen_fixed_indegree(n, 800, 20, 1, 0.0, 1.0, False, seed=5)
+    nb = NativeBlock.from_csr(props, csr["rowptr"], csr["post"], csr["kind"], csr["w0"], None, dt=0.1)
+    # mul_property_by_subblk: column 10 (g_AMPA) of sub-population 2 times 1.5, column 4 (C) of 0 times 2
+    nb.mul_props_by_pop([2, 0], [10, 4], [1.5, 2.0])
+    want = props.copy()
+    want[500:750, 10] *= np.float32(1.5)
+    want[0:250, 4] *= np.float32(2.0)
+    assert np.array_equal(nb.get("props"), want)
+    nb.set_props([7, 999], [9, 0], [-60.0, 0.25])
+    want[7, 9] = -60.0
+    want[999, 0] = 0.25
+    assert np.array_equal(nb.get("props"), want)
+    # gamma(alpha, beta) draws: mean alpha/beta, variance alpha/beta^2
+    big = 200000
+    p2 = np.tile(props[:1], (big, 1))
+    p2[:, 3] = np.arange(big) // (big // 2)
+    c2 = dict(rowptr=np.zeros(big + 1, np.int64), post=np.zeros(0, np.int32), kind=np.zeros(0, np.uint8),
+              w0=np.zeros(0, np.float32))
+    nb2 = NativeBlock.from_csr(p2, c2["rowptr"], c2["post"], c2["kind"], c2["w0"], None, dt=0.1)
+    nb2.gamma_props_by_pop([0, 1], [10, 12], [5.0, 0.5], [5.0, 2.0], seed=17)
+    g = nb2.get("props")
+    a0 = g[: big // 2, 10]
+    a1 = g[big // 2:, 12]
+    assert abs(a0.mean() - 1.0) < 0.01 and abs(a0.var() - 0.2) < 0.01
+    assert abs(a1.mean() - 0.25) < 0.005 and abs(a1.var() - 0.125) < 0.006
+    assert np.array_equal(g[big // 2:, 10], p2[big // 2:, 10])  # untouched
+
+
+def test_bold_kernel_matches_reference_golden(golden_dir):
+    from digital_twin_brain_amd import BOLD
+
+    z = np.load(os.path.join(golden_dir, "bold_v18.npz"))
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
+        b = BOLD(epsilon=200, tao_s=0.8, tao_f=0.4, tao_0=1, alpha=0.2, E_0=0.8, V_0=0.02)
+        k = 0
+        for s in range(z["u"].shape[0]):
+            out = b.run(torch.from_numpy(z["u"][s]).to(dtype).cuda())
+            np.testing.assert_allclose(out.cpu().numpy(), z["out"][s], rtol=tol, atol=tol * 0.05)
+            if (s + 1) % 10 == 0:
+                st = torch.stack([b.s, b.q, b.v, b.f_in]).cpu().numpy()
+                np.testing.assert_allclose(st, z["state"][k], rtol=tol, atol=tol * 0.05)
+                k += 1
+        b.state_update(torch.from_numpy(z["su_w"]).to(dtype).cuda())
+        st = torch.stack([b.s, b.q, b.v, b.f_in]).cpu().numpy()
+        np.testing.assert_allclose(st, z["su_state"], rtol=tol, atol=1e-12)
+        nv3 = z["su_w"].shape[0] * z["su_w"].shape[1]
+        out = b.run(torch.from_numpy(z["u"][0][:nv3]).to(dtype).cuda())
+        np.testing.assert_allclose(out.cpu().numpy(), z["su_out"], rtol=tol, atol=tol * 0.05)
+
+
+def test_enkf_kernel_matches_oracle():
+    from digital_twin_brain_amd import diffusion_enkf
+
+    rng = np.random.default_rng(8)
+    for E, V, S in ((64, 90, 10), (256, 90, 10), (7, 3, 2)):
+        w = rng.normal(size=(E, V, S)) * 0.1
+        y = rng.normal(size=V) * 0.05
+        xi = rng.normal(size=(E, V))
+        for rho in (0.0, 0.8, 1.0):
+            got = diffusion_enkf(torch.from_numpy(w).cuda(), 1e-4, torch.from_numpy(y).reshape(1, -1), rho,
+                                 xi=torch.from_numpy(xi))
+            want = orc.diffusion_enkf(w, 1e-4, y, rho, xi)
+            np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-10, atol=1e-12)
+
+
+def test_invalid_inputs_raise_cleanly():
+    from digital_twin_brain_amd import NativeBlock
+    from digital_twin_brain_amd._native import DTBError
+
+    props = _c1_like_props(10, 8, np.random.default_rng(0))
+    with pytest.raises(DTBError):
+        NativeBlock.from_csr(props, np.array([0] * 10 + [1], np.int64), np.array([10], np.int32),
+                             np.array([0], np.uint8), np.array([1.0], np.float32))
+    with pytest.raises(DTBError):
+        NativeBlock.from_csr(props, np.array([0] * 10 + [1], np.int64), np.array([3], np.int32),
+                             np.array([2], np.uint8), np.array([1.0], np.float32))
+    nb = NativeBlock.from_csr(props, np.zeros(11, np.int64), np.zeros(0, np.int32), np.zeros(0, np.uint8),
+                              np.zeros(0, np.float32))
+    with pytest.raises(DTBError) as e:
+        nb.run(3, sample_v=np.zeros(3, np.float32))
+    assert e.value.code == 2  # UNSET_SAMPLE_FAILURE
+    with pytest.raises(DTBError):
+        nb.mul_props_by_pop([5], [10], [2.0])  # sub-population out of range
