@@ -1,0 +1,178 @@
+"""Benchmark: synaptic events/s of the network step on one MI355X.
+
+Workload (default): the single large block -- N = 5,000,000 LIF neurons,
+in-degree 1,000 (5e9 synapses, int32 index + fp32 weight), per-synapse delays
+U{1..10} steps, per-neuron Poisson noise 5-20 Hz, dt = 0.1 ms, BOLD per
+1,000-neuron voxel once per ms; one step = one pass of the fused neuron
+kernel + the event-driven delivery kernel (+ the BOLD kernel every 10th
+step).  Inputs are resident in HBM before timing starts.
+
+python bench.py [--gpus 1] [--steps 10000] [--warmup 200]
+
+Prints one JSON line.  The CPU baseline times the oracle's FP32 restatement
+of the reference algorithm (a full pass over every synapse per step, as
+torch.sparse.mm in models/block.py:76-81) with OpenMP on a bounded sample of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_NEURON_COMPACT = 141  # bytes per neuron per step of neuron_step_kernel (compact weights, per-neuron noise)
+B_SPIKE_ROW = 20        # rowptr begin/end (16) + spike-list entry (4) per delivered row
+B_SYNAPSE = 8           # packed int32 target + fp32 weight per synaptic event
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pmc_traffic(kernel_prefix, workload):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_prefix}_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n_sample, in_degree, steps_warm, steps_timed):
+    """FP32 oracle, full-synapse pass per step (pull), OpenMP over the host cores."""
+    from digital_twin_brain_amd import networks
+    from oracle import oracle as orc
+
+    nb, props = networks.build_c2(n=n_sample, in_degree=in_degree)
+    csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
+               w0=nb.get("w0"), w1=None)
+    nb.close()
+    outdeg = np.diff(csr["rowptr"])
+    ob = orc.OracleBlock(props, csr, 0.1, real="f32", pull=True)
+    for _ in range(steps_warm):
+        ob.run(noise_mode=1, seed=3)
+    ev = 0
+    t0 = time.perf_counter()
+    for _ in range(steps_timed):
+        ob.run(noise_mode=1, seed=3)
+        ev += int(outdeg[ob.get("spike").astype(bool)].sum())
+    dt = time.perf_counter() - t0
+    return dict(value=ev / dt, unit="synaptic events/s", cores=orc.omp_threads("f32"), kind="port",
+                sample=(f"oracle FP32 pull (full synapse pass per step, the torch.sparse.mm algorithm) on the "
+                        f"same workload scaled to N={n_sample} (in-degree {in_degree}, delays 1-10, same "
+                        f"parameters): {steps_timed} steps timed after {steps_warm} warm-up steps, "
+                        f"{dt:.1f} s, {ev} events"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=5_000_000)
+    ap.add_argument("--in-degree", type=int, default=1000)
+    ap.add_argument("--kernel-window", type=int, default=200)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the single-GPU build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from digital_twin_brain_amd import networks
+
+    torch.cuda.init()
+    t0 = time.perf_counter()
+    nb, props = networks.build_c2(n=a.n, in_degree=a.in_degree)
+    log(f"built C2 network n={nb.n} nnz={nb.nnz} S={nb.S} D={nb.max_delay} in {time.perf_counter() - t0:.1f} s")
+    nb.attach_bold(steps_per_sample=10, delta_t=1e-3)
+    S = nb.S
+    # warm-up (also captures the step-batch graph)
+    W = a.warmup - a.warmup % 10
+    if W:
+        nb.run(W, noise_mode=1, graph=True)
+    log(f"warm-up {W} steps done")
+    if not a.no_tune:
+        from digital_twin_brain_amd.autotune import tune_delivery
+
+        best, table = tune_delivery(nb, dict(noise_mode=1), steps=20, use_cache=False)
+        log(f"autotuned delivery launch: {best} from {json.dumps(table)}")
+        nb.run(100, noise_mode=1, graph=True)  # re-capture the step graph with the chosen shape
+    K = a.steps - a.steps % 10
+    events = np.zeros(K, np.uint64)
+    nspk = np.zeros(K, np.uint32)
+    bold = np.zeros((K // 10, S), np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nb.run(K, noise_mode=1, graph=True, events=events, spikes=nspk, bold=bold)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tot_ev = int(events.sum())
+    log(f"timed {K} steps in {wall:.3f} s: {tot_ev} events, {int(nspk.sum())} spikes, "
+        f"mean rate {nspk.sum() / K / nb.n / 1e-4:.2f} Hz delivered")
+
+    # per-kernel device time over a window (hipEvents on the block's stream)
+    M = max(10, a.kernel_window - a.kernel_window % 10)
+    kms = np.zeros((M, 2), np.float32)
+    ev_w = np.zeros(M, np.uint64)
+    sp_w = np.zeros(M, np.uint32)
+    nb.run(M, noise_mode=1, graph=False, events=ev_w, spikes=sp_w, kernel_ms=kms)
+    t_neu = float(kms[:, 0].mean()) * 1e-3
+    t_del = float(kms[:, 1].mean()) * 1e-3
+    b_neu = B_NEURON_COMPACT * nb.n
+    b_del = float((B_SPIKE_ROW * sp_w.astype(np.float64) + B_SYNAPSE * ev_w.astype(np.float64)).mean())
+    if t_neu >= t_del:
+        dom, bytes_l, tl = "neuron_step_kernel", b_neu, t_neu
+    else:
+        dom, bytes_l, tl = "deliver_kernel", b_del, t_del
+    achieved = bytes_l / tl / 1e9
+    kernels = {"neuron_step_kernel": {"avg_us": t_neu * 1e6, "bytes_per_launch": b_neu,
+                                      "GBps": b_neu / t_neu / 1e9},
+               "deliver_kernel": {"avg_us": t_del * 1e6, "bytes_per_launch": b_del, "GBps": b_del / t_del / 1e9,
+                                  "events_per_launch": float(ev_w.mean()),
+                                  "atomic_rmw_bytes_per_launch": 8.0 * float(ev_w.mean())}}
+    log(f"kernels: {json.dumps(kernels)}")
+    launch = nb.get_launch()
+    nb.close()
+    del nb
+
+    cpu = None
+    if not a.no_cpu:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(a.cpu_sample, a.in_degree, 200, a.cpu_steps)
+        log(f"cpu baseline {cpu['value']:.3e} events/s ({time.perf_counter() - t0:.1f} s)")
+
+    value = tot_ev / wall
+    traffic = pmc_traffic(dom, "c2")
+    out = {
+        "metric": "synaptic events/s", "value": value, "unit": "events/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "C2 single large block", "neurons": a.n, "in_degree": a.in_degree,
+                   "synapses": a.n * a.in_degree, "delays": "1-10 steps", "dt_ms": 0.1,
+                   "noise_hz": "5-20 per neuron", "bold": "per 1000-neuron voxel, 1 ms", "weights": "compact fp32"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "kernels": kernels,
+        "launch": launch,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

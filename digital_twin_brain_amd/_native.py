@@ -55,7 +55,7 @@ class FixedIndegreeSpec(ctypes.Structure):
 class RunArgs(ctypes.Structure):
     _fields_ = [("noise_mode", _i32), ("noise_rate", _f32), ("uniforms", _vp), ("isolated", _i32),
                 ("use_graph", _i32), ("freqs", _vp), ("vmeans", _vp), ("imeans", _vp), ("sample_spike", _vp),
-                ("sample_v", _vp), ("events", _vp), ("spikes", _vp)]
+                ("sample_v", _vp), ("events", _vp), ("spikes", _vp), ("bold", _vp), ("kernel_ms", _vp)]
 
 
 _lib = None
@@ -84,6 +84,10 @@ def lib():
         "dtb_block_gamma_props_by_pop": [_vp, _i64, _vp, _vp, _vp, _vp, _u64],
         "dtb_block_replace_props": [_vp, _vp],
         "dtb_block_set_samples": [_vp, _i64, _vp],
+        "dtb_block_attach_bold": [_vp, _vp, _i32, _i32, _vp, _vp],
+        "dtb_block_set_launch": [_vp, _i32, _i32],
+        "dtb_block_get_launch": [_vp, _i32, _vp],
+        "dtb_block_bold_state": [_vp, _i32, _i32, _vp],
         "dtb_bold_step_f32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "dtb_bold_step_f64": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "dtb_enkf_analysis_f64": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
@@ -101,7 +105,9 @@ def exported_symbols():
     return ["dtb_last_error", "dtb_abi_version", "dtb_block_create", "dtb_block_create_fixed_indegree",
             "dtb_block_destroy", "dtb_block_run", "dtb_block_get", "dtb_block_info", "dtb_block_time",
             "dtb_block_set_props", "dtb_block_mul_props_by_pop", "dtb_block_gamma_props_by_pop",
-            "dtb_block_replace_props", "dtb_block_set_samples", "dtb_bold_step_f32", "dtb_bold_step_f64",
+            "dtb_block_replace_props", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",
+            "dtb_block_set_launch", "dtb_block_get_launch",
+            "dtb_bold_step_f32", "dtb_bold_step_f64",
             "dtb_enkf_analysis_f64"]
 
 

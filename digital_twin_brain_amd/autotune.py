@@ -1,0 +1,81 @@
+"""Launch-shape autotuner for the delivery kernel.
+
+The delivery kernel's cost depends on the spike rate, the row length per
+delay segment and the network size; the best (lanes per segment, workgroup
+count) pair is measured rather than guessed.  Each candidate advances the
+simulation by `steps` steps with per-kernel hipEvent timing on the block's
+stream (so tuning is part of warm-up: it changes the state, not the
+results' correctness), and the fastest is kept.  Choices are cached per
+(neurons, synapses per row-segment, delays, spikes-per-step) bucket in a JSON
+file (DTB_TUNE_CACHE, default ~/.cache/dtb_tune.json).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+CACHE = os.environ.get("DTB_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "dtb_tune.json"))
+
+
+def _bucket(nb, spikes_per_step):
+    seg = nb.nnz / max(1, nb.n * nb.max_delay)
+    return "n{}_seg{}_D{}_spk{}".format(int(math.log2(max(nb.n, 1))), int(math.log2(max(seg, 1))), nb.max_delay,
+                                        int(math.log2(max(spikes_per_step, 1))))
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(c):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as f:
+            json.dump(c, f, indent=1)
+    except OSError:
+        pass
+
+
+def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
+    """Pick the delivery launch shape for `nb` under `run_kwargs` (noise mode/rate).
+
+    Returns (best, table) where table maps 'lanes/grid' to mean delivery us.
+    """
+    step_q = getattr(nb, "bold_sps", 1) or 1
+    steps = max(step_q, steps - steps % step_q)
+    kms = np.zeros((steps, 2), np.float32)
+    spk = np.zeros(steps, np.uint32)
+    nb.run(steps, graph=False, kernel_ms=kms, spikes=spk, **run_kwargs)
+    key = _bucket(nb, float(spk.mean()) * nb.max_delay)
+    cache = _load() if use_cache else {}
+    if key in cache:
+        best = cache[key]
+        nb.set_launch(deliver_lanes=best["deliver_lanes"], deliver_grid=best["deliver_grid"])
+        return best, {"cached": key}
+    segs = max(64.0, float(spk.mean()) * nb.max_delay)
+    table = {}
+    for lanes in (64, 32, 16):
+        groups_per_wg = 4 * (64 // lanes)
+        base = max(4, int(math.ceil(segs / groups_per_wg)))
+        for mult in (0.25, 0.5, 1.0, 2.0):
+            grid = int(min(65536, max(4, base * mult)))
+            nb.set_launch(deliver_lanes=lanes, deliver_grid=grid)
+            nb.run(steps, graph=False, kernel_ms=kms, **run_kwargs)
+            table[f"{lanes}/{grid}"] = float(kms[:, 1].mean()) * 1e3
+            if log:
+                log(f"tune lanes={lanes} grid={grid}: deliver {table[f'{lanes}/{grid}']:.1f} us")
+    lanes, grid = min(table, key=table.get).split("/")
+    best = {"deliver_lanes": int(lanes), "deliver_grid": int(grid), "us": table[f"{lanes}/{grid}"]}
+    nb.set_launch(deliver_lanes=best["deliver_lanes"], deliver_grid=best["deliver_grid"])
+    if use_cache:
+        cache[key] = best
+        _save(cache)
+    return best, table
+

@@ -4,22 +4,25 @@
 // One step n (t_n = t_{n-1} + dt, accumulated in double) is two kernels:
 //
 //   neuron_step_kernel  (one lane per neuron, coalesced SoA)
-//     J_{n-1} = Jp_{n-1} + ring[(n-1) mod D]      ring slot cleared
+//     J_{n-1} = Jp_{n-1} + acc                     acc cleared
 //     I_syn   = sum_k g_k (V_uk - V_{n-1}) J_k     models/block.py:83-87
 //     V_n     = LIF Euler + refractory + clamp     models/block.py:89-111
 //     s_n     = noise | threshold                  models/block.py:136-138
 //     Jp_n    = J_{n-1} * exp(-dt/tau)             models/block.py:71
-//     spiking ids appended to the step's list (ballot + LDS prefix, one
-//     global atomic per workgroup); threshold counts and V sums per
+//     spiking ids appended to the step's spike list (ballot + LDS prefix,
+//     one global atomic per workgroup); threshold counts and V sums per
 //     sub-population (wavefront reductions)
 //
-//   deliver_kernel  (one wavefront per spiking row, 64 synapses per pass)
-//     ring[(n + d - 1) mod D][post] += w          models/block.py:72-73,
-//     the event-driven form of W.s with per-synapse delays d (d = 1 is the
+//   deliver_kernel  (one wavefront per (spiking row, delay) segment)
+//     for d = 1..D, every row j in the spike list of step n-d+1:
+//       acc[post] += w over the row's delay-d segment   models/block.py:72-73
+//     the event-driven form of W.s with per-synapse delays (d = 1 is the
 //     reference's same-step delivery).
 //
-// So J_n = Jp_n + ring[n mod D] once the step's delivery has run; the next
-// neuron kernel folds the slot in and clears it.
+// Delays live on the spike side: the last D+1 spike lists are kept, and each
+// CSR row is sorted by delay with per-row segment ends, so every arrival of
+// a step lands in one [n] accumulator (8 B/neuron) that stays cache-resident
+// instead of a [D][n] ring.  J_n = Jp_n + acc once the step's delivery ran.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -88,9 +91,10 @@ struct NeuronArgs {
     int32_t* __restrict__ last_step;
     double* __restrict__ t_last;
     uint8_t* __restrict__ flags;
-    void* __restrict__ ring;
+    void* __restrict__ acc;
     int32_t* __restrict__ spk;
     uint32_t* __restrict__ spk_cnt;
+    int32_t L; // spike-list slots (D + 1)
     const StepCtl* __restrict__ ctl;
     const double* __restrict__ ttab;
     double dt;
@@ -127,23 +131,23 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t step = a.ctl->nbase + k + 1;
     const double t = a.ttab[k];
-    const int par = (int)(step & 1);
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.spk_cnt[par ^ 1] = 0; // list of step+1
+    const int32_t slot = (int32_t)(step % a.L);
+    // the list of step+1 reuses the slot of step+1-L, whose last delivery was step
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.spk_cnt[(step + 1) % a.L] = 0;
 
     bool act = false, spike = false;
     float vout = 0.0f;
     int32_t p = -1;
     if (i < a.n) {
-        const uint32_t slot = (uint32_t)((step - 1) % a.D);
         const float V = a.V[i];
         float4 J = a.Jp[i];
         if (PAIR) {
-            float4* r = reinterpret_cast<float4*>(a.ring) + (int64_t)slot * a.n + i;
+            float4* r = reinterpret_cast<float4*>(a.acc) + i;
             const float4 d = *r;
             *r = make_float4(0.f, 0.f, 0.f, 0.f);
             J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
         } else {
-            float2* r = reinterpret_cast<float2*>(a.ring) + (int64_t)slot * a.n + i;
+            float2* r = reinterpret_cast<float2*>(a.acc) + i;
             const float2 d = *r;
             *r = make_float2(0.f, 0.f);
             J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
@@ -189,13 +193,13 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
-        s_base = tot ? atomicAdd(&a.spk_cnt[par], tot) : 0u;
+        s_base = tot ? atomicAdd(&a.spk_cnt[slot], tot) : 0u;
     }
     __syncthreads();
     if (spike) {
         uint32_t off = s_base;
         for (uint32_t w = 0; w < wid; ++w) off += s_wcnt[w];
-        a.spk[(int64_t)par * a.n + off + pre] = (int32_t)i;
+        a.spk[(int64_t)slot * a.n + off + pre] = (int32_t)i;
     }
 
     // per sub-population statistics (threshold spikes, V sum)
@@ -221,13 +225,14 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
 
 struct DeliverArgs {
     int64_t n;
-    int32_t D;
+    int32_t D, L;
     SynPack pk;
     const int64_t* __restrict__ rowptr;
+    const uint16_t* __restrict__ dend; // [n][D] end of each delay segment, relative to the row start
     const uint32_t* __restrict__ col;
     const float* __restrict__ w;
     const float2* __restrict__ w2;
-    void* __restrict__ ring;
+    void* __restrict__ acc;
     const int32_t* __restrict__ spk;
     const uint32_t* __restrict__ spk_cnt;
     const StepCtl* __restrict__ ctl;
@@ -235,42 +240,91 @@ struct DeliverArgs {
     uint32_t* __restrict__ rec_nspk;
 };
 
-template <bool PAIR>
+// G lanes per (row, delay) segment (64, 32 or 16): a wavefront works on 64/G
+// segments at once, each lane keeps two synapse loads in flight.
+template <bool PAIR, int G>
 __global__ __launch_bounds__(256) void deliver_kernel(DeliverArgs a, int32_t k)
 {
+    // cum[d] = number of (row, delay) segments with delay <= d
+    __shared__ uint32_t cum[DTB_MAX_DELAY + 1];
+    __shared__ int32_t slot_of[DTB_MAX_DELAY + 1];
     const int64_t step = a.ctl->nbase + k + 1;
-    const int par = (int)(step & 1);
-    const uint32_t cnt = min(a.spk_cnt[par], (uint32_t)a.n);
-    const uint32_t base_slot = (uint32_t)(step % a.D);
-    const int32_t* list = a.spk + (int64_t)par * a.n;
+    if (threadIdx.x == 0) {
+        uint32_t c = 0;
+        cum[0] = 0;
+        for (int32_t d = 1; d <= a.D; ++d) {
+            const int32_t sl = (int32_t)((step - d + 1 + a.L) % a.L);
+            slot_of[d] = sl;
+            c += min(a.spk_cnt[sl], (uint32_t)a.n);
+            cum[d] = c;
+        }
+    }
+    __syncthreads();
+    const uint32_t total = cum[a.D];
     const uint32_t lane = __lane_id();
-    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
-    if (gw == 0 && lane == 0 && a.rec_nspk) a.rec_nspk[k] = cnt;
+    const uint32_t sub = lane % G;
+    constexpr uint32_t kGroups = kWave / G;
+    const uint32_t gid = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kGroups + lane / G;
+    const uint32_t ngroups = ((gridDim.x * blockDim.x) >> 6) * kGroups;
+    if (gid == 0 && sub == 0 && a.rec_nspk) a.rec_nspk[k] = cum[1];
     uint64_t ev = 0;
-    for (uint32_t r = gw; r < cnt; r += nw) {
-        const int32_t j = list[r];
-        const int64_t beg = a.rowptr[j];
-        const int64_t end = a.rowptr[j + 1];
+    int32_t d = 1;
+    for (uint32_t r = gid; r < total; r += ngroups) {
+        while (r >= cum[d]) ++d; // r only grows, so d only moves forward
+        const int32_t j = a.spk[(int64_t)slot_of[d] * a.n + (r - cum[d - 1])];
+        const int64_t row = a.rowptr[j];
+        int64_t beg, end;
+        if (a.D == 1) {
+            beg = row;
+            end = a.rowptr[j + 1];
+        } else {
+            const uint16_t* de = a.dend + (int64_t)j * a.D;
+            beg = row + (d > 1 ? de[d - 2] : 0);
+            end = row + de[d - 1];
+        }
         ev += (uint64_t)(end - beg);
-        for (int64_t e = beg + lane; e < end; e += kWave) {
-            const uint32_t c = a.col[e];
-            const uint32_t post = c & a.pk.idx_mask;
-            uint32_t slot = base_slot + ((c >> a.pk.idx_bits) & a.pk.dmask);
-            if (slot >= (uint32_t)a.D) slot -= (uint32_t)a.D;
-            const uint32_t kind = c >> 31;
+        for (int64_t e = beg + sub; e < end; e += 2 * G) {
+            const int64_t e2 = e + G;
+            const bool two = e2 < end;
+            const uint32_t c0 = a.col[e];
+            const uint32_t c1 = two ? a.col[e2] : 0u;
             if (PAIR) {
-                const float2 wv = a.w2[e];
-                float* dst = reinterpret_cast<float*>(a.ring) + (((int64_t)slot * a.n + post) << 2) + (kind << 1);
-                unsafeAtomicAdd(dst, wv.x);
-                unsafeAtomicAdd(dst + 1, wv.y);
+                const float2 w0 = a.w2[e];
+                const float2 w1 = two ? a.w2[e2] : make_float2(0.f, 0.f);
+                float* d0 = reinterpret_cast<float*>(a.acc) + ((int64_t)(c0 & a.pk.idx_mask) << 2) + ((c0 >> 31) << 1);
+                unsafeAtomicAdd(d0, w0.x);
+                unsafeAtomicAdd(d0 + 1, w0.y);
+                if (two) {
+                    float* d1 =
+                        reinterpret_cast<float*>(a.acc) + ((int64_t)(c1 & a.pk.idx_mask) << 2) + ((c1 >> 31) << 1);
+                    unsafeAtomicAdd(d1, w1.x);
+                    unsafeAtomicAdd(d1 + 1, w1.y);
+                }
             } else {
-                float* dst = reinterpret_cast<float*>(a.ring) + (((int64_t)slot * a.n + post) << 1) + kind;
-                unsafeAtomicAdd(dst, a.w[e]);
+                const float w0 = a.w[e];
+                const float w1 = two ? a.w[e2] : 0.0f;
+                unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + ((int64_t)(c0 & a.pk.idx_mask) << 1) + (c0 >> 31), w0);
+                if (two)
+                    unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + ((int64_t)(c1 & a.pk.idx_mask) << 1) + (c1 >> 31),
+                                    w1);
             }
         }
     }
+    // per-group event counts -> one atomic per wavefront
+#pragma unroll
+    for (int off = G; off < kWave; off <<= 1) ev += __shfl_xor(ev, off);
     if (lane == 0 && ev && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev);
+}
+
+template <bool PAIR>
+static void launch_deliver(const Block* b, const DeliverArgs& da, int32_t k)
+{
+    const dim3 g(b->deliver_grid), t(256);
+    switch (b->deliver_lanes) {
+    case 16: hipLaunchKernelGGL((deliver_kernel<PAIR, 16>), g, t, 0, b->stream, da, k); break;
+    case 32: hipLaunchKernelGGL((deliver_kernel<PAIR, 32>), g, t, 0, b->stream, da, k); break;
+    default: hipLaunchKernelGGL((deliver_kernel<PAIR, 64>), g, t, 0, b->stream, da, k); break;
+    }
 }
 
 // Batch prologue: times of the K steps, step counters, record clearing.
@@ -307,18 +361,16 @@ template <bool PAIR>
 __global__ __launch_bounds__(256) void isum_kernel(NeuronArgs a, int32_t k, float* rec_isum)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t step = a.ctl->nbase + k + 1;
     float isyn = 0.0f;
     int32_t p = -1;
     if (i < a.n) {
-        const uint32_t slot = (uint32_t)(step % a.D);
         const float V = a.V[i];
         float4 J = a.Jp[i];
         if (PAIR) {
-            const float4 d = reinterpret_cast<const float4*>(a.ring)[(int64_t)slot * a.n + i];
+            const float4 d = reinterpret_cast<const float4*>(a.acc)[i];
             J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
         } else {
-            const float2 d = reinterpret_cast<const float2*>(a.ring)[(int64_t)slot * a.n + i];
+            const float2 d = reinterpret_cast<const float2*>(a.acc)[i];
             J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
         }
         const float4 g = a.g[i];
@@ -349,6 +401,47 @@ __global__ void sample_kernel(int64_t ns, const uint32_t* __restrict__ idx, cons
     const uint32_t j = idx[s];
     sv[(int64_t)k * ns + s] = V[j];
     sspk[(int64_t)k * ns + s] = (uint8_t)((flags[j] >> 1) & 1);
+}
+
+// Attached Balloon-Windkessel step per voxel: input = threshold spikes of
+// the voxel's sub-populations over the last sps steps / voxel size, floored
+// at 1e-5 (simulation/simulation.py:274-279, the np.histogram merge of
+// populations into voxels), then models/bold_model_pytorch.py:69-77.
+__global__ void bold_counts_kernel(int32_t S, int32_t V, int32_t k, int32_t sps, const uint32_t* __restrict__ rec_freq,
+                                   const int32_t* __restrict__ vptr, const int32_t* __restrict__ vpops,
+                                   const double* __restrict__ size, const double* __restrict__ par,
+                                   double* __restrict__ st, double* __restrict__ rec_bold)
+{
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint64_t c = 0;
+    for (int32_t x = vptr[v]; x < vptr[v + 1]; ++x) {
+        const int32_t p = vpops[x];
+        for (int32_t j = k - sps + 1; j <= k; ++j) c += rec_freq[(int64_t)j * S + p];
+    }
+    double u = (double)c / size[v];
+    u = u > 1e-5 ? u : (u == u ? 1e-5 : u);
+    const double eps = par[0], ts = par[1], tf = par[2], t0 = par[3], div_alpha = 1.0 / par[4], E0 = par[5],
+                 V0 = par[6], dt = par[7];
+    double* s = st;
+    double* q = st + V;
+    double* vv = st + 2 * V;
+    double* f = st + 3 * V;
+    const double si = s[v], qi = q[v], vi = vv[v], fi = f[v];
+    const double d_s = eps * u - si / ts - (fi - 1.0) / tf;
+    const double q_part = fi > 0.0 ? 1.0 - pow(1.0 - E0, 1.0 / fi) : 1.0;
+    const double qn = qi + (fi * q_part / E0 - qi * pow(vi, div_alpha - 1.0)) / t0 * dt;
+    const double vn = vi + (fi - pow(vi, div_alpha)) / t0 * dt;
+    double fn = fi + si * dt;
+    fn = fn > 0.0 ? fn : 0.0;
+    s[v] = si + d_s * dt;
+    q[v] = qn;
+    vv[v] = vn;
+    f[v] = fn;
+    const double out = V0 * (7.0 * E0 * (1.0 - qn) + 2.0 * (1.0 - qn / vn) + (2.0 * E0 - 0.2) * (1.0 - vn));
+    st[4 * V + v] = out;
+    st[5 * V + v] = u;
+    rec_bold[(int64_t)(k / sps) * V + v] = out;
 }
 
 __global__ void scale_kernel(float* x, const float* inv, int64_t K, int32_t S)
@@ -425,18 +518,18 @@ __global__ void unpack_kernel(int64_t nnz, const uint32_t* __restrict__ col, Syn
 }
 
 template <bool PAIR>
-__global__ void state_kernel(int64_t n, uint32_t slot, const float* __restrict__ V, const float4* __restrict__ Jp,
-                             const void* __restrict__ ring, const float4* __restrict__ g,
+__global__ void state_kernel(int64_t n, const float* __restrict__ V, const float4* __restrict__ Jp,
+                             const void* __restrict__ acc, const float4* __restrict__ g,
                              const float4* __restrict__ vu, float* J_out, float* I_out, float* Isyn_out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 J = Jp[i];
     if (PAIR) {
-        const float4 d = reinterpret_cast<const float4*>(ring)[(int64_t)slot * n + i];
+        const float4 d = reinterpret_cast<const float4*>(acc)[i];
         J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
     } else {
-        const float2 d = reinterpret_cast<const float2*>(ring)[(int64_t)slot * n + i];
+        const float2 d = reinterpret_cast<const float2*>(acc)[i];
         J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
     }
     const float v = V[i];
@@ -588,7 +681,12 @@ int derive_props(Block* b)
     std::vector<float> inv((size_t)b->S);
     for (int32_t s = 0; s < b->S; ++s) inv[(size_t)s] = b->pop_size[(size_t)s] ? 1.0f / (float)b->pop_size[(size_t)s] : NAN;
     if (!b->pop_inv) DTB_TRY(alloc_n(b, &b->pop_inv, b->S));
+    if (!b->pop_sized) DTB_TRY(alloc_n(b, &b->pop_sized, b->S));
+    std::vector<double> szd((size_t)b->S);
+    for (int32_t s = 0; s < b->S; ++s) szd[(size_t)s] = (double)b->pop_size[(size_t)s];
     DTB_TRY_HIP(hipMemcpyAsync(b->pop_inv, inv.data(), sizeof(float) * (size_t)b->S, hipMemcpyHostToDevice, b->stream));
+    DTB_TRY_HIP(hipMemcpyAsync(b->pop_sized, szd.data(), sizeof(double) * (size_t)b->S, hipMemcpyHostToDevice,
+                               b->stream));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     return DTB_OK;
 }
@@ -616,8 +714,8 @@ static int alloc_state(Block* b)
     DTB_TRY(alloc_n(b, &b->last_step, n));
     DTB_TRY(alloc_n(b, &b->t_last, n));
     DTB_TRY(alloc_n(b, &b->flags, n));
-    DTB_TRY(alloc_n(b, &b->spk, 2 * n));
-    DTB_TRY(alloc_n(b, &b->spk_cnt, 2));
+    DTB_TRY(alloc_n(b, &b->spk, (int64_t)(b->D + 1) * n));
+    DTB_TRY(alloc_n(b, &b->spk_cnt, b->D + 1));
     DTB_TRY(alloc_n(b, &b->ctl, 1));
     b->kmax = 1000;
     DTB_TRY(alloc_n(b, &b->ttab, b->kmax));
@@ -627,7 +725,7 @@ static int alloc_state(Block* b)
     DTB_TRY_HIP(hipMemsetAsync(b->last_step, 0, sizeof(int32_t) * (size_t)n, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->t_last, 0, sizeof(double) * (size_t)n, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->flags, 0, (size_t)n, b->stream));
-    DTB_TRY_HIP(hipMemsetAsync(b->spk_cnt, 0, 2 * sizeof(uint32_t), b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->spk_cnt, 0, (size_t)(b->D + 1) * sizeof(uint32_t), b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->ctl, 0, sizeof(StepCtl), b->stream));
     return DTB_OK;
 }
@@ -635,14 +733,18 @@ static int alloc_state(Block* b)
 static int alloc_ring_and_records(Block* b)
 {
     const size_t esz = b->pair ? sizeof(float4) : sizeof(float2);
-    DTB_TRY(block_alloc(b, &b->ring, esz * (size_t)b->D * (size_t)b->n));
-    DTB_TRY_HIP(hipMemsetAsync(b->ring, 0, esz * (size_t)b->D * (size_t)b->n, b->stream));
+    DTB_TRY(block_alloc(b, &b->acc, esz * (size_t)b->n));
+    DTB_TRY_HIP(hipMemsetAsync(b->acc, 0, esz * (size_t)b->n, b->stream));
     DTB_TRY(alloc_n(b, &b->rec_freq, b->kmax * b->S));
     DTB_TRY(alloc_n(b, &b->rec_vsum, b->kmax * b->S));
     DTB_TRY(alloc_n(b, &b->rec_isum, b->kmax * b->S));
-    // delivery grid: enough wavefronts for the expected spiking rows
-    const double expect = std::max(8.0, (double)b->n * 4e-3);
-    b->deliver_grid = (int32_t)std::min<double>(2048.0, std::max(4.0, std::ceil(expect / 4.0)));
+    // delivery launch shape: about one segment per lane group at ~20 Hz of
+    // spikes (an autotuner may replace it, dtb_block_set_launch)
+    const double segs = std::max(64.0, (double)b->n * 2e-3 * b->D);
+    const int32_t avg_seg = (int32_t)(b->n ? b->nnz / std::max<int64_t>(1, b->n * b->D) : 64);
+    b->deliver_lanes = avg_seg >= 48 ? 64 : avg_seg >= 24 ? 32 : 16;
+    const double groups_per_wg = 4.0 * (64 / b->deliver_lanes);
+    b->deliver_grid = (int32_t)std::min<double>(16384.0, std::max(4.0, std::ceil(segs / groups_per_wg)));
     return DTB_OK;
 }
 
@@ -662,6 +764,7 @@ static void destroy(Block* b)
     if (!b) return;
     if (b->stream) hipStreamSynchronize(b->stream);
     for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    for (hipEvent_t e : b->events) hipEventDestroy(e);
     for (void* p : b->allocs) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -732,6 +835,60 @@ static int create_from_csr(Block* b, const dtb_block_desc* d)
     DTB_TRY(alloc_n(b, &b->rowptr, n + 1));
     DTB_TRY(upload(b, b->rowptr, rp.data(), sizeof(int64_t) * (size_t)(n + 1)));
     DTB_TRY(alloc_n(b, &b->col, nnz));
+    // with delays, each row is reordered by delay (stable) and the segment
+    // ends recorded, so the delivery of a (row, delay) pair is contiguous
+    std::vector<int32_t> h_post;
+    std::vector<uint8_t> h_kind, h_delay;
+    std::vector<float> h_w0, h_w1;
+    dtb_block_desc dd = *d;
+    if (D > 1 && nnz > 0) {
+        h_post.resize((size_t)nnz);
+        h_kind.resize((size_t)nnz);
+        h_delay.resize((size_t)nnz);
+        h_w0.resize((size_t)nnz);
+        DTB_TRY_HIP(hipMemcpy(h_post.data(), d->post, 4 * (size_t)nnz, hipMemcpyDefault));
+        DTB_TRY_HIP(hipMemcpy(h_kind.data(), d->kind, (size_t)nnz, hipMemcpyDefault));
+        DTB_TRY_HIP(hipMemcpy(h_delay.data(), d->delay, (size_t)nnz, hipMemcpyDefault));
+        DTB_TRY_HIP(hipMemcpy(h_w0.data(), d->w0, 4 * (size_t)nnz, hipMemcpyDefault));
+        if (d->w1) {
+            h_w1.resize((size_t)nnz);
+            DTB_TRY_HIP(hipMemcpy(h_w1.data(), d->w1, 4 * (size_t)nnz, hipMemcpyDefault));
+        }
+        std::vector<uint16_t> dend((size_t)n * D);
+        std::vector<int64_t> perm((size_t)nnz);
+        std::vector<int64_t> cnt((size_t)D + 1);
+        for (int64_t j = 0; j < n; ++j) {
+            const int64_t b0 = rp[(size_t)j], b1 = rp[(size_t)j + 1];
+            DTB_REQUIRE(b1 - b0 <= 65535, "row %lld has %lld synapses; rows with delays are limited to 65535",
+                        (long long)j, (long long)(b1 - b0));
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int64_t e = b0; e < b1; ++e) {
+                const int dl = h_delay[(size_t)e];
+                DTB_REQUIRE(dl >= 1 && dl <= D, "delay %d outside 1..%d", dl, D);
+                cnt[(size_t)dl]++;
+            }
+            for (int x = 1; x <= D; ++x) cnt[(size_t)x] += cnt[(size_t)x - 1];
+            for (int x = 1; x <= D; ++x) dend[(size_t)j * D + x - 1] = (uint16_t)cnt[(size_t)x];
+            for (int64_t e = b0; e < b1; ++e) perm[(size_t)(b0 + cnt[(size_t)h_delay[(size_t)e] - 1]++)] = e;
+        }
+        auto permute = [&](auto& v) {
+            auto tmp = v;
+            for (int64_t e = 0; e < nnz; ++e) v[(size_t)e] = tmp[(size_t)perm[(size_t)e]];
+        };
+        permute(h_post);
+        permute(h_kind);
+        permute(h_delay);
+        permute(h_w0);
+        if (d->w1) permute(h_w1);
+        dd.post = h_post.data();
+        dd.kind = h_kind.data();
+        dd.delay = h_delay.data();
+        dd.w0 = h_w0.data();
+        dd.w1 = d->w1 ? h_w1.data() : nullptr;
+        DTB_TRY(alloc_n(b, &b->dend, n * (int64_t)D));
+        DTB_TRY(upload(b, b->dend, dend.data(), sizeof(uint16_t) * dend.size()));
+        d = &dd;
+    }
     if (nnz > 0) {
         // stage the raw arrays on the device, pack and validate there
         void *dpost, *dkind, *ddelay = nullptr, *derr;
@@ -795,7 +952,8 @@ static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* unif
     x.g = b->g; x.vu = b->vu; x.decay = b->decay; x.gL = b->gL; x.vL = b->vL; x.dtC = b->dtC; x.vth = b->vth;
     x.vrst = b->vrst; x.iext = b->iext; x.tref = b->tref; x.noisep = b->noisep; x.pop = b->pop;
     x.V = b->V; x.Jp = b->Jp; x.ref_end = b->ref_end; x.last_step = b->last_step; x.t_last = b->t_last;
-    x.flags = b->flags; x.ring = b->ring; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.ctl = b->ctl; x.ttab = b->ttab;
+    x.flags = b->flags; x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.L = b->D + 1; x.ctl = b->ctl;
+    x.ttab = b->ttab;
     x.dt = b->dt; x.seed = b->seed;
     x.noise_mode = a->noise_mode; x.noise_rate = a->noise_rate; x.uniforms = uniforms; x.isolated = a->isolated;
     x.rec_freq = freq ? b->rec_freq : nullptr;
@@ -806,18 +964,26 @@ static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* unif
 static DeliverArgs deliver_args(Block* b)
 {
     DeliverArgs x;
-    x.n = b->n; x.D = b->D; x.pk = b->pk; x.rowptr = b->rowptr; x.col = b->col; x.w = b->w; x.w2 = b->w2;
-    x.ring = b->ring; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.ctl = b->ctl; x.rec_events = b->rec_events;
+    x.n = b->n; x.D = b->D; x.L = b->D + 1; x.pk = b->pk; x.rowptr = b->rowptr; x.dend = b->dend; x.col = b->col;
+    x.w = b->w; x.w2 = b->w2;
+    x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.ctl = b->ctl; x.rec_events = b->rec_events;
     x.rec_nspk = b->rec_nspk;
     return x;
 }
 
 struct Want {
-    bool freq, vsum, isum, sample;
+    bool freq, vsum, isum, sample, bold, timing;
 };
 
 static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float* uniforms, Want w)
 {
+    if (w.timing && (int64_t)b->events.size() < 3ll * K) {
+        for (size_t e = b->events.size(); e < (size_t)(3 * K); ++e) {
+            hipEvent_t ev;
+            DTB_TRY_HIP(hipEventCreate(&ev));
+            b->events.push_back(ev);
+        }
+    }
     const int64_t nrec = (int64_t)K * b->S;
     hipLaunchKernelGGL(prologue_kernel, dim3(grid_for(std::max<int64_t>(nrec, K), 256 * 4)), dim3(256), 0,
                        b->stream, b->ctl, b->ttab, K, b->dt, w.freq ? b->rec_freq : nullptr,
@@ -827,15 +993,27 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
     const DeliverArgs da = deliver_args(b);
     const unsigned ng = grid_for(b->n, 256);
     for (int32_t k = 0; k < K; ++k) {
-        if (b->pair) {
+        if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k)], b->stream));
+        if (b->pair)
             hipLaunchKernelGGL(neuron_step_kernel<true>, dim3(ng), dim3(256), 0, b->stream, na, k);
-            hipLaunchKernelGGL(deliver_kernel<true>, dim3(b->deliver_grid), dim3(256), 0, b->stream, da, k);
-            if (w.isum) hipLaunchKernelGGL(isum_kernel<true>, dim3(ng), dim3(256), 0, b->stream, na, k, b->rec_isum);
-        } else {
+        else
             hipLaunchKernelGGL(neuron_step_kernel<false>, dim3(ng), dim3(256), 0, b->stream, na, k);
-            hipLaunchKernelGGL(deliver_kernel<false>, dim3(b->deliver_grid), dim3(256), 0, b->stream, da, k);
-            if (w.isum) hipLaunchKernelGGL(isum_kernel<false>, dim3(ng), dim3(256), 0, b->stream, na, k, b->rec_isum);
+        if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
+        if (b->pair)
+            launch_deliver<true>(b, da, k);
+        else
+            launch_deliver<false>(b, da, k);
+        if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 2)], b->stream));
+        if (w.isum) {
+            if (b->pair)
+                hipLaunchKernelGGL(isum_kernel<true>, dim3(ng), dim3(256), 0, b->stream, na, k, b->rec_isum);
+            else
+                hipLaunchKernelGGL(isum_kernel<false>, dim3(ng), dim3(256), 0, b->stream, na, k, b->rec_isum);
         }
+        if (w.bold && (k + 1) % b->bold_sps == 0)
+            hipLaunchKernelGGL(bold_counts_kernel, dim3(grid_for(b->nvox, 256)), dim3(256), 0, b->stream, b->S,
+                               b->nvox, k, b->bold_sps, b->rec_freq, b->vox_ptr, b->vox_pops, b->vox_size,
+                               b->bold_state + 6 * (int64_t)b->nvox, b->bold_state, b->rec_bold);
         if (w.sample)
             hipLaunchKernelGGL(sample_kernel, dim3(grid_for(b->ns, 256)), dim3(256), 0, b->stream, b->ns, b->samples,
                                b->V, b->flags, k, b->rec_sspk, b->rec_sv);
@@ -874,8 +1052,12 @@ static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
     DTB_REQUIRE(a->noise_mode >= 0 && a->noise_mode <= 2, "noise_mode must be 0, 1 or 2");
     DTB_REQUIRE(a->noise_mode != DTB_NOISE_UNIFORMS || a->uniforms, "noise_mode 2 needs uniforms");
     DTB_REQUIRE(b->steps_done + nsteps < 0x7fff0000ll, "step counter limit (2^31) reached");
-    const Want w{a->freqs != nullptr, a->vmeans != nullptr, a->imeans != nullptr,
-                 a->sample_spike != nullptr || a->sample_v != nullptr};
+    const bool bold = b->bold_sps > 0;
+    const Want w{a->freqs != nullptr || bold, a->vmeans != nullptr, a->imeans != nullptr,
+                 a->sample_spike != nullptr || a->sample_v != nullptr, bold, a->kernel_ms != nullptr};
+    DTB_REQUIRE(!bold || nsteps % b->bold_sps == 0, "with BOLD attached, nsteps must be a multiple of %d",
+                b->bold_sps);
+    DTB_REQUIRE(bold || !a->bold, "bold output requested but no BOLD integrator is attached");
     if (w.sample && b->ns == 0) {
         set_error("sample outputs requested but no samples are set");
         return DTB_UNSET_SAMPLE_FAILURE;
@@ -887,7 +1069,9 @@ static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
     int st = DTB_OK;
     int64_t done = 0;
     while (done < nsteps && st == DTB_OK) {
-        const int32_t K = (int32_t)std::min<int64_t>(nsteps - done, b->kmax);
+        int64_t kcap = b->kmax;
+        if (bold) kcap -= kcap % b->bold_sps;
+        const int32_t K = (int32_t)std::min<int64_t>(nsteps - done, kcap);
         const float* u = nullptr;
         if (a->noise_mode == DTB_NOISE_UNIFORMS) {
             const float* src = a->uniforms + (size_t)done * (size_t)b->n;
@@ -898,13 +1082,13 @@ static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
                 u = src;
             }
         }
-        const bool graph = a->use_graph && a->noise_mode != DTB_NOISE_UNIFORMS;
+        const bool graph = a->use_graph && a->noise_mode != DTB_NOISE_UNIFORMS && !w.timing;
         if (graph) {
             uint32_t nr;
             std::memcpy(&nr, &a->noise_rate, 4);
             std::vector<uint64_t> key{(uint64_t)K, (uint64_t)a->noise_mode, (uint64_t)a->isolated, nr,
                                       (uint64_t)w.freq, (uint64_t)w.vsum, (uint64_t)w.isum, (uint64_t)w.sample,
-                                      (uint64_t)b->ns};
+                                      (uint64_t)b->ns, (uint64_t)b->bold_sps};
             auto it = b->graphs.find(key);
             if (it == b->graphs.end()) {
                 hipGraph_t g;
@@ -950,6 +1134,24 @@ static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
             break;
         if ((st = copy_out(b, a->events ? a->events + done : nullptr, b->rec_events, sizeof(uint64_t) * K))) break;
         if ((st = copy_out(b, a->spikes ? a->spikes + done : nullptr, b->rec_nspk, sizeof(uint32_t) * K))) break;
+        if (bold && (st = copy_out(b, a->bold ? a->bold + (done / b->bold_sps) * b->nvox : nullptr, b->rec_bold,
+                                   sizeof(double) * (size_t)b->nvox * (size_t)(K / b->bold_sps))))
+            break;
+        if (w.timing) {
+            if (hipStreamSynchronize(b->stream) != hipSuccess) {
+                set_error("step batch failed: %s", hipGetErrorString(hipGetLastError()));
+                st = DTB_RUNTIME_ERROR;
+                break;
+            }
+            for (int32_t k = 0; k < K; ++k) {
+                float t0 = 0.f, t1 = 0.f;
+                hipEventElapsedTime(&t0, b->events[(size_t)(3 * k)], b->events[(size_t)(3 * k + 1)]);
+                hipEventElapsedTime(&t1, b->events[(size_t)(3 * k + 1)], b->events[(size_t)(3 * k + 2)]);
+                float tt[2] = {t0, t1};
+                if ((st = copy_out(b, a->kernel_ms + 2 * (done + k), tt, sizeof(tt)))) break;
+            }
+            if (st) break;
+        }
         if (uhost || K < nsteps - done) {
             // the staging buffer and records are reused by the next batch
             if (hipStreamSynchronize(b->stream) != hipSuccess) {
@@ -1001,13 +1203,12 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
         float* J = field == DTB_FIELD_J ? (float*)tmp : nullptr;
         float* I = field == DTB_FIELD_I_UI ? (float*)tmp : nullptr;
         float* Is = field == DTB_FIELD_I_SYN ? (float*)tmp : nullptr;
-        const uint32_t slot = (uint32_t)(b->steps_done % b->D);
         if (b->pair)
-            hipLaunchKernelGGL(state_kernel<true>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, slot, b->V,
-                               b->Jp, b->ring, b->g, b->vu, J, I, Is);
+            hipLaunchKernelGGL(state_kernel<true>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, b->V, b->Jp,
+                               b->acc, b->g, b->vu, J, I, Is);
         else
-            hipLaunchKernelGGL(state_kernel<false>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, slot, b->V,
-                               b->Jp, b->ring, b->g, b->vu, J, I, Is);
+            hipLaunchKernelGGL(state_kernel<false>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, b->V, b->Jp,
+                               b->acc, b->g, b->vu, J, I, Is);
         st = copy_out(b, dst, tmp, (size_t)sz);
         break;
     }
@@ -1020,16 +1221,16 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
         st = copy_out(b, dst, tmp, (size_t)n);
         break;
     case DTB_FIELD_SPIKE_LIST: {
-        const int par = (int)(b->steps_done & 1);
+        const int64_t slot = b->steps_done % (b->D + 1);
         uint32_t cnt = 0;
         if (b->steps_done > 0) {
-            DTB_TRY_HIP(hipMemcpyAsync(&cnt, b->spk_cnt + par, 4, hipMemcpyDeviceToHost, b->stream));
+            DTB_TRY_HIP(hipMemcpyAsync(&cnt, b->spk_cnt + slot, 4, hipMemcpyDeviceToHost, b->stream));
             DTB_TRY_HIP(hipStreamSynchronize(b->stream));
         }
         DTB_TRY(need(4 * (int64_t)cnt));
         if (cnt) {
             std::vector<int32_t> l(cnt);
-            DTB_TRY_HIP(hipMemcpyAsync(l.data(), b->spk + (int64_t)par * n, 4 * (size_t)cnt, hipMemcpyDeviceToHost,
+            DTB_TRY_HIP(hipMemcpyAsync(l.data(), b->spk + slot * n, 4 * (size_t)cnt, hipMemcpyDeviceToHost,
                                        b->stream));
             DTB_TRY_HIP(hipStreamSynchronize(b->stream));
             std::sort(l.begin(), l.end());
@@ -1203,6 +1404,7 @@ int dtb_block_info(dtb_block* b, int64_t* info)
     info[5] = b->ns;
     info[6] = b->pair ? 1 : 0;
     info[7] = b->device;
+    info[8] = b->nvox;
     return DTB_OK;
 }
 
@@ -1313,6 +1515,109 @@ int dtb_block_replace_props(dtb_block* b, const float* props)
     DTB_TRY_HIP(hipSetDevice(b->device));
     DTB_TRY(upload(b, b->props, props, sizeof(float) * (size_t)b->n * DTB_NPROP));
     return derive_props(b);
+}
+
+int dtb_block_attach_bold(dtb_block* b, const double* par, int32_t steps_per_sample, int32_t nvox,
+                          const int32_t* voxel_of_pop, const double* voxel_size)
+{
+    DTB_REQUIRE(b != nullptr && par != nullptr, "block or par is NULL");
+    DTB_REQUIRE(steps_per_sample >= 1 && steps_per_sample <= b->kmax, "steps_per_sample must be in 1..%lld",
+                (long long)b->kmax);
+    DTB_REQUIRE(b->steps_done % steps_per_sample == 0, "attach the integrator at a multiple of steps_per_sample");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    const int32_t S = b->S;
+    if (!voxel_of_pop) nvox = S;
+    DTB_REQUIRE(nvox >= 1, "nvox must be >= 1");
+    std::vector<int32_t> vop((size_t)S);
+    for (int32_t p = 0; p < S; ++p) vop[(size_t)p] = p;
+    if (voxel_of_pop) DTB_TRY_HIP(hipMemcpy(vop.data(), voxel_of_pop, 4 * (size_t)S, hipMemcpyDefault));
+    std::vector<int32_t> vptr((size_t)nvox + 1, 0), vpops;
+    for (int32_t p = 0; p < S; ++p) {
+        DTB_REQUIRE(vop[(size_t)p] >= -1 && vop[(size_t)p] < nvox, "voxel of sub-population %d out of range", p);
+        if (vop[(size_t)p] >= 0) vptr[(size_t)vop[(size_t)p] + 1]++;
+    }
+    for (int32_t v = 0; v < nvox; ++v) vptr[(size_t)v + 1] += vptr[(size_t)v];
+    vpops.assign((size_t)std::max(vptr[(size_t)nvox], 1), 0);
+    std::vector<int32_t> cur(vptr.begin(), vptr.end() - 1);
+    for (int32_t p = 0; p < S; ++p)
+        if (vop[(size_t)p] >= 0) vpops[(size_t)cur[(size_t)vop[(size_t)p]]++] = p;
+    std::vector<double> vsz((size_t)nvox, 0.0);
+    if (voxel_size) {
+        DTB_TRY_HIP(hipMemcpy(vsz.data(), voxel_size, 8 * (size_t)nvox, hipMemcpyDefault));
+    } else {
+        for (int32_t p = 0; p < S; ++p)
+            if (vop[(size_t)p] >= 0) vsz[(size_t)vop[(size_t)p]] += (double)b->pop_size[(size_t)p];
+    }
+    if (nvox > b->nvox_cap) {
+        DTB_TRY(alloc_n(b, &b->bold_state, 6 * (int64_t)nvox + 8));
+        DTB_TRY(alloc_n(b, &b->rec_bold, b->kmax * (int64_t)nvox));
+        DTB_TRY(alloc_n(b, &b->vox_ptr, (int64_t)nvox + 1));
+        DTB_TRY(alloc_n(b, &b->vox_size, nvox));
+        b->nvox_cap = nvox;
+    }
+    if (!b->vox_pops) DTB_TRY(alloc_n(b, &b->vox_pops, S));
+    std::vector<double> init((size_t)(6 * nvox + 8), 0.0);
+    for (int32_t v = 0; v < nvox; ++v) {
+        init[(size_t)(2 * nvox + v)] = 1.0; // v
+        init[(size_t)(3 * nvox + v)] = 1.0; // f_in
+    }
+    for (int j = 0; j < 8; ++j) init[(size_t)(6 * nvox + j)] = par[j];
+    std::memcpy(b->bold_par, par, sizeof(b->bold_par));
+    DTB_TRY(upload(b, b->bold_state, init.data(), sizeof(double) * init.size()));
+    DTB_TRY(upload(b, b->vox_ptr, vptr.data(), sizeof(int32_t) * vptr.size()));
+    DTB_TRY(upload(b, b->vox_pops, vpops.data(), sizeof(int32_t) * std::min<size_t>(vpops.size(), (size_t)S)));
+    DTB_TRY(upload(b, b->vox_size, vsz.data(), sizeof(double) * vsz.size()));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    b->nvox = nvox;
+    b->bold_sps = steps_per_sample;
+    for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    return DTB_OK;
+}
+
+int dtb_block_bold_state(dtb_block* b, int32_t which, int32_t dir, double* data)
+{
+    DTB_REQUIRE(b != nullptr && data != nullptr, "block or data is NULL");
+    DTB_REQUIRE(b->bold_sps > 0, "no BOLD integrator attached");
+    DTB_REQUIRE(which >= 0 && which < 6 && (dir == 0 || dir == 1), "which must be 0..5 and dir 0 or 1");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    double* src = b->bold_state + (int64_t)which * b->nvox;
+    if (dir == 0)
+        DTB_TRY_HIP(hipMemcpyAsync(data, src, sizeof(double) * (size_t)b->nvox, hipMemcpyDefault, b->stream));
+    else
+        DTB_TRY_HIP(hipMemcpyAsync(src, data, sizeof(double) * (size_t)b->nvox, hipMemcpyDefault, b->stream));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    return DTB_OK;
+}
+
+int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    if (what == DTB_LAUNCH_DELIVER_GRID) {
+        DTB_REQUIRE(value >= 1 && value <= 65536, "delivery grid must be in 1..65536 workgroups");
+        b->deliver_grid = value;
+    } else if (what == DTB_LAUNCH_DELIVER_LANES) {
+        DTB_REQUIRE(value == 16 || value == 32 || value == 64, "lanes per segment must be 16, 32 or 64");
+        b->deliver_lanes = value;
+    } else {
+        set_error("unknown launch parameter %d", what);
+        return DTB_INVALID_PARAMETER;
+    }
+    for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    return DTB_OK;
+}
+
+int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
+{
+    DTB_REQUIRE(b != nullptr && value != nullptr, "block or value is NULL");
+    if (what == DTB_LAUNCH_DELIVER_GRID) *value = b->deliver_grid;
+    else if (what == DTB_LAUNCH_DELIVER_LANES) *value = b->deliver_lanes;
+    else {
+        set_error("unknown launch parameter %d", what);
+        return DTB_INVALID_PARAMETER;
+    }
+    return DTB_OK;
 }
 
 int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx)

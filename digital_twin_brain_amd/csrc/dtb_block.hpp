@@ -45,17 +45,18 @@ struct Block {
     int32_t* last_step = nullptr;  // step of the last threshold spike (0: none)
     double* t_last = nullptr;      // time of the last threshold spike
     uint8_t* flags = nullptr;      // bit0 threshold spike, bit1 delivered spike (last step)
-    void* ring = nullptr;          // [D][n] float2 (E,I) or float4 (AMPA,NMDA,GABAa,GABAb)
+    void* acc = nullptr;           // [n] arrivals of the current step: float2 (E,I) or float4 (AMPA..GABAb)
 
-    // presynaptic-major CSR
+    // presynaptic-major CSR, each row sorted by delay
     int64_t* rowptr = nullptr;
+    uint16_t* dend = nullptr; // [n][D] end of each delay segment relative to the row start (D > 1)
     uint32_t* col = nullptr; // packed (post, delay-1, kind)
     float* w = nullptr;      // compact weights
     float2* w2 = nullptr;    // pair weights
 
-    // spike lists, double-buffered by step parity
-    int32_t* spk = nullptr;     // [2][n]
-    uint32_t* spk_cnt = nullptr; // [2]
+    // spike lists of the last D+1 steps (slot = step mod (D+1))
+    int32_t* spk = nullptr;      // [D+1][n]
+    uint32_t* spk_cnt = nullptr; // [D+1]
 
     StepCtl* ctl = nullptr;
     double* ttab = nullptr; // [kmax] time of each step of the batch
@@ -71,13 +72,26 @@ struct Block {
     float* rec_sv = nullptr;        // [kmax][ns]
     float* pop_inv = nullptr;       // [S] 1/size (NaN for empty)
 
+    // attached BOLD integrator (per sub-population), FP64
+    int32_t bold_sps = 0;
+    double bold_par[8] = {0};
+    int32_t nvox = 0, nvox_cap = 0;
+    double* bold_state = nullptr; // [6][nvox]: s, q, v, f_in, bold, u; then the 8 parameters
+    double* rec_bold = nullptr;   // [kmax][nvox]
+    int32_t* vox_ptr = nullptr;   // [nvox+1] sub-populations of each voxel (CSR)
+    int32_t* vox_pops = nullptr;  // [S]
+    double* vox_size = nullptr;   // [nvox] neurons per voxel (input divisor)
+    double* pop_sized = nullptr;  // [S] sizes as double
+    std::vector<hipEvent_t> events;
+
     uint32_t* samples = nullptr;
     int64_t ns = 0;
 
     std::vector<int64_t> pop_size;
     int64_t steps_done = 0;
     double t_host = 0.0;
-    int32_t deliver_grid = 64;
+    int32_t deliver_grid = 64;  // workgroups of the delivery kernel
+    int32_t deliver_lanes = 64; // lanes per (row, delay) segment
 
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
     std::vector<void*> allocs;

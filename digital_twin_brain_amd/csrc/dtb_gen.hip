@@ -9,10 +9,11 @@
 //   w1    = fmaf(w_hi - w_lo, u24(r3), w_lo)      (pair weights)
 //   kind  = pre >= n_exc
 // Passes: (1) out-degree histogram, (2) exclusive scan -> rowptr,
-// (3) scatter (post, k) keys into their rows, (4) per-row LDS bitonic sort of
-// the keys, then re-draw each synapse from its key and write the packed word
-// and weight.  Sorting makes the CSR independent of atomic ordering, so it is
-// bit-identical to the sequential generator of the oracle.
+// (3) scatter (delay, post, k) keys into their rows, (4) per-row LDS bitonic
+// sort of the keys, then re-draw each synapse from its key and write the
+// packed word, the weight and the row's delay-segment ends.  Rows come out
+// ordered by (delay, post, k): independent of atomic ordering, and equal to
+// the oracle's sequential generator after a stable per-row sort by delay.
 #include <algorithm>
 #include <vector>
 
@@ -138,8 +139,9 @@ __global__ void gen_fill_kernel(GenSpec s, unsigned long long* __restrict__ curs
         const int32_t k = (int32_t)(x - i * s.K);
         const u32x4 r = conn_draw(s, i, k);
         const uint32_t pre = conn_pre(s, i, r.x);
+        const uint64_t d = ((uint64_t)r.z * (uint64_t)s.D) >> 32; // delay - 1
         const unsigned long long pos = atomicAdd(&cursor[pre], 1ull);
-        keys[pos] = ((uint64_t)i << 32) | (uint32_t)k;
+        keys[pos] = (d << 58) | ((uint64_t)i << 32) | (uint32_t)k;
     }
 }
 
@@ -155,6 +157,7 @@ __global__ __launch_bounds__(kSortThreads) void gen_sort_finalize_kernel(GenSpec
                                                                           uint32_t* __restrict__ col,
                                                                           float* __restrict__ w,
                                                                           float2* __restrict__ w2,
+                                                                          uint16_t* __restrict__ dend,
                                                                           uint32_t* __restrict__ too_long)
 {
     __shared__ uint64_t lds[kMaxRow];
@@ -165,7 +168,11 @@ __global__ __launch_bounds__(kSortThreads) void gen_sort_finalize_kernel(GenSpec
             if (threadIdx.x == 0) atomicAdd(too_long, 1u);
             continue;
         }
-        if (len == 0) continue;
+        if (len == 0) {
+            if (dend)
+                for (int32_t dd = threadIdx.x; dd < s.D; dd += kSortThreads) dend[j * s.D + dd] = 0;
+            continue;
+        }
         int32_t P = 1;
         while (P < len) P <<= 1;
         for (int32_t x = threadIdx.x; x < P; x += kSortThreads) lds[x] = x < len ? keys[beg + x] : ~0ull;
@@ -186,12 +193,24 @@ __global__ __launch_bounds__(kSortThreads) void gen_sort_finalize_kernel(GenSpec
             }
         }
         const uint32_t kind = (j >= s.n_exc) ? 1u : 0u;
+        if (dend) {
+            // end of the delay-(dd+1) segment = first key with delay-1 > dd
+            for (int32_t dd = threadIdx.x; dd < s.D; dd += kSortThreads) {
+                int32_t lo = 0, hi = len;
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if ((int32_t)(lds[mid] >> 58) <= dd) lo = mid + 1;
+                    else hi = mid;
+                }
+                dend[j * s.D + dd] = (uint16_t)lo;
+            }
+        }
         for (int32_t x = threadIdx.x; x < len; x += kSortThreads) {
             const uint64_t key = lds[x];
-            const int64_t i = (int64_t)(key >> 32);
+            const int64_t i = (int64_t)((key >> 32) & 0x3FFFFFFull);
             const int32_t k = (int32_t)(uint32_t)key;
+            const uint32_t d = (uint32_t)(key >> 58);
             const u32x4 r = conn_draw(s, i, k);
-            const uint32_t d = (uint32_t)(((uint64_t)r.z * (uint64_t)s.D) >> 32); // delay - 1
             col[beg + x] = (uint32_t)i | (d << pk.idx_bits) | (kind << 31);
             const float a0 = fmaf(s.span, u24(r.y), s.w_lo);
             if (PAIR)
@@ -214,9 +233,15 @@ int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
     const int64_t nnz = b->nnz;
     GenSpec s{n, spec->n_exc, spec->in_degree, spec->max_delay, spec->w_lo, spec->w_hi - spec->w_lo,
               (uint32_t)spec->seed, (uint32_t)(spec->seed >> 32)};
+    DTB_REQUIRE(n <= (1ll << 26), "generated networks are limited to 2^26 neurons");
+    DTB_REQUIRE(spec->max_delay <= 63, "generated networks are limited to delays <= 63");
     void* p = nullptr;
     DTB_TRY(block_alloc(b, &p, sizeof(int64_t) * (size_t)(n + 1)));
     b->rowptr = (int64_t*)p;
+    if (spec->max_delay > 1) {
+        DTB_TRY(block_alloc(b, &p, sizeof(uint16_t) * (size_t)n * (size_t)spec->max_delay));
+        b->dend = (uint16_t*)p;
+    }
     DTB_TRY(block_alloc(b, &p, sizeof(uint32_t) * (size_t)std::max<int64_t>(nnz, 1)));
     b->col = (uint32_t*)p;
     if (b->pair) {
@@ -258,10 +283,10 @@ int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
         const unsigned rg = grid_of(n, 1, 1 << 20);
         if (b->pair)
             hipLaunchKernelGGL(gen_sort_finalize_kernel<true>, dim3(rg), dim3(kSortThreads), 0, b->stream, s, b->pk,
-                               b->rowptr, keys, b->col, b->w, b->w2, flag);
+                               b->rowptr, keys, b->col, b->w, b->w2, b->dend, flag);
         else
             hipLaunchKernelGGL(gen_sort_finalize_kernel<false>, dim3(rg), dim3(kSortThreads), 0, b->stream, s, b->pk,
-                               b->rowptr, keys, b->col, b->w, b->w2, flag);
+                               b->rowptr, keys, b->col, b->w, b->w2, b->dend, flag);
         if ((e = hipGetLastError()) != hipSuccess) { fail(e, "generator launch"); break; }
         uint32_t nlong = 0;
         hipMemcpyAsync(&nlong, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream);

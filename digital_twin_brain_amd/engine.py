@@ -92,7 +92,7 @@ class NativeBlock:
 
     # ---------------------------------------------------------------- info
     def info(self):
-        out = np.zeros(8, dtype=np.int64)
+        out = np.zeros(9, dtype=np.int64)
         nat.check(nat.lib().dtb_block_info(self._h, nat.ptr(out)))
         return [int(x) for x in out]
 
@@ -108,15 +108,59 @@ class NativeBlock:
 
     # ----------------------------------------------------------------- run
     def run(self, nsteps, noise_mode=nat.NOISE_SCALAR, noise_rate=0.0, uniforms=None, isolated=False, graph=True,
-            freqs=None, vmeans=None, imeans=None, sample_spike=None, sample_v=None, events=None, spikes=None):
+            freqs=None, vmeans=None, imeans=None, sample_spike=None, sample_v=None, events=None, spikes=None,
+            bold=None, kernel_ms=None):
         """Advance nsteps; output buffers (numpy or torch, host or device) are filled in place."""
         if uniforms is not None and isinstance(uniforms, np.ndarray):
             uniforms = np.ascontiguousarray(uniforms, dtype=np.float32)
         a = nat.RunArgs(noise_mode=int(noise_mode), noise_rate=float(noise_rate), uniforms=nat.ptr(uniforms),
                         isolated=int(bool(isolated)), use_graph=int(bool(graph)), freqs=nat.ptr(freqs),
                         vmeans=nat.ptr(vmeans), imeans=nat.ptr(imeans), sample_spike=nat.ptr(sample_spike),
-                        sample_v=nat.ptr(sample_v), events=nat.ptr(events), spikes=nat.ptr(spikes))
+                        sample_v=nat.ptr(sample_v), events=nat.ptr(events), spikes=nat.ptr(spikes),
+                        bold=nat.ptr(bold), kernel_ms=nat.ptr(kernel_ms))
         nat.check(nat.lib().dtb_block_run(self._h, int(nsteps), ctypes.byref(a)))
+
+    # ---------------------------------------------------------------- BOLD
+    BOLD_FIELDS = dict(s=0, q=1, v=2, f_in=3, bold=4, u=5)
+
+    def attach_bold(self, epsilon=200, tao_s=0.8, tao_f=0.4, tao_0=1, alpha=0.2, E_0=0.8, V_0=0.02,
+                    delta_t=1e-3, steps_per_sample=1, voxel_of_pop=None, voxel_size=None, nvox=None):
+        """Balloon-Windkessel per voxel fed by its spike counts (dtb_block_attach_bold)."""
+        par = np.array([epsilon, tao_s, tao_f, tao_0, alpha, E_0, V_0, delta_t], dtype=np.float64)
+        vop = None if voxel_of_pop is None else np.ascontiguousarray(voxel_of_pop, dtype=np.int32)
+        vsz = None if voxel_size is None else np.ascontiguousarray(voxel_size, dtype=np.float64)
+        if vop is not None and nvox is None:
+            nvox = int(vop.max()) + 1
+        nat.check(nat.lib().dtb_block_attach_bold(self._h, nat.ptr(par), int(steps_per_sample),
+                                                  int(nvox or self.S), nat.ptr(vop), nat.ptr(vsz)))
+        self.bold_sps = int(steps_per_sample)
+        self.nvox = int(nvox or self.S)
+
+    def bold_state(self, which):
+        out = np.zeros(self.nvox, dtype=np.float64)
+        nat.check(nat.lib().dtb_block_bold_state(self._h, self.BOLD_FIELDS[which], 0, nat.ptr(out)))
+        return out
+
+    def set_bold_state(self, which, values):
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+        if v.shape[0] != self.nvox:
+            raise ValueError(f"need {self.nvox} values")
+        nat.check(nat.lib().dtb_block_bold_state(self._h, self.BOLD_FIELDS[which], 1, nat.ptr(v)))
+
+    # -------------------------------------------------------- launch shape
+    LAUNCH = dict(deliver_grid=0, deliver_lanes=1)
+
+    def set_launch(self, **kw):
+        for k, v in kw.items():
+            nat.check(nat.lib().dtb_block_set_launch(self._h, self.LAUNCH[k], int(v)))
+
+    def get_launch(self):
+        out = {}
+        for k, w in self.LAUNCH.items():
+            v = ctypes.c_int32(0)
+            nat.check(nat.lib().dtb_block_get_launch(self._h, w, ctypes.byref(v)))
+            out[k] = v.value
+        return out
 
     # ----------------------------------------------------------------- get
     def get(self, name, out=None):

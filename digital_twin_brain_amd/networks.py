@@ -1,0 +1,67 @@
+"""Synthetic workloads of the benchmark configurations (seeded).
+
+C1  plain block: N=10,000 (80% E), g_ui ~ U[0.001, 0.05] per neuron and
+    channel, in-degree 100, weights U[0,1], delay 1, noise 10 Hz, 8,000 steps.
+C2  large block: N=5e6, in-degree 1,000 (5e9 synapses, compact int32 index +
+    fp32 weight), delays U{1..10}, per-neuron noise 5-20 Hz, firing 2-20 Hz,
+    10,000 steps, BOLD per 1,000-neuron voxel once per ms.
+Property columns follow generation/make_block.py:394-434.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .engine import NativeBlock
+
+# conductances of the calibrated small block, test/test_generation.py:313-314
+G_CAL = np.array([2.13037975e-02, 1.39240506e-04, 1.58227848e-01, 1.89873418e-02], np.float32)
+
+
+def base_props(n, dt=0.1):
+    p = np.zeros((n, 22), np.float32)
+    p[:, 4] = 1.0    # C
+    p[:, 5] = 2.0    # T_ref (ms)
+    p[:, 6] = 0.03   # g_L
+    p[:, 7] = -75.0  # V_L
+    p[:, 8] = -50.0  # V_th
+    p[:, 9] = -65.0  # V_reset
+    p[:, 14:18] = (0.0, 0.0, -70.0, -100.0)
+    p[:, 18:22] = (2.0, 40.0, 10.0, 50.0)
+    return p
+
+
+def c1_props(n=10_000, seed=2, dt=0.1):
+    rng = np.random.default_rng(seed)
+    p = base_props(n, dt)
+    n_exc = int(round(0.8 * n))
+    p[:, 0] = 10.0 * dt * 1e-3          # 10 Hz -> 1e-3 per step
+    p[:, 3] = (np.arange(n) >= n_exc)   # sub-population 0 = E, 1 = I
+    p[:, 10:14] = rng.uniform(0.001, 0.05, size=(n, 4)).astype(np.float32)
+    return p, n_exc
+
+
+def c2_props(n=5_000_000, seed=2, dt=0.1, voxel=1000):
+    """C2 parameters: inhibition-dominated coupling (0.1 x / 0.05 x the
+    calibrated E / I conductances, +-20% per neuron) with I_ext = 5 gives an
+    asynchronous state at ~8 Hz under in-degree 1,000."""
+    rng = np.random.default_rng(seed)
+    p = base_props(n, dt)
+    n_exc = int(round(0.8 * n))
+    p[:, 0] = rng.uniform(5.0, 20.0, size=n).astype(np.float32) * np.float32(dt * 1e-3)
+    p[:, 2] = 5.0
+    p[:, 3] = np.arange(n) // voxel
+    scale = np.array([0.1, 0.1, 0.05, 0.05], np.float32)
+    p[:, 10:14] = G_CAL * scale * rng.uniform(0.8, 1.2, size=(n, 4)).astype(np.float32)
+    return p, n_exc
+
+
+def build_c1(n=10_000, device=0, conn_seed=1, noise_seed=3):
+    p, ne = c1_props(n)
+    return NativeBlock.fixed_indegree(p, ne, 100, max_delay=1, w_lo=0.0, w_hi=1.0, pair_weights=False,
+                                      conn_seed=conn_seed, noise_seed=noise_seed, device=device), p
+
+
+def build_c2(n=5_000_000, in_degree=1000, device=0, conn_seed=1, noise_seed=3):
+    p, ne = c2_props(n)
+    return NativeBlock.fixed_indegree(p, ne, in_degree, max_delay=10, w_lo=0.0, w_hi=1.0, pair_weights=False,
+                                      conn_seed=conn_seed, noise_seed=noise_seed, device=device), p

@@ -109,12 +109,33 @@ typedef struct {
     float* sample_v;       /* [nsteps][n_samples] V of sampled neurons */
     uint64_t* events;      /* [nsteps] synaptic events delivered (sum of out-degrees of spiking rows) */
     uint32_t* spikes;      /* [nsteps] spiking rows delivered */
+    double* bold;          /* [nsteps/steps_per_sample][nvox] BOLD of the attached integrator */
+    float* kernel_ms;      /* [nsteps][2] device time of the neuron and delivery kernels of each step,
+                              measured with hipEvents on the block's stream (direct launches) */
 } dtb_run_args;
 
 /* nsteps of block.run (models/block.py:116-150) / BrainBlock's step
  * sequence (cuda/tools/dist_simulator.cpp:3822-4114); returns when the
  * outputs are written. */
 int dtb_block_run(dtb_block* b, int64_t nsteps, const dtb_run_args* args);
+
+/* Attach a Balloon-Windkessel integrator over nvox voxels, each the union
+ * of the sub-populations p with voxel_of_pop[p] == voxel (-1: none; NULL:
+ * one voxel per sub-population).  Every steps_per_sample steps,
+ * u = max(threshold spikes of the voxel in the window / voxel_size, 1e-5)
+ * drives one Euler step of par[7] seconds, in FP64 -- the per-step merge and
+ * BOLD call of simulation/simulation.py:269-279 with
+ * models/bold_model_pytorch.py:43-78.  voxel_size NULL: neurons counted in
+ * statistics.  The state starts at s=0, q=0, v=1, f_in=1
+ * (models/bold_model_pytorch.py:60-67).  Runs must then advance a multiple
+ * of steps_per_sample steps. */
+int dtb_block_attach_bold(dtb_block* b, const double* par, int32_t steps_per_sample, int32_t nvox,
+                          const int32_t* voxel_of_pop, const double* voxel_size);
+
+/* Read (dir=0) or write (dir=1) the attached integrator's state:
+ * which = 0 s, 1 q, 2 v, 3 f_in, 4 last BOLD, 5 last input u; [nvox] doubles
+ * (BOLD.state_update, models/bold_model_pytorch.py:34-41, writes 0..3). */
+int dtb_block_bold_state(dtb_block* b, int32_t which, int32_t dir, double* data);
 
 /* State fields for dtb_block_get. */
 #define DTB_FIELD_V 0          /* float[n] */
@@ -139,8 +160,8 @@ int dtb_block_run(dtb_block* b, int64_t nsteps, const dtb_run_args* args);
  * byte count of the list is returned through *bytes_out. */
 int dtb_block_get(dtb_block* b, int32_t field, void* dst, int64_t bytes, int64_t* bytes_out);
 
-/* info[0..7] = n, nnz, S (sub-populations), max delay, steps done,
- * n_samples, pair weights (0/1), device. */
+/* info[0..8] = n, nnz, S (sub-populations), max delay, steps done,
+ * n_samples, pair weights (0/1), device, BOLD voxels. */
 int dtb_block_info(dtb_block* b, int64_t* info);
 /* Simulated time t (models/block.py:58,133), accumulated in double. */
 int dtb_block_time(dtb_block* b, double* t);
@@ -166,6 +187,15 @@ int dtb_block_replace_props(dtb_block* b, const float* props);
 /* Sampled neurons for sample_spike / sample_v (BlockWrapper.set_samples,
  * cuda/python/dist_blockwrapper_pytorch.py:505-528). */
 int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx);
+
+/* Launch shape of the delivery kernel (no counterpart in the reference,
+ * whose launch shapes are fixed, cuda/include/common.hpp:70-71): the
+ * workgroup count and the lanes that share one (row, delay) segment.  Used
+ * by the autotuner; results do not depend on them. */
+#define DTB_LAUNCH_DELIVER_GRID 0  /* 1..65536 */
+#define DTB_LAUNCH_DELIVER_LANES 1 /* 16, 32 or 64 */
+int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
+int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 
 /* Balloon-Windkessel forward-Euler step on device arrays of n voxels,
  * models/bold_model_pytorch.py:43-78.  par = {epsilon, tao_s, tao_f, tao_0,
