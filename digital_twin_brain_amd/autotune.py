@@ -17,6 +17,12 @@ import os
 
 import numpy as np
 
+def nat_error():
+    from ._native import DTBError
+
+    return DTBError
+
+
 CACHE = os.environ.get("DTB_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "dtb_tune.json"))
 
 
@@ -57,23 +63,33 @@ def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
     cache = _load() if use_cache else {}
     if key in cache:
         best = cache[key]
-        nb.set_launch(deliver_lanes=best["deliver_lanes"], deliver_grid=best["deliver_grid"])
+        nb.set_launch(deliver_mode=best["deliver_mode"], deliver_lanes=best["deliver_lanes"],
+                      deliver_grid=best["deliver_grid"])
         return best, {"cached": key}
     segs = max(64.0, float(spk.mean()) * nb.max_delay)
-    table = {}
-    for lanes in (64, 32, 16):
-        groups_per_wg = 4 * (64 // lanes)
-        base = max(4, int(math.ceil(segs / groups_per_wg)))
+    cands = []
+    for lanes in (64, 32, 16):  # direct atomics: one lane group per segment
+        base = max(4, int(math.ceil(segs / (4 * (64 // lanes)))))
         for mult in (0.25, 0.5, 1.0, 2.0):
-            grid = int(min(65536, max(4, base * mult)))
-            nb.set_launch(deliver_lanes=lanes, deliver_grid=grid)
-            nb.run(steps, graph=False, kernel_ms=kms, **run_kwargs)
-            table[f"{lanes}/{grid}"] = float(kms[:, 1].mean()) * 1e3
-            if log:
-                log(f"tune lanes={lanes} grid={grid}: deliver {table[f'{lanes}/{grid}']:.1f} us")
-    lanes, grid = min(table, key=table.get).split("/")
-    best = {"deliver_lanes": int(lanes), "deliver_grid": int(grid), "us": table[f"{lanes}/{grid}"]}
-    nb.set_launch(deliver_lanes=best["deliver_lanes"], deliver_grid=best["deliver_grid"])
+            cands.append((0, lanes, int(min(65536, max(4, base * mult)))))
+    if nb.n <= (1 << 26):  # bucketed: workgroups each expanding 256 segments at a time
+        chunks = max(1, int(math.ceil(segs / 256)))
+        for grid in sorted({max(1, chunks // 4), max(1, chunks // 2), chunks, 256, 1024}):
+            cands.append((1, 64, grid))
+    table = {}
+    for mode, lanes, grid in cands:
+        try:
+            nb.set_launch(deliver_mode=mode, deliver_lanes=lanes, deliver_grid=grid)
+        except nat_error():
+            continue
+        nb.run(steps, graph=False, kernel_ms=kms, **run_kwargs)
+        key = f"{mode}/{lanes}/{grid}"
+        table[key] = float(kms[:, 1].mean()) * 1e3
+        if log:
+            log(f"tune mode={mode} lanes={lanes} grid={grid}: deliver {table[key]:.1f} us")
+    mode, lanes, grid = (int(x) for x in min(table, key=table.get).split("/"))
+    best = {"deliver_mode": mode, "deliver_lanes": lanes, "deliver_grid": grid, "us": table[f"{mode}/{lanes}/{grid}"]}
+    nb.set_launch(deliver_mode=mode, deliver_lanes=lanes, deliver_grid=grid)
     if use_cache:
         cache[key] = best
         _save(cache)

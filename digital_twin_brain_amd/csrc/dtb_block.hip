@@ -94,7 +94,8 @@ struct NeuronArgs {
     void* __restrict__ acc;
     int32_t* __restrict__ spk;
     uint32_t* __restrict__ spk_cnt;
-    int32_t L; // spike-list slots (D + 1)
+    int32_t L;         // spike-list slots (D + 1)
+    int32_t clear_acc; // direct delivery accumulates into acc, so it is cleared after reading
     const StepCtl* __restrict__ ctl;
     const double* __restrict__ ttab;
     double dt;
@@ -144,12 +145,12 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
         if (PAIR) {
             float4* r = reinterpret_cast<float4*>(a.acc) + i;
             const float4 d = *r;
-            *r = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (a.clear_acc) *r = make_float4(0.f, 0.f, 0.f, 0.f);
             J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
         } else {
             float2* r = reinterpret_cast<float2*>(a.acc) + i;
             const float2 d = *r;
-            *r = make_float2(0.f, 0.f);
+            if (a.clear_acc) *r = make_float2(0.f, 0.f);
             J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
         }
         const float4 g = a.g[i];
@@ -324,6 +325,282 @@ static void launch_deliver(const Block* b, const DeliverArgs& da, int32_t k)
     case 16: hipLaunchKernelGGL((deliver_kernel<PAIR, 16>), g, t, 0, b->stream, da, k); break;
     case 32: hipLaunchKernelGGL((deliver_kernel<PAIR, 32>), g, t, 0, b->stream, da, k); break;
     default: hipLaunchKernelGGL((deliver_kernel<PAIR, 64>), g, t, 0, b->stream, da, k); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bucketed delivery.  Random float atomics on MI355X saturate near 2e10/s
+// whatever the footprint, scope or XCD placement, so a large step's events
+// are instead sorted by destination bucket (BucketCfg::kSize consecutive
+// neurons, one 128 KB LDS slice) and accumulated in LDS:
+//   phase 1 (deliver_stage_kernel): each workgroup expands 256 (row, delay)
+//     segments at a time into rounds of 256*kEPT events held in registers;
+//     an LDS histogram ranks every event inside its bucket, one global atomic
+//     per touched bucket reserves space, and events are written to that
+//     bucket's staging array (bucket-major, contiguous per bucket).
+//   phase 2 (deliver_accum_kernel): one workgroup per bucket sums its staged
+//     events into the LDS slice and writes the bucket's slice of acc.
+// A bucket whose staging array is full takes the rest of its events through
+// global atomics into acc2, which phase 2 then folds in.
+template <bool PAIR>
+struct BucketCfg {
+    static constexpr int kBits = PAIR ? 13 : 14;     // 8192 / 16384 neurons per bucket
+    static constexpr int kSize = 1 << kBits;
+    static constexpr int kChan = PAIR ? 4 : 2;       // floats per neuron in acc
+    static constexpr int kEPT = PAIR ? 16 : 32;      // events per lane per round
+    static constexpr size_t kLds = (size_t)kSize * kChan * sizeof(float); // 128 KB
+};
+constexpr int kSegPerChunk = 256;
+constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
+constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
+
+struct StageArgs {
+    DeliverArgs d;
+    int32_t nbk;      // buckets
+    int64_t cap;      // staged events per bucket
+    uint32_t* cursor; // [nbk]
+    void* stage;      // [nbk][cap] uint2 (compact) / uint4 (pair)
+    void* acc2;       // overflow accumulator [n]
+    uint32_t* ovf;    // [nbk] overflow flags
+};
+
+// Inclusive scan over the 256 lanes of a workgroup (shuffles within each
+// wavefront, then the four wavefront totals through LDS).
+__device__ inline uint32_t block_inclusive_scan_256(uint32_t v, uint32_t* wtot)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += y;
+    }
+    if (lane == kWave - 1) wtot[wid] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wid; ++w) add += wtot[w];
+    __syncthreads();
+    return v + add;
+}
+
+template <bool PAIR>
+__global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t k)
+{
+    using Cfg = BucketCfg<PAIR>;
+    constexpr int EPT = Cfg::kEPT;
+    __shared__ uint32_t cum[DTB_MAX_DELAY + 1];
+    __shared__ int32_t slot_of[DTB_MAX_DELAY + 1];
+    __shared__ uint32_t segpre[kSegPerChunk + 1];
+    __shared__ int64_t segbeg[kSegPerChunk];
+    __shared__ uint32_t hist[kMaxBucketsLds];
+    __shared__ uint32_t wtot[4];
+    const DeliverArgs& a = s.d;
+    const int64_t step = a.ctl->nbase + k + 1;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        uint32_t c = 0;
+        cum[0] = 0;
+        for (int32_t d = 1; d <= a.D; ++d) {
+            const int32_t sl = (int32_t)((step - d + 1 + a.L) % a.L);
+            slot_of[d] = sl;
+            c += min(a.spk_cnt[sl], (uint32_t)a.n);
+            cum[d] = c;
+        }
+    }
+    for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
+    __syncthreads();
+    const uint32_t total = cum[a.D];
+    if (blockIdx.x == 0 && t == 0 && a.rec_nspk) a.rec_nspk[k] = cum[1];
+    const uint32_t nchunks = (total + kSegPerChunk - 1) / kSegPerChunk;
+    uint64_t ev_wg = 0;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        // segment lengths of this chunk -> LDS prefix
+        const uint32_t g = c * kSegPerChunk + t;
+        uint32_t len = 0;
+        int64_t beg = 0;
+        if (g < total) {
+            int32_t d = 1;
+            while (g >= cum[d]) ++d;
+            const int32_t j = a.spk[(int64_t)slot_of[d] * a.n + (g - cum[d - 1])];
+            const int64_t row = a.rowptr[j];
+            int64_t end;
+            if (a.D == 1) {
+                beg = row;
+                end = a.rowptr[j + 1];
+            } else {
+                const uint16_t* de = a.dend + (int64_t)j * a.D;
+                beg = row + (d > 1 ? de[d - 2] : 0);
+                end = row + de[d - 1];
+            }
+            len = (uint32_t)(end - beg);
+        }
+        segbeg[t] = beg;
+        segpre[t + 1] = block_inclusive_scan_256(len, wtot);
+        if (t == 0) segpre[0] = 0;
+        __syncthreads();
+        const uint32_t E = segpre[kSegPerChunk];
+        ev_wg += E;
+        for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
+            uint32_t br[EPT], pk_[EPT]; // br = bucket << kRankBits | rank
+            float w0[EPT], w1[EPT];
+            // (a) where each event lives: segment (8 bits) and offset in it (24 bits), from LDS only
+            uint32_t loc[EPT];
+            int sg = 0;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const uint32_t e = r0 + (uint32_t)m * 256 + t;
+                loc[m] = 0xffffffffu;
+                if (e < E) {
+                    while (segpre[sg + 1] <= e) ++sg; // e grows with m
+                    loc[m] = ((uint32_t)sg << 24) | (e - segpre[sg]);
+                }
+            }
+            // (b) all loads in flight together
+            uint32_t cw[EPT];
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                cw[m] = 0u;
+                w0[m] = 0.0f;
+                w1[m] = 0.0f;
+                if (loc[m] != 0xffffffffu) {
+                    const int64_t idx = segbeg[loc[m] >> 24] + (loc[m] & 0xffffffu);
+                    cw[m] = a.col[idx];
+                    if (PAIR) {
+                        const float2 wv = a.w2[idx];
+                        w0[m] = wv.x;
+                        w1[m] = wv.y;
+                    } else {
+                        w0[m] = a.w[idx];
+                    }
+                }
+            }
+            // (c) bucket and rank inside the bucket
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                br[m] = 0xffffffffu;
+                if (loc[m] != 0xffffffffu) {
+                    const uint32_t post = cw[m] & a.pk.idx_mask;
+                    const uint32_t bk = post >> Cfg::kBits;
+                    pk_[m] = (post & (Cfg::kSize - 1)) | (cw[m] & 0x80000000u);
+                    br[m] = (bk << kRankBits) | atomicAdd(&hist[bk], 1u);
+                }
+            }
+            __syncthreads();
+            // reserve space in each touched bucket; hist[b] becomes the base
+            for (int b = t; b < s.nbk; b += 256) {
+                const uint32_t h = hist[b];
+                hist[b] = h ? atomicAdd(&s.cursor[b], h) : 0u;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                if (br[m] == 0xffffffffu) continue;
+                const uint32_t bk = br[m] >> kRankBits;
+                const uint64_t pos = (uint64_t)hist[bk] + (br[m] & ((1u << kRankBits) - 1));
+                if (pos < (uint64_t)s.cap) {
+                    const int64_t at = (int64_t)bk * s.cap + (int64_t)pos;
+                    if (PAIR)
+                        reinterpret_cast<uint4*>(s.stage)[at] =
+                            make_uint4(pk_[m], __float_as_uint(w0[m]), __float_as_uint(w1[m]), 0u);
+                    else
+                        reinterpret_cast<uint2*>(s.stage)[at] = make_uint2(pk_[m], __float_as_uint(w0[m]));
+                } else {
+                    const int64_t post = ((int64_t)bk << Cfg::kBits) + (pk_[m] & (Cfg::kSize - 1));
+                    const uint32_t kind = pk_[m] >> 31;
+                    if (PAIR) {
+                        float* dst = reinterpret_cast<float*>(s.acc2) + (post << 2) + (kind << 1);
+                        unsafeAtomicAdd(dst, w0[m]);
+                        unsafeAtomicAdd(dst + 1, w1[m]);
+                    } else {
+                        unsafeAtomicAdd(reinterpret_cast<float*>(s.acc2) + (post << 1) + kind, w0[m]);
+                    }
+                    s.ovf[bk] = 1u;
+                }
+            }
+            __syncthreads();
+            for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
+            __syncthreads();
+        }
+    }
+    if (t == 0 && ev_wg && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev_wg);
+}
+
+template <bool PAIR>
+__global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
+{
+    using Cfg = BucketCfg<PAIR>;
+    constexpr int C = Cfg::kChan;
+    extern __shared__ float slice[]; // [kSize][C]
+    const DeliverArgs& a = s.d;
+    const int32_t b = blockIdx.x;
+    const int t = threadIdx.x;
+    for (int x = t; x < Cfg::kSize * C; x += blockDim.x) slice[x] = 0.0f;
+    const uint32_t cnt = (uint32_t)min((uint64_t)s.cursor[b], (uint64_t)s.cap);
+    const bool ovf = s.ovf[b] != 0u;
+    __syncthreads();
+    constexpr int U = 8; // staged events in flight per lane
+    const int64_t sb = (int64_t)b * s.cap;
+    for (uint32_t base = t; base < cnt; base += blockDim.x * U) {
+        if (PAIR) {
+            uint4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t e = base + (uint32_t)u * blockDim.x;
+                v[u] = e < cnt ? reinterpret_cast<const uint4*>(s.stage)[sb + e] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (base + (uint32_t)u * blockDim.x >= cnt) break;
+                const uint32_t o = (v[u].x & (Cfg::kSize - 1)) * 4 + ((v[u].x >> 31) << 1);
+                atomicAdd(&slice[o], __uint_as_float(v[u].y));
+                atomicAdd(&slice[o + 1], __uint_as_float(v[u].z));
+            }
+        } else {
+            uint2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t e = base + (uint32_t)u * blockDim.x;
+                v[u] = e < cnt ? reinterpret_cast<const uint2*>(s.stage)[sb + e] : make_uint2(0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (base + (uint32_t)u * blockDim.x >= cnt) break;
+                atomicAdd(&slice[(v[u].x & (Cfg::kSize - 1)) * 2 + (v[u].x >> 31)], __uint_as_float(v[u].y));
+            }
+        }
+    }
+    __syncthreads();
+    const int64_t i0 = (int64_t)b << Cfg::kBits;
+    const int32_t nloc = (int32_t)min((int64_t)Cfg::kSize, a.n - i0);
+    if (PAIR) {
+        float4* acc = reinterpret_cast<float4*>(a.acc) + i0;
+        float4* acc2 = reinterpret_cast<float4*>(s.acc2) + i0;
+        for (int x = t; x < nloc; x += blockDim.x) {
+            float4 v = make_float4(slice[4 * x], slice[4 * x + 1], slice[4 * x + 2], slice[4 * x + 3]);
+            if (ovf) {
+                const float4 o = acc2[x];
+                v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+                acc2[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            acc[x] = v;
+        }
+    } else {
+        float2* acc = reinterpret_cast<float2*>(a.acc) + i0;
+        float2* acc2 = reinterpret_cast<float2*>(s.acc2) + i0;
+        for (int x = t; x < nloc; x += blockDim.x) {
+            float2 v = make_float2(slice[2 * x], slice[2 * x + 1]);
+            if (ovf) {
+                const float2 o = acc2[x];
+                v.x += o.x; v.y += o.y;
+                acc2[x] = make_float2(0.f, 0.f);
+            }
+            acc[x] = v;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        s.cursor[b] = 0u;
+        s.ovf[b] = 0u;
     }
 }
 
@@ -730,6 +1007,8 @@ static int alloc_state(Block* b)
     return DTB_OK;
 }
 
+static int ensure_bucketed(Block* b);
+
 static int alloc_ring_and_records(Block* b)
 {
     const size_t esz = b->pair ? sizeof(float4) : sizeof(float2);
@@ -745,6 +1024,42 @@ static int alloc_ring_and_records(Block* b)
     b->deliver_lanes = avg_seg >= 48 ? 64 : avg_seg >= 24 ? 32 : 16;
     const double groups_per_wg = 4.0 * (64 / b->deliver_lanes);
     b->deliver_grid = (int32_t)std::min<double>(16384.0, std::max(4.0, std::ceil(segs / groups_per_wg)));
+    if (b->n >= (1 << 17) && b->n <= (1ll << 26)) {
+        b->deliver_mode = 1;
+        b->deliver_grid = 1024;
+        DTB_TRY(ensure_bucketed(b));
+    }
+    return DTB_OK;
+}
+
+// Staging buffers of the bucketed delivery, allocated on first use.
+static int ensure_bucketed(Block* b)
+{
+    if (b->stage) return DTB_OK;
+    DTB_REQUIRE(b->n <= (1ll << 26), "bucketed delivery supports n <= 2^26");
+    DTB_REQUIRE(b->max_row < (1ll << 24), "bucketed delivery supports rows of fewer than 2^24 synapses");
+    static bool attr_done = false;
+    if (!attr_done) {
+        DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)BucketCfg<true>::kLds));
+        DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)BucketCfg<false>::kLds));
+        attr_done = true;
+    }
+    const int64_t bsz = b->pair ? BucketCfg<true>::kSize : BucketCfg<false>::kSize;
+    b->nbk = (int32_t)((b->n + bsz - 1) / bsz);
+    // room for 1% of all synapses per step (a mean rate of 100 Hz at 0.1 ms),
+    // at least 8192 events per bucket; fuller buckets fall back to atomics
+    b->stage_cap = std::max<int64_t>(8192, (b->nnz / 100 + b->nbk - 1) / b->nbk);
+    const size_t esz = b->pair ? 16 : 8;
+    DTB_TRY(alloc_n(b, &b->cursor, b->nbk));
+    DTB_TRY(alloc_n(b, &b->ovf, b->nbk));
+    DTB_TRY(block_alloc(b, &b->stage, esz * (size_t)b->nbk * (size_t)b->stage_cap));
+    DTB_TRY(block_alloc(b, &b->acc2, (b->pair ? 16 : 8) * (size_t)b->n));
+    DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)b->nbk, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)b->nbk, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->acc2, 0, (b->pair ? 16 : 8) * (size_t)b->n, b->stream));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     return DTB_OK;
 }
 
@@ -821,8 +1136,10 @@ static int create_from_csr(Block* b, const dtb_block_desc* d)
     std::vector<int64_t> rp((size_t)n + 1);
     DTB_TRY_HIP(hipMemcpy(rp.data(), d->rowptr, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyDefault));
     DTB_REQUIRE(rp[0] == 0 && rp[(size_t)n] == nnz, "rowptr must start at 0 and end at nnz");
-    for (int64_t j = 0; j < n; ++j)
+    for (int64_t j = 0; j < n; ++j) {
         DTB_REQUIRE(rp[(size_t)j + 1] >= rp[(size_t)j], "rowptr must be non-decreasing (row %lld)", (long long)j);
+        b->max_row = std::max<int64_t>(b->max_row, rp[(size_t)j + 1] - rp[(size_t)j]);
+    }
     int32_t D = 1;
     if (d->delay && nnz > 0) {
         std::vector<uint8_t> dl((size_t)nnz);
@@ -953,6 +1270,7 @@ static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* unif
     x.vrst = b->vrst; x.iext = b->iext; x.tref = b->tref; x.noisep = b->noisep; x.pop = b->pop;
     x.V = b->V; x.Jp = b->Jp; x.ref_end = b->ref_end; x.last_step = b->last_step; x.t_last = b->t_last;
     x.flags = b->flags; x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.L = b->D + 1; x.ctl = b->ctl;
+    x.clear_acc = b->deliver_mode == 0;
     x.ttab = b->ttab;
     x.dt = b->dt; x.seed = b->seed;
     x.noise_mode = a->noise_mode; x.noise_rate = a->noise_rate; x.uniforms = uniforms; x.isolated = a->isolated;
@@ -999,10 +1317,22 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         else
             hipLaunchKernelGGL(neuron_step_kernel<false>, dim3(ng), dim3(256), 0, b->stream, na, k);
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
-        if (b->pair)
+        if (b->deliver_mode == 1) {
+            StageArgs sa{da, b->nbk, b->stage_cap, b->cursor, b->stage, b->acc2, b->ovf};
+            if (b->pair) {
+                hipLaunchKernelGGL(deliver_stage_kernel<true>, dim3(b->deliver_grid), dim3(256), 0, b->stream, sa, k);
+                hipLaunchKernelGGL(deliver_accum_kernel<true>, dim3(b->nbk), dim3(1024), BucketCfg<true>::kLds,
+                                   b->stream, sa);
+            } else {
+                hipLaunchKernelGGL(deliver_stage_kernel<false>, dim3(b->deliver_grid), dim3(256), 0, b->stream, sa, k);
+                hipLaunchKernelGGL(deliver_accum_kernel<false>, dim3(b->nbk), dim3(1024), BucketCfg<false>::kLds,
+                                   b->stream, sa);
+            }
+        } else if (b->pair) {
             launch_deliver<true>(b, da, k);
-        else
+        } else {
             launch_deliver<false>(b, da, k);
+        }
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 2)], b->stream));
         if (w.isum) {
             if (b->pair)
@@ -1363,6 +1693,7 @@ int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_sp
         b->pair = spec->pair_weights != 0;
         b->nnz = spec->n * (int64_t)spec->in_degree;
         if ((st = make_pack(b, spec->max_delay))) break;
+        b->max_row = 8192; // the generator's row limit
         if ((st = alloc_state(b))) break;
         if ((st = gen_fixed_indegree(b, spec))) break;
         st = finish_create(b, props);
@@ -1599,6 +1930,10 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
     } else if (what == DTB_LAUNCH_DELIVER_LANES) {
         DTB_REQUIRE(value == 16 || value == 32 || value == 64, "lanes per segment must be 16, 32 or 64");
         b->deliver_lanes = value;
+    } else if (what == DTB_LAUNCH_DELIVER_MODE) {
+        DTB_REQUIRE(value == 0 || value == 1, "delivery mode must be 0 (atomics) or 1 (bucketed)");
+        if (value == 1) DTB_TRY(ensure_bucketed(b));
+        b->deliver_mode = value;
     } else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;
@@ -1613,6 +1948,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     DTB_REQUIRE(b != nullptr && value != nullptr, "block or value is NULL");
     if (what == DTB_LAUNCH_DELIVER_GRID) *value = b->deliver_grid;
     else if (what == DTB_LAUNCH_DELIVER_LANES) *value = b->deliver_lanes;
+    else if (what == DTB_LAUNCH_DELIVER_MODE) *value = b->deliver_mode;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

@@ -22,6 +22,7 @@ struct Block {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t n = 0, nnz = 0;
+    int64_t max_row = 0; // longest presynaptic row
     int32_t S = 0;    // statistics sub-populations: max(sub_idx)+1
     int32_t D = 1;    // ring depth = largest delay
     bool pair = false; // float2 weights (else one weight per synapse)
@@ -92,6 +93,13 @@ struct Block {
     double t_host = 0.0;
     int32_t deliver_grid = 64;  // workgroups of the delivery kernel
     int32_t deliver_lanes = 64; // lanes per (row, delay) segment
+    int32_t deliver_mode = 0;   // 0: global atomics, 1: bucketed (staged + LDS accumulation)
+    int32_t nbk = 0;            // destination buckets
+    int64_t stage_cap = 0;      // staged events per bucket
+    uint32_t* cursor = nullptr; // [nbk]
+    uint32_t* ovf = nullptr;    // [nbk]
+    void* stage = nullptr;      // [nbk][stage_cap]
+    void* acc2 = nullptr;       // [n] overflow accumulator
 
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
     std::vector<void*> allocs;

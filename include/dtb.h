@@ -190,10 +190,12 @@ int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx);
 
 /* Launch shape of the delivery kernel (no counterpart in the reference,
  * whose launch shapes are fixed, cuda/include/common.hpp:70-71): the
- * workgroup count and the lanes that share one (row, delay) segment.  Used
- * by the autotuner; results do not depend on them. */
+ * workgroup count, the lanes that share one (row, delay) segment and the
+ * delivery mode.  Used by the autotuner; results do not depend on them
+ * beyond floating-point summation order. */
 #define DTB_LAUNCH_DELIVER_GRID 0  /* 1..65536 */
 #define DTB_LAUNCH_DELIVER_LANES 1 /* 16, 32 or 64 */
+#define DTB_LAUNCH_DELIVER_MODE 2  /* 0: global atomics, 1: bucketed LDS accumulation */
 int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
 int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 

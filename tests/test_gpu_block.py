@@ -392,11 +392,11 @@ def test_results_independent_of_delivery_launch_shape():
     ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, False, seed=8)
     ref["w0"] = (np.floor(ref["w0"] * 256) / 256).astype(np.float32)
     outs = []
-    for lanes, grid in ((64, 4), (32, 37), (16, 300), (64, 2048)):
+    for mode, lanes, grid in ((0, 64, 4), (0, 32, 37), (0, 16, 300), (0, 64, 2048), (1, 64, 1), (1, 64, 300)):
         nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"],
                                   dt=0.1, noise_seed=2)
-        nb.set_launch(deliver_lanes=lanes, deliver_grid=grid)
-        assert nb.get_launch() == {"deliver_grid": grid, "deliver_lanes": lanes}
+        nb.set_launch(deliver_lanes=lanes, deliver_grid=grid, deliver_mode=mode)
+        assert nb.get_launch() == {"deliver_grid": grid, "deliver_lanes": lanes, "deliver_mode": mode}
         ev = np.zeros(300, np.uint64)
         nb.run(300, noise_rate=0.02, graph=True, events=ev)
         outs.append((nb.get("V"), nb.get("J"), ev))
@@ -404,3 +404,32 @@ def test_results_independent_of_delivery_launch_shape():
         for x, y in zip(outs[0], o):
             assert np.array_equal(x, y)
     assert outs[0][2].sum() > 0
+
+
+@pytest.mark.parametrize("pair", [False, True])
+def test_bucketed_delivery_matches_atomics_including_overflow(pair):
+    """Bucketed delivery (staging + LDS sums, with the atomic fallback of full
+    buckets) against direct atomics: dense activity across several buckets,
+    dyadic weights, so both must agree bitwise."""
+    from digital_twin_brain_amd import NativeBlock
+
+    n, ne, K, D = 20000, 16000, 400, 3
+    rng = np.random.default_rng(9)
+    props = _c1_like_props(n, ne, rng, g_scale=0.2)
+    ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, pair, seed=10)
+    for key in ("w0", "w1"):
+        if ref.get(key) is not None:
+            ref[key] = (np.floor(ref[key] * 64) / 64).astype(np.float32)
+    outs = []
+    for mode in (0, 1):
+        nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], ref["w1"] if pair else None,
+                                  ref["delay"], dt=0.1, noise_seed=4)
+        nb.set_launch(deliver_mode=mode, deliver_grid=512)
+        ev = np.zeros(40, np.uint64)
+        # 20 steps of heavy noise (up to ~10^7 events per step: buckets overflow), then 20 light ones
+        nb.run(20, noise_rate=0.25, graph=True, events=ev[:20])
+        nb.run(20, noise_rate=0.01, graph=False, events=ev[20:])
+        outs.append((nb.get("V"), nb.get("J"), ev))
+    for x, y in zip(outs[0], outs[1]):
+        assert np.array_equal(x, y)
+    assert outs[0][2][:20].max() > 3 * 8192 * 3  # some bucket saw more events than its staging room
