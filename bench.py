@@ -29,9 +29,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-B_NEURON_COMPACT = 141  # bytes per neuron per step of neuron_step_kernel (compact weights, per-neuron noise)
-B_SPIKE_ROW = 20        # rowptr begin/end (16) + spike-list entry (4) per delivered row
-B_SYNAPSE = 8           # packed int32 target + fp32 weight per synaptic event
+B_SEGMENT = 16  # rowptr begin (8) + delay-segment ends (4) + spike-list entry (4) per delivered (row, delay)
+B_SYNAPSE = 8   # packed int32 target + fp32 weight per synaptic event
+
+
+def neuron_bytes(launch, per_neuron_noise=True, pair=False):
+    """Bytes per neuron per step that neuron_step_kernel reads and writes.
+
+    Dynamic state: V 4, Jp 16, acc 8 (16 pair), ref_end 4 read; V 4, Jp 16,
+    flags 1 written (+ the acc clear with direct delivery).  Constants: g 16,
+    pop 4, noise p 4, and either a 1-byte parameter-set id or 56 B of
+    per-neuron V_u, decay, g_L, V_L, dt/C, V_th, V_reset, I_ext."""
+    acc = 16 if pair else 8
+    b = 4 + 16 + acc + 4 + 4 + 16 + 1 + 16 + 4 + (4 if per_neuron_noise else 0)
+    b += 1 if launch.get("neuron_psets", 0) > 0 else 56
+    if launch.get("deliver_mode", 0) == 0:
+        b += acc
+    return b
 
 
 def log(*a):
@@ -85,6 +99,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--da-windows", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the single-GPU build has no multi-GPU path "
@@ -133,22 +148,37 @@ def main():
     nb.run(M, noise_mode=1, graph=False, events=ev_w, spikes=sp_w, kernel_ms=kms)
     t_neu = float(kms[:, 0].mean()) * 1e-3
     t_del = float(kms[:, 1].mean()) * 1e-3
-    b_neu = B_NEURON_COMPACT * nb.n
-    b_del = float((B_SPIKE_ROW * sp_w.astype(np.float64) + B_SYNAPSE * ev_w.astype(np.float64)).mean())
+    launch = nb.get_launch()
+    bpn = neuron_bytes(launch)
+    b_neu = bpn * nb.n
+    # delivered (row, delay) segments: every spike of the last D steps contributes one
+    seg = np.convolve(sp_w.astype(np.float64), np.ones(nb.max_delay))[:M]
+    b_del = float((B_SEGMENT * seg + B_SYNAPSE * ev_w.astype(np.float64))[nb.max_delay:].mean())
     if t_neu >= t_del:
         dom, bytes_l, tl = "neuron_step_kernel", b_neu, t_neu
     else:
         dom, bytes_l, tl = "deliver_kernel", b_del, t_del
     achieved = bytes_l / tl / 1e9
-    kernels = {"neuron_step_kernel": {"avg_us": t_neu * 1e6, "bytes_per_launch": b_neu,
+    kernels = {"neuron_step_kernel": {"avg_us": t_neu * 1e6, "bytes_per_launch": b_neu, "bytes_per_neuron": bpn,
                                       "GBps": b_neu / t_neu / 1e9},
                "deliver_kernel": {"avg_us": t_del * 1e6, "bytes_per_launch": b_del, "GBps": b_del / t_del / 1e9,
                                   "events_per_launch": float(ev_w.mean()),
                                   "atomic_rmw_bytes_per_launch": 8.0 * float(ev_w.mean())}}
     log(f"kernels: {json.dumps(kernels)}")
-    launch = nb.get_launch()
     nb.close()
     del nb
+    torch.cuda.empty_cache()
+
+    da = None
+    if a.da_windows > 0:
+        from digital_twin_brain_amd.da_runner import run_c4
+
+        t0 = time.perf_counter()
+        da, dao, _ = run_c4(E=64, voxels=90, windows=a.da_windows, window_ms=800.0, dt=0.1, log=log)
+        dao.block_model.nb.close()
+        del dao
+        da["total_s"] = time.perf_counter() - t0
+        da["mode"] = "full simulation (every window simulated)"
 
     cpu = None
     if not a.no_cpu:
@@ -169,6 +199,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernels": kernels,
         "launch": launch,
+        "da_c4": da,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

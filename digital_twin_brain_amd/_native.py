@@ -52,6 +52,10 @@ class FixedIndegreeSpec(ctypes.Structure):
                 ("w_hi", _f32), ("pair_weights", _i32), ("seed", _u64)]
 
 
+class MemberSpec(ctypes.Structure):
+    _fields_ = [("n", _i64), ("n_exc", _i64), ("in_degree", _i32), ("reserved", _i32)]
+
+
 class RunArgs(ctypes.Structure):
     _fields_ = [("noise_mode", _i32), ("noise_rate", _f32), ("uniforms", _vp), ("isolated", _i32),
                 ("use_graph", _i32), ("freqs", _vp), ("vmeans", _vp), ("imeans", _vp), ("sample_spike", _vp),
@@ -75,6 +79,7 @@ def lib():
         "dtb_block_create": [_vp, _vp],
         "dtb_block_create_fixed_indegree": [_vp, _vp, _vp, _f64, _u64, _i32, _i32],
         "dtb_block_destroy": [_vp],
+        "dtb_block_create_members": [_vp, _vp, _i32, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32, _i32],
         "dtb_block_run": [_vp, _i64, _vp],
         "dtb_block_get": [_vp, _i32, _vp, _i64, _vp],
         "dtb_block_info": [_vp, _vp],
@@ -103,6 +108,7 @@ def lib():
 def exported_symbols():
     """Names the header declares, for the load test."""
     return ["dtb_last_error", "dtb_abi_version", "dtb_block_create", "dtb_block_create_fixed_indegree",
+            "dtb_block_create_members",
             "dtb_block_destroy", "dtb_block_run", "dtb_block_get", "dtb_block_info", "dtb_block_time",
             "dtb_block_set_props", "dtb_block_mul_props_by_pop", "dtb_block_gamma_props_by_pop",
             "dtb_block_replace_props", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",

@@ -62,6 +62,8 @@ static int alloc_n(Block* b, T** p, int64_t count)
     return block_alloc(b, reinterpret_cast<void**>(p), sizeof(T) * (size_t)std::max<int64_t>(count, 1));
 }
 
+constexpr int kMaxPsets = 64; // parameter sets held in LDS by neuron_step_kernel
+
 __device__ inline float wave_sum(float v)
 {
 #pragma unroll
@@ -85,6 +87,9 @@ struct NeuronArgs {
     const float* __restrict__ tref;
     const float* __restrict__ noisep;
     const int32_t* __restrict__ pop;
+    const uint8_t* __restrict__ pset; // parameter set of each neuron (PSET kernels)
+    const float* __restrict__ ptab;   // [npsets][16]: V_u[4], decay[4], g_L, V_L, dt/C, V_th, V_reset, I_ext
+    int32_t npsets;
     float* __restrict__ V;
     float4* __restrict__ Jp;
     int32_t* __restrict__ ref_end;
@@ -124,11 +129,19 @@ __device__ inline int32_t refractory_end(int64_t step, double t_spike, float tre
     return (int32_t)(e > 0x7fffffff ? 0x7fffffff : e);
 }
 
-template <bool PAIR>
+// PSET: the constants shared by groups of neurons (reversal potentials,
+// synaptic decays, leak, threshold, reset, I_ext) come from a small table in
+// LDS indexed by a 1-byte set id instead of 56 B of per-neuron arrays.
+template <bool PAIR, bool PSET>
 __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t k)
 {
     __shared__ uint32_t s_wcnt[4];
     __shared__ uint32_t s_base;
+    __shared__ float s_tab[kMaxPsets * 16];
+    if (PSET) {
+        for (int x = threadIdx.x; x < a.npsets * 16; x += 256) s_tab[x] = a.ptab[x];
+        __syncthreads();
+    }
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t step = a.ctl->nbase + k + 1;
     const double t = a.ttab[k];
@@ -154,16 +167,26 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
             J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
         }
         const float4 g = a.g[i];
-        const float4 vu = a.vu[i];
+        float4 vu, dc;
+        float gL, vL, dtC, vth, vrst, iext;
+        if (PSET) {
+            const float* T = s_tab + 16 * a.pset[i];
+            vu = make_float4(T[0], T[1], T[2], T[3]);
+            dc = make_float4(T[4], T[5], T[6], T[7]);
+            gL = T[8]; vL = T[9]; dtC = T[10]; vth = T[11]; vrst = T[12]; iext = T[13];
+        } else {
+            vu = a.vu[i];
+            dc = a.decay[i];
+            gL = a.gL[i]; vL = a.vL[i]; dtC = a.dtC[i]; vth = a.vth[i]; vrst = a.vrst[i]; iext = a.iext[i];
+        }
         float isyn = g.x * (vu.x - V) * J.x;
         isyn += g.y * (vu.y - V) * J.y;
         isyn += g.z * (vu.z - V) * J.z;
         isyn += g.w * (vu.w - V) * J.w;
-        const float main_part = -a.gL[i] * (V - a.vL[i]);
-        const float vn = V + a.dtC[i] * (main_part + isyn + a.iext[i]);
+        const float main_part = -gL * (V - vL);
+        const float vn = V + dtC * (main_part + isyn + iext);
         const bool not_ref = step >= (int64_t)a.ref_end[i];
-        const float vi = not_ref ? vn : a.vrst[i];
-        const float vth = a.vth[i];
+        const float vi = not_ref ? vn : vrst;
         act = vi >= vth;
         vout = vi > vth ? vth : vi;
         a.V[i] = vout;
@@ -179,7 +202,6 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
             u = noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
         const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
         spike = (u < pr) || (act && !a.isolated);
-        const float4 dc = a.decay[i];
         a.Jp[i] = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
         a.flags[i] = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
         p = a.pop[i];
@@ -754,6 +776,25 @@ __global__ void derive_kernel(int64_t n, const float* __restrict__ props, double
     ref_end[i] = refractory_end(last_step[i], t_last[i], p[5], dt);
 }
 
+// Parameter-set table gathered from the derived arrays of one representative
+// neuron per set, so both kernel paths read bit-identical constants.
+__global__ void pset_table_kernel(int32_t m, const int64_t* __restrict__ rep, const float4* __restrict__ vu,
+                                  const float4* __restrict__ decay, const float* __restrict__ gL,
+                                  const float* __restrict__ vL, const float* __restrict__ dtC,
+                                  const float* __restrict__ vth, const float* __restrict__ vrst,
+                                  const float* __restrict__ iext, float* __restrict__ tab)
+{
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int64_t r = rep[k];
+    float* T = tab + 16 * k;
+    const float4 u = vu[r], d = decay[r];
+    T[0] = u.x; T[1] = u.y; T[2] = u.z; T[3] = u.w;
+    T[4] = d.x; T[5] = d.y; T[6] = d.z; T[7] = d.w;
+    T[8] = gL[r]; T[9] = vL[r]; T[10] = dtC[r]; T[11] = vth[r]; T[12] = vrst[r]; T[13] = iext[r];
+    T[14] = 0.0f; T[15] = 0.0f;
+}
+
 __global__ void init_state_kernel(int64_t n, const float* __restrict__ props, float* V)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -933,7 +974,55 @@ static int upload(Block* b, void* dst, const void* src, size_t bytes)
     return DTB_OK;
 }
 
-int derive_props(Block* b)
+// Columns whose values enter a parameter set: I_ext, C, g_L, V_L, V_th,
+// V_reset, V_ui[4], tau_ui[4].
+static bool pset_column(uint32_t c) { return c == 2 || c == 4 || (c >= 6 && c <= 9) || (c >= 14 && c <= 21); }
+
+static int build_psets(Block* b)
+{
+    if (b->pset_off) {
+        b->npsets = 0;
+        return DTB_OK;
+    }
+    std::vector<float> props((size_t)b->n * DTB_NPROP);
+    DTB_TRY_HIP(hipMemcpyAsync(props.data(), b->props, sizeof(float) * props.size(), hipMemcpyDeviceToHost, b->stream));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    static const int cols[14] = {2, 4, 6, 7, 8, 9, 14, 15, 16, 17, 18, 19, 20, 21};
+    std::map<std::vector<uint32_t>, int32_t> sets;
+    std::vector<uint8_t> idx((size_t)b->n);
+    std::vector<int64_t> rep;
+    std::vector<uint32_t> key(14);
+    for (int64_t i = 0; i < b->n; ++i) {
+        for (int c = 0; c < 14; ++c) std::memcpy(&key[(size_t)c], &props[(size_t)i * DTB_NPROP + cols[c]], 4);
+        auto it = sets.find(key);
+        if (it == sets.end()) {
+            if ((int32_t)sets.size() == kMaxPsets) {
+                b->npsets = 0; // too many distinct sets: per-neuron arrays
+                return DTB_OK;
+            }
+            it = sets.emplace(key, (int32_t)sets.size()).first;
+            rep.push_back(i);
+        }
+        idx[(size_t)i] = (uint8_t)it->second;
+    }
+    if (!b->pset) DTB_TRY(alloc_n(b, &b->pset, b->n));
+    if (!b->ptab) DTB_TRY(alloc_n(b, &b->ptab, kMaxPsets * 16));
+    int64_t* drep = nullptr;
+    DTB_TRY_HIP(hipMalloc(&drep, sizeof(int64_t) * rep.size()));
+    int st = upload(b, drep, rep.data(), sizeof(int64_t) * rep.size());
+    if (!st) st = upload(b, b->pset, idx.data(), (size_t)b->n);
+    if (!st) {
+        hipLaunchKernelGGL(pset_table_kernel, dim3(1), dim3(kMaxPsets), 0, b->stream, (int32_t)rep.size(), drep,
+                           b->vu, b->decay, b->gL, b->vL, b->dtC, b->vth, b->vrst, b->iext, b->ptab);
+        if (hipStreamSynchronize(b->stream) != hipSuccess) st = DTB_RUNTIME_ERROR;
+    }
+    hipFree(drep);
+    if (st) return st;
+    b->npsets = (int32_t)rep.size();
+    return DTB_OK;
+}
+
+int derive_props(Block* b, bool psets_dirty)
 {
     hipLaunchKernelGGL(derive_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->props, b->dt,
                        b->stats_excl, b->g, b->vu, b->decay, b->gL, b->vL, b->dtC, b->vth, b->vrst, b->iext, b->tref,
@@ -965,6 +1054,11 @@ int derive_props(Block* b)
     DTB_TRY_HIP(hipMemcpyAsync(b->pop_sized, szd.data(), sizeof(double) * (size_t)b->S, hipMemcpyHostToDevice,
                                b->stream));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    if (psets_dirty) {
+        DTB_TRY(build_psets(b));
+        for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second); // the kernel variant may change
+        b->graphs.clear();
+    }
     return DTB_OK;
 }
 
@@ -1268,6 +1362,7 @@ static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* unif
     x.n = b->n; x.D = b->D; x.S = b->S;
     x.g = b->g; x.vu = b->vu; x.decay = b->decay; x.gL = b->gL; x.vL = b->vL; x.dtC = b->dtC; x.vth = b->vth;
     x.vrst = b->vrst; x.iext = b->iext; x.tref = b->tref; x.noisep = b->noisep; x.pop = b->pop;
+    x.pset = b->pset; x.ptab = b->ptab; x.npsets = b->npsets;
     x.V = b->V; x.Jp = b->Jp; x.ref_end = b->ref_end; x.last_step = b->last_step; x.t_last = b->t_last;
     x.flags = b->flags; x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.L = b->D + 1; x.ctl = b->ctl;
     x.clear_acc = b->deliver_mode == 0;
@@ -1312,10 +1407,17 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
     const unsigned ng = grid_for(b->n, 256);
     for (int32_t k = 0; k < K; ++k) {
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k)], b->stream));
-        if (b->pair)
-            hipLaunchKernelGGL(neuron_step_kernel<true>, dim3(ng), dim3(256), 0, b->stream, na, k);
-        else
-            hipLaunchKernelGGL(neuron_step_kernel<false>, dim3(ng), dim3(256), 0, b->stream, na, k);
+        if (b->pair) {
+            if (b->npsets)
+                hipLaunchKernelGGL((neuron_step_kernel<true, true>), dim3(ng), dim3(256), 0, b->stream, na, k);
+            else
+                hipLaunchKernelGGL((neuron_step_kernel<true, false>), dim3(ng), dim3(256), 0, b->stream, na, k);
+        } else {
+            if (b->npsets)
+                hipLaunchKernelGGL((neuron_step_kernel<false, true>), dim3(ng), dim3(256), 0, b->stream, na, k);
+            else
+                hipLaunchKernelGGL((neuron_step_kernel<false, false>), dim3(ng), dim3(256), 0, b->stream, na, k);
+        }
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
         if (b->deliver_mode == 1) {
             StageArgs sa{da, b->nbk, b->stage_cap, b->cursor, b->stage, b->acc2, b->ovf};
@@ -1706,6 +1808,46 @@ int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_sp
     return DTB_OK;
 }
 
+int dtb_block_create_members(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
+                             int32_t max_delay, float w_lo, float w_hi, int32_t pair_weights, uint64_t seed,
+                             const float* props, double dt, uint64_t noise_seed, int32_t stats_exclude_blocked,
+                             int32_t device)
+{
+    DTB_REQUIRE(members != nullptr && props != nullptr && n_members >= 1, "members and props are required");
+    DTB_REQUIRE(max_delay >= 1 && max_delay <= DTB_MAX_DELAY, "max_delay must be in 1..64");
+    DTB_REQUIRE(dt > 0.0, "dt must be > 0");
+    std::vector<dtb_member_spec> hm((size_t)n_members);
+    DTB_TRY_HIP(hipMemcpy(hm.data(), members, sizeof(dtb_member_spec) * (size_t)n_members, hipMemcpyDefault));
+    int64_t n = 0, nnz = 0;
+    for (const auto& m : hm) {
+        DTB_REQUIRE(m.n >= 1 && m.n_exc >= 0 && m.n_exc <= m.n && m.in_degree >= 0, "bad member spec");
+        DTB_REQUIRE(m.in_degree == 0 || m.n >= 2, "a member with synapses needs at least 2 neurons");
+        n += m.n;
+        nnz += m.n * (int64_t)m.in_degree;
+    }
+    Block* b = nullptr;
+    DTB_TRY(begin(&b, device, n));
+    int st = DTB_OK;
+    do {
+        b->dt = dt;
+        b->seed = noise_seed;
+        b->stats_excl = stats_exclude_blocked;
+        b->pair = pair_weights != 0;
+        b->nnz = nnz;
+        if ((st = make_pack(b, max_delay))) break;
+        b->max_row = 8192;
+        if ((st = alloc_state(b))) break;
+        if ((st = gen_members(b, hm.data(), n_members, max_delay, w_lo, w_hi, seed))) break;
+        st = finish_create(b, props);
+    } while (0);
+    if (st) {
+        destroy(b);
+        return st;
+    }
+    *out = static_cast<dtb_block*>(b);
+    return DTB_OK;
+}
+
 int dtb_block_destroy(dtb_block* b)
 {
     destroy(b);
@@ -1770,7 +1912,7 @@ int dtb_block_set_props(dtb_block* b, int64_t m, const uint32_t* neuron, const u
     if (!st) {
         hipLaunchKernelGGL(set_props_kernel, dim3(grid_for(m, 256)), dim3(256), 0, b->stream, m, (const uint32_t*)dn,
                            (const uint32_t*)dc, (const float*)dv, b->props);
-        st = derive_props(b);
+        st = derive_props(b, std::any_of(hc.begin(), hc.end(), pset_column));
     }
     hipStreamSynchronize(b->stream);
     hipFree(dn);
@@ -1797,7 +1939,7 @@ int dtb_block_mul_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, con
     if (!st) {
         hipLaunchKernelGGL(mul_props_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->S, b->sub,
                            (const float*)dt, (const int32_t*)dc, (int32_t)cols.size(), b->props);
-        st = derive_props(b);
+        st = derive_props(b, std::any_of(cols.begin(), cols.end(), [](int32_t c) { return pset_column((uint32_t)c); }));
     }
     hipStreamSynchronize(b->stream);
     hipFree(dt);
@@ -1832,7 +1974,7 @@ int dtb_block_gamma_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, c
     if (!st) {
         hipLaunchKernelGGL(gamma_props_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->S, b->sub,
                            (const float2*)dt, (const int32_t*)dc, (int32_t)cols.size(), seed, b->props);
-        st = derive_props(b);
+        st = derive_props(b, std::any_of(cols.begin(), cols.end(), [](int32_t c) { return pset_column((uint32_t)c); }));
     }
     hipStreamSynchronize(b->stream);
     hipFree(dt);
@@ -1930,6 +2072,10 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
     } else if (what == DTB_LAUNCH_DELIVER_LANES) {
         DTB_REQUIRE(value == 16 || value == 32 || value == 64, "lanes per segment must be 16, 32 or 64");
         b->deliver_lanes = value;
+    } else if (what == DTB_LAUNCH_NEURON_PSETS) {
+        DTB_REQUIRE(value == 0 || value == 1, "parameter sets must be 0 (off) or 1 (auto)");
+        b->pset_off = value == 0;
+        DTB_TRY(build_psets(b));
     } else if (what == DTB_LAUNCH_DELIVER_MODE) {
         DTB_REQUIRE(value == 0 || value == 1, "delivery mode must be 0 (atomics) or 1 (bucketed)");
         if (value == 1) DTB_TRY(ensure_bucketed(b));
@@ -1949,6 +2095,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     if (what == DTB_LAUNCH_DELIVER_GRID) *value = b->deliver_grid;
     else if (what == DTB_LAUNCH_DELIVER_LANES) *value = b->deliver_lanes;
     else if (what == DTB_LAUNCH_DELIVER_MODE) *value = b->deliver_mode;
+    else if (what == DTB_LAUNCH_NEURON_PSETS) *value = b->npsets;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

@@ -37,6 +37,10 @@ struct Block {
     float *gL = nullptr, *vL = nullptr, *dtC = nullptr, *vth = nullptr, *vrst = nullptr, *iext = nullptr;
     float *tref = nullptr, *noisep = nullptr;
     int32_t* pop = nullptr; // statistics sub-population (-1: excluded)
+    uint8_t* pset = nullptr; // parameter set of each neuron (when npsets > 0)
+    float* ptab = nullptr;   // [64][16] parameter-set table
+    int32_t npsets = 0;
+    bool pset_off = false;
     int32_t* sub = nullptr; // raw sub_idx (property column 3)
 
     // dynamic state
@@ -106,7 +110,9 @@ struct Block {
 };
 
 int block_alloc(Block* b, void** p, size_t bytes);
-int derive_props(Block* b);
+int derive_props(Block* b, bool psets_dirty = true);
 int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec);
+int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_delay, float w_lo, float w_hi,
+                uint64_t seed);
 
 } // namespace dtb

@@ -1,13 +1,17 @@
 // dtb_gen.hip -- seeded fixed in-degree networks built on the GPU, directly
 // in the presynaptic-major packed CSR the step kernels read.
 //
-// For post i and slot k, one Philox4x32-10 call (ctr = (k, i_lo, i_hi,
-// CONN), key = seed) gives r0..r3:
-//   pre   = floor(r0 (n-1) / 2^32), shifted past i (no self-connection)
+// The network is a list of members (one member for a single block; an
+// uneven data-assimilation ensemble packs many): member m owns neurons
+// [off_m, off_m + n_m), the first n_exc_m of them excitatory, and every one of
+// its neurons receives K_m synapses from inside the member.  For post i of
+// member m and slot k, one Philox4x32-10 call (ctr = (k, i_lo, i_hi, CONN),
+// key = seed) gives r0..r3:
+//   pre   = off_m + floor(r0 (n_m-1) / 2^32), shifted past i (no self-connection)
 //   w0    = fmaf(w_hi - w_lo, u24(r1), w_lo)
 //   delay = 1 + floor(r2 D / 2^32)
 //   w1    = fmaf(w_hi - w_lo, u24(r3), w_lo)      (pair weights)
-//   kind  = pre >= n_exc
+//   kind  = pre - off_m >= n_exc_m
 // Passes: (1) out-degree histogram, (2) exclusive scan -> rowptr,
 // (3) scatter (delay, post, k) keys into their rows, (4) per-row LDS bitonic
 // sort of the keys, then re-draw each synapse from its key and write the
@@ -21,11 +25,18 @@
 
 namespace dtb {
 
+struct GenMember {
+    int64_t off, n, n_exc, syn_off; // syn_off: synapses of the members before this one
+    int32_t K, pad;
+};
+
 struct GenSpec {
-    int64_t n, n_exc;
-    int32_t K, D;
+    int64_t n;       // neurons in all members
+    int64_t total;   // synapses in all members
+    int32_t M, D;
     float w_lo, span;
     uint32_t k0, k1;
+    const GenMember* __restrict__ mem;
 };
 
 __device__ inline u32x4 conn_draw(const GenSpec& s, int64_t i, int32_t k)
@@ -33,22 +44,56 @@ __device__ inline u32x4 conn_draw(const GenSpec& s, int64_t i, int32_t k)
     return philox4x32_10(u32x4{(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), kStreamConn}, s.k0, s.k1);
 }
 
-__device__ inline uint32_t conn_pre(const GenSpec& s, int64_t i, uint32_t r0)
+// member whose range [key_of(m), key_of(m+1)) holds v (members sorted)
+__device__ inline int32_t member_of_syn(const GenSpec& s, int64_t x)
 {
-    uint32_t pre = (uint32_t)(((uint64_t)r0 * (uint64_t)(s.n - 1)) >> 32);
-    if ((int64_t)pre >= i) pre += 1;
-    return pre;
+    int32_t lo = 0, hi = s.M - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (s.mem[mid].syn_off <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline int32_t member_of_neuron(const GenSpec& s, int64_t j)
+{
+    int32_t lo = 0, hi = s.M - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (s.mem[mid].off <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// synapse x -> (post i, slot k, member m)
+__device__ inline void syn_coords(const GenSpec& s, int64_t x, int64_t& i, int32_t& k, int32_t& m)
+{
+    m = member_of_syn(s, x);
+    const GenMember& g = s.mem[m];
+    const int64_t local = x - g.syn_off;
+    const int64_t il = local / g.K;
+    k = (int32_t)(local - il * g.K);
+    i = g.off + il;
+}
+
+__device__ inline uint32_t conn_pre(const GenMember& g, int64_t i, uint32_t r0)
+{
+    uint32_t pre = (uint32_t)(((uint64_t)r0 * (uint64_t)(g.n - 1)) >> 32);
+    if ((int64_t)pre >= i - g.off) pre += 1;
+    return (uint32_t)(g.off + pre);
 }
 
 __global__ void gen_count_kernel(GenSpec s, uint32_t* __restrict__ outdeg)
 {
-    const int64_t total = s.n * (int64_t)s.K;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
-        const int64_t i = x / s.K;
-        const int32_t k = (int32_t)(x - i * s.K);
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < s.total; x += stride) {
+        int64_t i;
+        int32_t k, m;
+        syn_coords(s, x, i, k, m);
         const u32x4 r = conn_draw(s, i, k);
-        atomicAdd(&outdeg[conn_pre(s, i, r.x)], 1u);
+        atomicAdd(&outdeg[conn_pre(s.mem[m], i, r.x)], 1u);
     }
 }
 
@@ -132,13 +177,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_tiles_kernel(const uint32_t
 
 __global__ void gen_fill_kernel(GenSpec s, unsigned long long* __restrict__ cursor, uint64_t* __restrict__ keys)
 {
-    const int64_t total = s.n * (int64_t)s.K;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
-        const int64_t i = x / s.K;
-        const int32_t k = (int32_t)(x - i * s.K);
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < s.total; x += stride) {
+        int64_t i;
+        int32_t k, m;
+        syn_coords(s, x, i, k, m);
         const u32x4 r = conn_draw(s, i, k);
-        const uint32_t pre = conn_pre(s, i, r.x);
+        const uint32_t pre = conn_pre(s.mem[m], i, r.x);
         const uint64_t d = ((uint64_t)r.z * (uint64_t)s.D) >> 32; // delay - 1
         const unsigned long long pos = atomicAdd(&cursor[pre], 1ull);
         keys[pos] = (d << 58) | ((uint64_t)i << 32) | (uint32_t)k;
@@ -192,7 +237,8 @@ __global__ __launch_bounds__(kSortThreads) void gen_sort_finalize_kernel(GenSpec
                 __syncthreads();
             }
         }
-        const uint32_t kind = (j >= s.n_exc) ? 1u : 0u;
+        const GenMember& gm = s.mem[member_of_neuron(s, j)];
+        const uint32_t kind = (j - gm.off >= gm.n_exc) ? 1u : 0u;
         if (dend) {
             // end of the delay-(dd+1) segment = first key with delay-1 > dd
             for (int32_t dd = threadIdx.x; dd < s.D; dd += kSortThreads) {
@@ -227,19 +273,15 @@ static inline unsigned grid_of(int64_t work, int bs, int64_t cap)
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, (work + bs - 1) / bs));
 }
 
-int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
+static int gen_run(Block* b, const GenSpec& s, int32_t max_delay)
 {
-    const int64_t n = spec->n;
+    const int64_t n = b->n;
     const int64_t nnz = b->nnz;
-    GenSpec s{n, spec->n_exc, spec->in_degree, spec->max_delay, spec->w_lo, spec->w_hi - spec->w_lo,
-              (uint32_t)spec->seed, (uint32_t)(spec->seed >> 32)};
-    DTB_REQUIRE(n <= (1ll << 26), "generated networks are limited to 2^26 neurons");
-    DTB_REQUIRE(spec->max_delay <= 63, "generated networks are limited to delays <= 63");
     void* p = nullptr;
     DTB_TRY(block_alloc(b, &p, sizeof(int64_t) * (size_t)(n + 1)));
     b->rowptr = (int64_t*)p;
-    if (spec->max_delay > 1) {
-        DTB_TRY(block_alloc(b, &p, sizeof(uint16_t) * (size_t)n * (size_t)spec->max_delay));
+    if (max_delay > 1) {
+        DTB_TRY(block_alloc(b, &p, sizeof(uint16_t) * (size_t)n * (size_t)max_delay));
         b->dend = (uint16_t*)p;
     }
     DTB_TRY(block_alloc(b, &p, sizeof(uint32_t) * (size_t)std::max<int64_t>(nnz, 1)));
@@ -302,6 +344,35 @@ int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
     if (keys) hipFree(keys);
     if (flag) hipFree(flag);
     return st;
+}
+
+int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_delay, float w_lo, float w_hi,
+                uint64_t seed)
+{
+    const int64_t n = b->n;
+    const int64_t nnz = b->nnz;
+    DTB_REQUIRE(n <= (1ll << 26), "generated networks are limited to 2^26 neurons");
+    DTB_REQUIRE(max_delay <= 63, "generated networks are limited to delays <= 63");
+    std::vector<GenMember> hm((size_t)M);
+    int64_t off = 0, so = 0;
+    for (int32_t m = 0; m < M; ++m) {
+        hm[(size_t)m] = GenMember{off, mem[m].n, mem[m].n_exc, so, mem[m].in_degree, 0};
+        off += mem[m].n;
+        so += mem[m].n * (int64_t)mem[m].in_degree;
+    }
+    GenMember* dmem = nullptr;
+    DTB_TRY_HIP(hipMalloc(&dmem, sizeof(GenMember) * (size_t)M));
+    DTB_TRY_HIP(hipMemcpy(dmem, hm.data(), sizeof(GenMember) * (size_t)M, hipMemcpyHostToDevice));
+    GenSpec s{n, nnz, M, max_delay, w_lo, w_hi - w_lo, (uint32_t)seed, (uint32_t)(seed >> 32), dmem};
+    int st = gen_run(b, s, max_delay);
+    hipFree(dmem);
+    return st;
+}
+
+int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
+{
+    dtb_member_spec m{spec->n, spec->n_exc, spec->in_degree, 0};
+    return gen_members(b, &m, 1, spec->max_delay, spec->w_lo, spec->w_hi, spec->seed);
 }
 
 } // namespace dtb

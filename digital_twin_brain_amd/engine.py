@@ -79,6 +79,20 @@ class NativeBlock:
                                                             int(bool(stats_exclude_blocked)), int(device)))
         return cls(h.value)
 
+    @classmethod
+    def members(cls, props, members, max_delay=1, w_lo=0.0, w_hi=1.0, pair_weights=False, conn_seed=1, dt=0.1,
+                noise_seed=3, stats_exclude_blocked=False, device=0):
+        """Uneven members packed in one block; members = [(n, n_exc, in_degree), ...]."""
+        nat.require_gpu()
+        props = np.ascontiguousarray(_as_np(props, np.float32)[:, :nat.NPROP])
+        arr = (nat.MemberSpec * len(members))(*[nat.MemberSpec(int(n), int(ne), int(k), 0) for n, ne, k in members])
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().dtb_block_create_members(ctypes.byref(h), arr, len(members), int(max_delay), float(w_lo),
+                                                     float(w_hi), int(bool(pair_weights)), int(conn_seed),
+                                                     nat.ptr(props), float(dt), int(noise_seed),
+                                                     int(bool(stats_exclude_blocked)), int(device)))
+        return cls(h.value)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             nat.lib().dtb_block_destroy(self._h)
@@ -148,7 +162,7 @@ class NativeBlock:
         nat.check(nat.lib().dtb_block_bold_state(self._h, self.BOLD_FIELDS[which], 1, nat.ptr(v)))
 
     # -------------------------------------------------------- launch shape
-    LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2)
+    LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2, neuron_psets=3)
 
     def set_launch(self, **kw):
         for k, v in kw.items():

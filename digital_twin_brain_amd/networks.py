@@ -65,3 +65,48 @@ def build_c2(n=5_000_000, in_degree=1000, device=0, conn_seed=1, noise_seed=3):
     p, ne = c2_props(n)
     return NativeBlock.fixed_indegree(p, ne, in_degree, max_delay=10, w_lo=0.0, w_hi=1.0, pair_weights=False,
                                       conn_seed=conn_seed, noise_seed=noise_seed, device=device), p
+
+
+def c4_members(E=64, seed=4, n_lo=2000, n_hi=200_000, k_lo=100, k_hi=500, truth_n=20_000, truth_k=300):
+    """C4: E members with N log-uniform in [n_lo, n_hi] and in-degree uniform in
+    [k_lo, k_hi], plus one hidden "truth" member that produces the observed BOLD."""
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), size=E))).astype(np.int64)
+    ks = rng.integers(k_lo, k_hi + 1, size=E)
+    mem = [(int(n), int(round(0.8 * n)), int(k)) for n, k in zip(ns, ks)]
+    if truth_n:
+        mem.append((int(truth_n), int(round(0.8 * truth_n)), int(truth_k)))
+    return mem
+
+
+def c4_props(members, voxels=90, seed=5, dt=0.1, member_jitter=0.3):
+    """Per member: `voxels` voxels of an E and an I sub-population
+    (sub_idx = (member*voxels + v)*2 + {0: E, 1: I}); conductances of the C2
+    regime rescaled by 1000/in-degree, perturbed +-member_jitter per member
+    and channel, +-20% per neuron."""
+    rng = np.random.default_rng(seed)
+    n = sum(m[0] for m in members)
+    p = base_props(n, dt)
+    p[:, 0] = rng.uniform(5.0, 20.0, size=n).astype(np.float32) * np.float32(dt * 1e-3)
+    p[:, 2] = 5.0
+    scale = np.array([0.1, 0.1, 0.05, 0.05], np.float32)
+    off = 0
+    for m, (nm, ne, k) in enumerate(members):
+        ni = nm - ne
+        sub = np.empty(nm, np.float32)
+        sub[:ne] = (m * voxels + (np.arange(ne) * voxels) // max(ne, 1)) * 2
+        sub[ne:] = (m * voxels + (np.arange(ni) * voxels) // max(ni, 1)) * 2 + 1
+        p[off:off + nm, 3] = sub
+        fac = rng.uniform(1 - member_jitter, 1 + member_jitter, size=4).astype(np.float32)
+        p[off:off + nm, 10:14] = (G_CAL * scale * fac * np.float32(1000.0 / max(k, 1)) *
+                                  rng.uniform(0.8, 1.2, size=(nm, 4)).astype(np.float32))
+        off += nm
+    return p
+
+
+def build_c4(E=64, voxels=90, device=0, conn_seed=1, noise_seed=3, max_delay=1, **kw):
+    mem = c4_members(E=E, **kw)
+    p = c4_props(mem, voxels=voxels)
+    nb = NativeBlock.members(p, mem, max_delay=max_delay, w_lo=0.0, w_hi=1.0, conn_seed=conn_seed,
+                             noise_seed=noise_seed, stats_exclude_blocked=True, device=device)
+    return nb, p, mem

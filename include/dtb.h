@@ -88,6 +88,24 @@ int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_sp
                                     double dt, uint64_t noise_seed, int32_t stats_exclude_blocked,
                                     int32_t device);
 
+/* A network of members packed in one block (a data-assimilation ensemble of
+ * uneven sizes): member m owns the next n neurons, the first
+ * n_exc excitatory, each receiving in_degree synapses from inside the member
+ * with the draws of dtb_block_create_fixed_indegree.  No reference
+ * counterpart: the reference replicates equal members inside one network
+ * (data_assimilation/DataAssimilation.py:97). */
+typedef struct {
+    int64_t n;
+    int64_t n_exc;
+    int32_t in_degree;
+    int32_t reserved;
+} dtb_member_spec;
+
+int dtb_block_create_members(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
+                             int32_t max_delay, float w_lo, float w_hi, int32_t pair_weights, uint64_t seed,
+                             const float* props, double dt, uint64_t noise_seed, int32_t stats_exclude_blocked,
+                             int32_t device);
+
 int dtb_block_destroy(dtb_block* b);
 
 /* Noise modes of dtb_block_run. */
@@ -196,6 +214,8 @@ int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx);
 #define DTB_LAUNCH_DELIVER_GRID 0  /* 1..65536 */
 #define DTB_LAUNCH_DELIVER_LANES 1 /* 16, 32 or 64 */
 #define DTB_LAUNCH_DELIVER_MODE 2  /* 0: global atomics, 1: bucketed LDS accumulation */
+#define DTB_LAUNCH_NEURON_PSETS 3  /* set: 0 per-neuron constants, 1 shared parameter sets when <= 64
+                                      distinct; get: number of sets in use (0 = per-neuron) */
 int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
 int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 

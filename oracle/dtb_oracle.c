@@ -104,44 +104,58 @@ void orc_noise_uniforms(uint64_t seed, uint64_t step, int64_t n, float* out)
  * Row pre lists its synapses by ascending (post, k).
  * rowptr must hold n+1 entries; the other outputs nnz = n*K entries.
  */
+/* Members packed in one network: member m owns neurons [off_m, off_m+n_m),
+ * the first nexc_m excitatory, each receiving K_m synapses from inside the
+ * member; draws as above with pre = off_m + floor(r0 (n_m-1) / 2^32) shifted
+ * past i, and kind = (pre - off_m >= nexc_m). */
+void orc_gen_members(int32_t M, const int64_t* ns, const int64_t* nexcs, const int32_t* Ks, int32_t D, float w_lo,
+                     float w_hi, int32_t pair, uint64_t seed, int64_t* rowptr, int32_t* post, uint8_t* kind,
+                     uint8_t* delay, float* w0, float* w1)
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    float span = w_hi - w_lo;
+    int64_t n = 0;
+    for (int32_t m = 0; m < M; ++m) n += ns[m];
+    int64_t* cursor = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    memset(rowptr, 0, sizeof(int64_t) * (size_t)(n + 1));
+    for (int pass = 0; pass < 2; ++pass) {
+        int64_t off = 0;
+        for (int32_t m = 0; m < M; ++m) {
+            for (int64_t i = off; i < off + ns[m]; ++i) {
+                for (int32_t k = 0; k < Ks[m]; ++k) {
+                    uint32_t ctr[4] = {(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), DTB_STREAM_CONN};
+                    uint32_t r[4];
+                    orc_philox4x32_10(ctr, key, r);
+                    int64_t pl = (int64_t)(((uint64_t)r[0] * (uint64_t)(ns[m] - 1)) >> 32);
+                    if (pl >= i - off) pl += 1;
+                    int64_t pre = off + pl;
+                    if (pass == 0) {
+                        rowptr[pre + 1] += 1;
+                        continue;
+                    }
+                    int64_t e = cursor[pre]++;
+                    post[e] = (int32_t)i;
+                    kind[e] = (uint8_t)(pl >= nexcs[m]);
+                    delay[e] = (uint8_t)(1 + (uint32_t)(((uint64_t)r[2] * (uint64_t)D) >> 32));
+                    w0[e] = fmaf(span, u24(r[1]), w_lo);
+                    if (w1) w1[e] = pair ? fmaf(span, u24(r[3]), w_lo) : w0[e];
+                }
+            }
+            off += ns[m];
+        }
+        if (pass == 0) {
+            for (int64_t j = 0; j < n; ++j) rowptr[j + 1] += rowptr[j];
+            for (int64_t j = 0; j < n; ++j) cursor[j] = rowptr[j];
+        }
+    }
+    free(cursor);
+}
+
 void orc_gen_fixed_indegree(int64_t n, int64_t n_exc, int32_t K, int32_t D, float w_lo, float w_hi,
                             int32_t pair, uint64_t seed, int64_t* rowptr, int32_t* post, uint8_t* kind,
                             uint8_t* delay, float* w0, float* w1)
 {
-    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    float span = w_hi - w_lo;
-    int64_t* cursor = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
-    memset(rowptr, 0, sizeof(int64_t) * (size_t)(n + 1));
-    /* pass 1: out-degree histogram */
-    for (int64_t i = 0; i < n; ++i) {
-        for (int32_t k = 0; k < K; ++k) {
-            uint32_t ctr[4] = {(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), DTB_STREAM_CONN};
-            uint32_t r[4];
-            orc_philox4x32_10(ctr, key, r);
-            uint32_t pre = (uint32_t)(((uint64_t)r[0] * (uint64_t)(n - 1)) >> 32);
-            if ((int64_t)pre >= i) pre += 1;
-            rowptr[pre + 1] += 1;
-        }
-    }
-    for (int64_t j = 0; j < n; ++j) rowptr[j + 1] += rowptr[j];
-    for (int64_t j = 0; j < n; ++j) cursor[j] = rowptr[j];
-    /* pass 2: stable fill in (post, k) order */
-    for (int64_t i = 0; i < n; ++i) {
-        for (int32_t k = 0; k < K; ++k) {
-            uint32_t ctr[4] = {(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), DTB_STREAM_CONN};
-            uint32_t r[4];
-            orc_philox4x32_10(ctr, key, r);
-            uint32_t pre = (uint32_t)(((uint64_t)r[0] * (uint64_t)(n - 1)) >> 32);
-            if ((int64_t)pre >= i) pre += 1;
-            int64_t e = cursor[pre]++;
-            post[e] = (int32_t)i;
-            kind[e] = (uint8_t)((int64_t)pre >= n_exc);
-            delay[e] = (uint8_t)(1 + (uint32_t)(((uint64_t)r[2] * (uint64_t)D) >> 32));
-            w0[e] = fmaf(span, u24(r[1]), w_lo);
-            if (w1) w1[e] = pair ? fmaf(span, u24(r[3]), w_lo) : w0[e];
-        }
-    }
-    free(cursor);
+    orc_gen_members(1, &n, &n_exc, &K, D, w_lo, w_hi, pair, seed, rowptr, post, kind, delay, w0, w1);
 }
 
 /* ------------------------------------------------------------------ */

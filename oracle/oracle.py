@@ -45,6 +45,8 @@ def _lib(real="f64"):
     lib.orc_noise_uniforms.argtypes = [_c_u64, _c_u64, _c_i64, _vp]
     lib.orc_gen_fixed_indegree.argtypes = [_c_i64, _c_i64, _c_i32, _c_i32, _c_f32, _c_f32, _c_i32, _c_u64,
                                            _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.orc_gen_members.argtypes = [_c_i32, _vp, _vp, _vp, _c_i32, _c_f32, _c_f32, _c_i32, _c_u64,
+                                    _vp, _vp, _vp, _vp, _vp, _vp]
     f = getattr(lib, "orc_block_create" + sfx)
     f.argtypes = [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_f64, _c_i32]
     f.restype = _vp
@@ -98,6 +100,24 @@ def gen_fixed_indegree(n, n_exc, in_degree, max_delay, w_lo, w_hi, pair, seed):
     w1 = np.zeros(nnz, dtype=np.float32)
     _lib().orc_gen_fixed_indegree(n, n_exc, in_degree, max_delay, w_lo, w_hi, int(pair), seed,
                                   _p(rowptr), _p(post), _p(kind), _p(delay), _p(w0), _p(w1))
+    return dict(rowptr=rowptr, post=post, kind=kind, delay=delay, w0=w0, w1=w1 if pair else None)
+
+
+def gen_members(members, max_delay, w_lo, w_hi, pair, seed):
+    """Uneven members packed in one network; members = [(n, n_exc, in_degree), ...]."""
+    ns = np.array([m[0] for m in members], np.int64)
+    nexc = np.array([m[1] for m in members], np.int64)
+    ks = np.array([m[2] for m in members], np.int32)
+    n = int(ns.sum())
+    nnz = int((ns * ks).sum())
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    post = np.zeros(nnz, dtype=np.int32)
+    kind = np.zeros(nnz, dtype=np.uint8)
+    delay = np.zeros(nnz, dtype=np.uint8)
+    w0 = np.zeros(nnz, dtype=np.float32)
+    w1 = np.zeros(nnz, dtype=np.float32)
+    _lib().orc_gen_members(len(members), _p(ns), _p(nexc), _p(ks), max_delay, w_lo, w_hi, int(pair), seed,
+                           _p(rowptr), _p(post), _p(kind), _p(delay), _p(w0), _p(w1))
     return dict(rowptr=rowptr, post=post, kind=kind, delay=delay, w0=w0, w1=w1 if pair else None)
 
 

@@ -395,8 +395,9 @@ def test_results_independent_of_delivery_launch_shape():
     for mode, lanes, grid in ((0, 64, 4), (0, 32, 37), (0, 16, 300), (0, 64, 2048), (1, 64, 1), (1, 64, 300)):
         nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"],
                                   dt=0.1, noise_seed=2)
-        nb.set_launch(deliver_lanes=lanes, deliver_grid=grid, deliver_mode=mode)
-        assert nb.get_launch() == {"deliver_grid": grid, "deliver_lanes": lanes, "deliver_mode": mode}
+        nb.set_launch(deliver_lanes=lanes, deliver_grid=grid, deliver_mode=mode, neuron_psets=mode)
+        assert nb.get_launch() == {"deliver_grid": grid, "deliver_lanes": lanes, "deliver_mode": mode,
+                                   "neuron_psets": mode}  # one shared parameter set when enabled
         ev = np.zeros(300, np.uint64)
         nb.run(300, noise_rate=0.02, graph=True, events=ev)
         outs.append((nb.get("V"), nb.get("J"), ev))
@@ -433,3 +434,20 @@ def test_bucketed_delivery_matches_atomics_including_overflow(pair):
     for x, y in zip(outs[0], outs[1]):
         assert np.array_equal(x, y)
     assert outs[0][2][:20].max() > 3 * 8192 * 3  # some bucket saw more events than its staging room
+
+
+def test_gpu_members_generator_csr_bit_exact():
+    """Uneven packed members (the batched ensemble layout) against the oracle."""
+    from digital_twin_brain_amd import NativeBlock
+
+    members = [(500, 400, 30), (2, 1, 1), (2500, 2000, 120), (1, 1, 0), (900, 700, 60)]
+    n = sum(m[0] for m in members)
+    props = np.zeros((n, 22), np.float32)
+    props[:, 4] = 1; props[:, 5] = 2; props[:, 6] = 0.03; props[:, 7] = -75; props[:, 8] = -50; props[:, 9] = -65
+    props[:, 18:22] = (2, 40, 10, 50)
+    for pair, D in ((False, 1), (True, 6)):
+        nb = NativeBlock.members(props, members, max_delay=D, pair_weights=pair, conn_seed=21)
+        ref = rows_by_delay(orc.gen_members(members, D, 0.0, 1.0, pair, seed=21))
+        for k in ("rowptr", "post", "kind", "delay", "w0"):
+            assert np.array_equal(nb.get(k), ref[k]), k
+        assert np.array_equal(nb.get("w1"), ref["w1"] if pair else ref["w0"])

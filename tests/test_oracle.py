@@ -69,6 +69,27 @@ def test_generator_fixed_indegree_properties():
         assert np.all(np.diff(seg) >= 0)
 
 
+def test_generator_members_stay_inside_their_member():
+    members = [(50, 40, 7), (3, 2, 2), (200, 150, 30), (1, 1, 0), (90, 0, 11)]
+    g = orc.gen_members(members, 4, 0.0, 1.0, False, seed=5)
+    offs = np.cumsum([0] + [m[0] for m in members])
+    rp = g["rowptr"]
+    pre = np.repeat(np.arange(offs[-1]), np.diff(rp))
+    post = g["post"]
+    mem_of = np.repeat(np.arange(len(members)), [m[0] for m in members])
+    assert np.array_equal(mem_of[pre], mem_of[post])
+    assert not np.any(pre == post)
+    indeg = np.bincount(post, minlength=offs[-1])
+    assert np.array_equal(indeg, np.repeat([m[2] for m in members], [m[0] for m in members]))
+    local_pre = pre - offs[mem_of[pre]]
+    assert np.array_equal(g["kind"], (local_pre >= np.array([m[1] for m in members])[mem_of[pre]]).astype(np.uint8))
+    # one member reproduces the single-network generator
+    a = orc.gen_members([(300, 240, 17)], 5, 0.25, 0.75, True, seed=99)
+    b = orc.gen_fixed_indegree(300, 240, 17, 5, 0.25, 0.75, True, seed=99)
+    for k in ("rowptr", "post", "kind", "delay", "w0", "w1"):
+        assert np.array_equal(a[k], b[k])
+
+
 # ------------------------------------------------- block step vs golden
 @pytest.mark.parametrize("name,exact", [("dense64", True), ("isolated64", True), ("sparse400", False)])
 @pytest.mark.parametrize("pull", [False, True])
