@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--da-windows", type=int, default=20)
+    ap.add_argument("--no-graph", action="store_true", help="launch step batches directly (for counter runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the single-GPU build has no multi-GPU path "
@@ -111,6 +114,17 @@ def main():
     from digital_twin_brain_amd import networks
 
     torch.cuda.init()
+    if a.copy_calibration:
+        n = a.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                          "launches": 5}), flush=True)
+        return
+    graph = not a.no_graph
     t0 = time.perf_counter()
     nb, props = networks.build_c2(n=a.n, in_degree=a.in_degree)
     log(f"built C2 network n={nb.n} nnz={nb.nnz} S={nb.S} D={nb.max_delay} in {time.perf_counter() - t0:.1f} s")
@@ -119,49 +133,53 @@ def main():
     # warm-up (also captures the step-batch graph)
     W = a.warmup - a.warmup % 10
     if W:
-        nb.run(W, noise_mode=1, graph=True)
+        nb.run(W, noise_mode=1, graph=graph)
     log(f"warm-up {W} steps done")
     if not a.no_tune:
         from digital_twin_brain_amd.autotune import tune_delivery
 
         best, table = tune_delivery(nb, dict(noise_mode=1), steps=20, use_cache=False)
         log(f"autotuned delivery launch: {best} from {json.dumps(table)}")
-        nb.run(100, noise_mode=1, graph=True)  # re-capture the step graph with the chosen shape
+        nb.run(100, noise_mode=1, graph=graph)  # re-capture the step graph with the chosen shape
     K = a.steps - a.steps % 10
+    # the last M steps of the timed region run with hipEvents around each kernel phase
+    # (on the block's stream, direct launches); the rest run as captured step-batch graphs
+    M = min(K, max(10, a.kernel_window - a.kernel_window % 10))
+    G = K - M
     events = np.zeros(K, np.uint64)
     nspk = np.zeros(K, np.uint32)
     bold = np.zeros((K // 10, S), np.float64)
+    kms = np.zeros((M, 2), np.float32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    nb.run(K, noise_mode=1, graph=True, events=events, spikes=nspk, bold=bold)
+    if G:
+        nb.run(G, noise_mode=1, graph=graph, events=events[:G], spikes=nspk[:G], bold=bold[:G // 10])
+    nb.run(M, noise_mode=1, graph=False, events=events[G:], spikes=nspk[G:], bold=bold[G // 10:], kernel_ms=kms)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     tot_ev = int(events.sum())
     log(f"timed {K} steps in {wall:.3f} s: {tot_ev} events, {int(nspk.sum())} spikes, "
         f"mean rate {nspk.sum() / K / nb.n / 1e-4:.2f} Hz delivered")
-
-    # per-kernel device time over a window (hipEvents on the block's stream)
-    M = max(10, a.kernel_window - a.kernel_window % 10)
-    kms = np.zeros((M, 2), np.float32)
-    ev_w = np.zeros(M, np.uint64)
-    sp_w = np.zeros(M, np.uint32)
-    nb.run(M, noise_mode=1, graph=False, events=ev_w, spikes=sp_w, kernel_ms=kms)
+    ev_w = events[G:]
+    sp_w = nspk[G:]
     t_neu = float(kms[:, 0].mean()) * 1e-3
     t_del = float(kms[:, 1].mean()) * 1e-3
     launch = nb.get_launch()
     bpn = neuron_bytes(launch)
     b_neu = bpn * nb.n
     # delivered (row, delay) segments: every spike of the last D steps contributes one
-    seg = np.convolve(sp_w.astype(np.float64), np.ones(nb.max_delay))[:M]
-    b_del = float((B_SEGMENT * seg + B_SYNAPSE * ev_w.astype(np.float64))[nb.max_delay:].mean())
+    seg = np.convolve(nspk.astype(np.float64), np.ones(nb.max_delay))[:K][G:]
+    b_del = float((B_SEGMENT * seg + B_SYNAPSE * ev_w.astype(np.float64)).mean())
+    del_name = ("delivery (deliver_stage_kernel + deliver_accum_kernel)" if launch.get("deliver_mode", 0) == 1
+                else "delivery (deliver_kernel)")
     if t_neu >= t_del:
-        dom, bytes_l, tl = "neuron_step_kernel", b_neu, t_neu
+        dom, bytes_l, tl, prof = "neuron_step_kernel", b_neu, t_neu, "neuron_step_kernel"
     else:
-        dom, bytes_l, tl = "deliver_kernel", b_del, t_del
+        dom, bytes_l, tl, prof = del_name, b_del, t_del, "delivery"
     achieved = bytes_l / tl / 1e9
     kernels = {"neuron_step_kernel": {"avg_us": t_neu * 1e6, "bytes_per_launch": b_neu, "bytes_per_neuron": bpn,
                                       "GBps": b_neu / t_neu / 1e9},
-               "deliver_kernel": {"avg_us": t_del * 1e6, "bytes_per_launch": b_del, "GBps": b_del / t_del / 1e9,
+               del_name: {"avg_us": t_del * 1e6, "bytes_per_launch": b_del, "GBps": b_del / t_del / 1e9,
                                   "events_per_launch": float(ev_w.mean()),
                                   "atomic_rmw_bytes_per_launch": 8.0 * float(ev_w.mean())}}
     log(f"kernels: {json.dumps(kernels)}")
@@ -187,7 +205,7 @@ def main():
         log(f"cpu baseline {cpu['value']:.3e} events/s ({time.perf_counter() - t0:.1f} s)")
 
     value = tot_ev / wall
-    traffic = pmc_traffic(dom, "c2")
+    traffic = pmc_traffic(prof, "c2")
     out = {
         "metric": "synaptic events/s", "value": value, "unit": "events/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",

@@ -96,6 +96,14 @@ def lib():
         "dtb_bold_step_f32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "dtb_bold_step_f64": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "dtb_enkf_analysis_f64": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
+        "dtb_mlp_gemm_bf16_nt": [_i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _i32, _vp, _i64, _vp, _i64, _vp],
+        "dtb_mlp_relu_backward": [_i64, _vp, _vp, _vp, _vp, _vp],
+        "dtb_mlp_bias_grad": [_i32, _i32, _vp, _vp, _vp],
+        "dtb_mlp_mse_grad": [_i64, _vp, _vp, _vp, _f32, _vp, _vp, _vp, _vp],
+        "dtb_mlp_adam": [_i64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _i32, _vp, _vp],
+        "dtb_mlp_transpose_bf16": [_i32, _i32, _vp, _vp, _vp],
+        "dtb_mlp_f32_to_bf16": [_i64, _vp, _vp, _vp],
+        "dtb_mlp_affine": [_i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -114,7 +122,9 @@ def exported_symbols():
             "dtb_block_replace_props", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",
             "dtb_block_set_launch", "dtb_block_get_launch",
             "dtb_bold_step_f32", "dtb_bold_step_f64",
-            "dtb_enkf_analysis_f64"]
+            "dtb_enkf_analysis_f64",
+            "dtb_mlp_gemm_bf16_nt", "dtb_mlp_relu_backward", "dtb_mlp_bias_grad", "dtb_mlp_mse_grad",
+            "dtb_mlp_adam", "dtb_mlp_transpose_bf16", "dtb_mlp_f32_to_bf16", "dtb_mlp_affine"]
 
 
 def check(status):

@@ -1,0 +1,263 @@
+// dtb_mlp.hip -- kernels of the learned surrogate: a bf16 MFMA GEMM with
+// FP32 accumulation and fused bias + activation, and the small elementwise
+// kernels of MLP training (ReLU backward, bias gradient, MSE gradient, Adam).
+//
+// GEMM form "NT": C[M,N] = act(A[M,K] * B[N,K]^T + bias[N]); both operands are
+// K-contiguous row-major bf16, so a forward layer is (X, W^T) and the two
+// backward products use transposed copies made by dtb_mlp_transpose_bf16.
+// One workgroup (4 wavefronts) computes a 64x64 tile; each wavefront owns a
+// 32x32 sub-tile and issues v_mfma_f32_32x32x16_bf16 over K in steps of 16.
+//
+// MFMA operand layout (32x32x16 bf16): lane l holds 8 consecutive K values
+// k = 8*(l/32) .. +7 of row (l % 32) for A and of column (l % 32) for B; the
+// 16 accumulator registers hold C[8*(r/4) + 4*(l/32) + r%4][l % 32].
+#include <cmath>
+
+#include "dtb_internal.hpp"
+
+namespace dtb {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline uint16_t f2bf(float f)
+{
+    // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40); // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// act: 0 identity, 1 ReLU
+__global__ __launch_bounds__(256) void gemm_nt_kernel(int32_t M, int32_t N, int32_t K, const uint16_t* __restrict__ A,
+                                                      int64_t lda, const uint16_t* __restrict__ B, int64_t ldb,
+                                                      const float* __restrict__ bias, int32_t act,
+                                                      float* __restrict__ C, int64_t ldc,
+                                                      uint16_t* __restrict__ Cb, int64_t ldcb)
+{
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int m0 = blockIdx.y * 64 + (wid >> 1) * 32;
+    const int n0 = blockIdx.x * 64 + (wid & 1) * 32;
+    if (m0 >= M || n0 >= N) return; // whole 32x32 sub-tiles only (M, N multiples of 32)
+    const int r = lane & 31;
+    const int kh = (lane >> 5) * 8;
+    const uint16_t* pa = A + (int64_t)(m0 + r) * lda + kh;
+    const uint16_t* pb = B + (int64_t)(n0 + r) * ldb + kh;
+    f32x16 acc;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) acc[x] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + k0);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + k0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int col = n0 + r;
+    const float bv = bias ? bias[col] : 0.0f;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+        const int row = m0 + 8 * (x >> 2) + 4 * (lane >> 5) + (x & 3);
+        float v = acc[x] + bv;
+        if (act == 1) v = v > 0.0f ? v : 0.0f;
+        if (C) C[(int64_t)row * ldc + col] = v;
+        if (Cb) Cb[(int64_t)row * ldcb + col] = f2bf(v);
+    }
+}
+
+// dZ = dY * relu'(Z)  (Z > 0), written in FP32 and bf16
+__global__ void relu_backward_kernel(int64_t n, const float* __restrict__ dY, const float* __restrict__ Z,
+                                     float* __restrict__ dZ, uint16_t* __restrict__ dZb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = Z[i] > 0.0f ? dY[i] : 0.0f;
+    if (dZ) dZ[i] = v;
+    if (dZb) dZb[i] = f2bf(v);
+}
+
+// db[j] = sum_i dZ[i][j]
+__global__ void bias_grad_kernel(int32_t M, int32_t N, const float* __restrict__ dZ, float* __restrict__ db)
+{
+    const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    float s = 0.0f;
+    for (int32_t i = 0; i < M; ++i) s += dZ[(int64_t)i * N + j];
+    db[j] = s;
+}
+
+// grad = scale * (out - target) masked; loss partial sums in lsum[blockIdx]
+__global__ void mse_grad_kernel(int64_t n, const float* __restrict__ out, const float* __restrict__ target,
+                                const float* __restrict__ mask, float scale, float* __restrict__ grad,
+                                uint16_t* __restrict__ gradb, float* __restrict__ lsum)
+{
+    __shared__ float red[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float e = 0.0f;
+    if (i < n) {
+        const float mk = mask ? mask[i] : 1.0f;
+        const float d = (out[i] - target[i]) * mk;
+        e = d * d;
+        const float g = scale * d;
+        if (grad) grad[i] = g;
+        if (gradb) gradb[i] = f2bf(g);
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && lsum) lsum[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba, 2015) on FP32 master weights, refreshing the bf16 copy
+__global__ void adam_kernel(int64_t n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2,
+                            uint16_t* __restrict__ wb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void transpose_bf16_kernel(int32_t M, int32_t N, const uint16_t* __restrict__ in, uint16_t* __restrict__ out)
+{
+    __shared__ uint16_t tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 256 threads: 32 x 8
+    for (int y = ty; y < 32; y += 8) {
+        const int r = by + y, c = bx + tx;
+        if (r < M && c < N) tile[y][tx] = in[(int64_t)r * N + c];
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+        const int r = bx + y, c = by + tx; // out is [N][M]
+        if (r < N && c < M) out[(int64_t)r * M + c] = tile[tx][y];
+    }
+}
+
+__global__ void f32_to_bf16_kernel(int64_t n, const float* __restrict__ in, uint16_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f2bf(in[i]);
+}
+
+// out[r][c] = in[r][c] * scale[c] + shift[c] for c < cin, 0 for cin <= c < cout
+__global__ void affine_kernel(int32_t rows, int32_t cin, int32_t cout, const float* __restrict__ in, int64_t ldi,
+                              const float* __restrict__ scale, const float* __restrict__ shift,
+                              float* __restrict__ out, uint16_t* __restrict__ outb, int64_t ldo)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * cout) return;
+    const int32_t r = (int32_t)(i / cout), c = (int32_t)(i % cout);
+    float v = 0.0f;
+    if (c < cin) v = in[(int64_t)r * ldi + c] * (scale ? scale[c] : 1.0f) + (shift ? shift[c] : 0.0f);
+    if (out) out[(int64_t)r * ldo + c] = v;
+    if (outb) outb[(int64_t)r * ldo + c] = f2bf(v);
+}
+
+static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+} // namespace dtb
+
+using namespace dtb;
+
+extern "C" {
+
+int dtb_mlp_gemm_bf16_nt(int32_t M, int32_t N, int32_t K, const void* A, int64_t lda, const void* B, int64_t ldb,
+                         const float* bias, int32_t act, float* C, int64_t ldc, void* Cb, int64_t ldcb, void* stream)
+{
+    DTB_REQUIRE(M > 0 && N > 0 && K > 0, "M, N, K must be positive");
+    DTB_REQUIRE(M % 32 == 0 && N % 32 == 0 && K % 16 == 0, "M and N must be multiples of 32 and K of 16");
+    DTB_REQUIRE(A && B && (C || Cb), "A, B and an output are required");
+    DTB_REQUIRE(lda >= K && ldb >= K && lda % 8 == 0 && ldb % 8 == 0, "lda, ldb must be >= K and multiples of 8");
+    DTB_REQUIRE(act == 0 || act == 1, "act must be 0 (identity) or 1 (ReLU)");
+    DTB_REQUIRE((reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0,
+                "A and B must be 16-byte aligned");
+    dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+    hipLaunchKernelGGL(gemm_nt_kernel, grid, dim3(256), 0, (hipStream_t)stream, M, N, K, (const uint16_t*)A, lda,
+                       (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_mlp_relu_backward(int64_t n, const float* dY, const float* Z, float* dZ, void* dZb, void* stream)
+{
+    DTB_REQUIRE(n >= 0 && dY && Z && (dZ || dZb), "bad arguments");
+    if (n == 0) return DTB_OK;
+    hipLaunchKernelGGL(relu_backward_kernel, g1(n), dim3(256), 0, (hipStream_t)stream, n, dY, Z, dZ, (uint16_t*)dZb);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_mlp_bias_grad(int32_t M, int32_t N, const float* dZ, float* db, void* stream)
+{
+    DTB_REQUIRE(M > 0 && N > 0 && dZ && db, "bad arguments");
+    hipLaunchKernelGGL(bias_grad_kernel, g1(N), dim3(256), 0, (hipStream_t)stream, M, N, dZ, db);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_mlp_mse_grad(int64_t n, const float* out, const float* target, const float* mask, float scale, float* grad,
+                     void* gradb, float* lsum, void* stream)
+{
+    DTB_REQUIRE(n > 0 && out && target, "bad arguments");
+    hipLaunchKernelGGL(mse_grad_kernel, g1(n), dim3(256), 0, (hipStream_t)stream, n, out, target, mask, scale, grad,
+                       (uint16_t*)gradb, lsum);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_mlp_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                 int32_t t, void* wb, void* stream)
+{
+    DTB_REQUIRE(n > 0 && w && g && m && v && t >= 1, "bad arguments");
+    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    hipLaunchKernelGGL(adam_kernel, g1(n), dim3(256), 0, (hipStream_t)stream, n, w, g, m, v, lr, b1, b2, eps, c1, c2,
+                       (uint16_t*)wb);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_mlp_transpose_bf16(int32_t M, int32_t N, const void* in, void* out, void* stream)
+{
+    DTB_REQUIRE(M > 0 && N > 0 && in && out, "bad arguments");
+    dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32));
+    hipLaunchKernelGGL(transpose_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, M, N, (const uint16_t*)in,
+                       (uint16_t*)out);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_mlp_f32_to_bf16(int64_t n, const float* in, void* out, void* stream)
+{
+    DTB_REQUIRE(n >= 0 && in && out, "bad arguments");
+    if (n == 0) return DTB_OK;
+    hipLaunchKernelGGL(f32_to_bf16_kernel, g1(n), dim3(256), 0, (hipStream_t)stream, n, in, (uint16_t*)out);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_mlp_affine(int32_t rows, int32_t cin, int32_t cout, const float* in, int64_t ldi, const float* scale,
+                   const float* shift, float* out, void* outb, int64_t ldo, void* stream)
+{
+    DTB_REQUIRE(rows >= 0 && cin >= 0 && cout >= cin && in && (out || outb), "bad arguments");
+    DTB_REQUIRE(ldi >= cin && ldo >= cout, "ldi must be >= cin and ldo >= cout");
+    if ((int64_t)rows * cout == 0) return DTB_OK;
+    hipLaunchKernelGGL(affine_kernel, g1((int64_t)rows * cout), dim3(256), 0, (hipStream_t)stream, rows, cin, cout,
+                       in, ldi, scale, shift, out, (uint16_t*)outb, ldo);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+} // extern "C"

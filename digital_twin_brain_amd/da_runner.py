@@ -19,12 +19,10 @@ from .data_assimilation import DataAssimilation
 HP_COLUMNS = (10, 12)  # g_AMPA and g_GABAa of every sub-population
 
 
-def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, log=None,
-           **member_kw):
-    rng = np.random.default_rng(seed)
-    t0 = time.perf_counter()
+def _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw):
+    """Block of E members + truth, DataAssimilation over the members, random
+    initial hyper-parameters pushed, truth factors drawn and pushed."""
     nb, props, mem = networks.build_c4(E=E, voxels=voxels, **member_kw)
-    t_build = time.perf_counter() - t0
     P = voxels * 2
     sps = int(round(1.0 / dt))  # BOLD once per ms
     da = DataAssimilation(network=nb, column=False, ensemble=E, dt=dt, bold_steps=sps, da_populations=E * P,
@@ -40,6 +38,15 @@ def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_h
     truth_hp = rng.uniform(hp_lo, hp_hi, size=(P, H)).astype(np.float32)
     nb.mul_props_by_pop(np.repeat(truth_pops, H).astype(np.uint32), np.tile(HP_COLUMNS, P).astype(np.uint32),
                         truth_hp.reshape(-1))
+    return nb, da, truth_hp, (low, high)
+
+
+def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, log=None,
+           **member_kw):
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    nb, da, truth_hp, _ = _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw)
+    t_build = time.perf_counter() - t0
     steps = int(round(window_ms / dt))
     truth_vox = slice(E * voxels, (E + 1) * voxels)
     obs = []
@@ -62,3 +69,68 @@ def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_h
     if log:
         log(f"C4 DA: {out}")
     return out, da, dict(w_save=w_save, w_fix=w_fix, obs=obs, truth_hp=truth_hp)
+
+
+def run_c5(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
+           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=200, finetune_epochs=20, log=None, **member_kw):
+    """Surrogate-assisted DA against simulator-only DA on the same ensemble.
+
+    1. `train_windows` simulator windows with fresh random hyper-parameters
+       give E training samples each; the window surrogate is fitted on them.
+    2. Simulator-only DA runs `windows` windows against the truth member and
+       records the observations.
+    3. From the same initial hyper-parameters, surrogate-assisted DA runs the
+       same number of windows against the recorded observations, simulating
+       every `every`-th window and after any window whose NRMSE > nrmse_max.
+    Returns timings (gain = simulator-only seconds / surrogate-assisted
+    seconds), the fallback pattern, NRMSE per simulated window and the
+    ensemble-mean BOLD error of both runs."""
+    from .surrogate_da import SurrogateDA, WindowSurrogate, collect_windows
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    nb, da, truth_hp, (low, high) = _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw)
+    t_build = time.perf_counter() - t0
+    steps = int(round(window_ms / dt))
+    truth_vox = slice(E * voxels, (E + 1) * voxels)
+    da.evolve(steps, bold_detail=True)
+    t0 = time.perf_counter()
+    X, Y = collect_windows(da, train_windows, steps, rng=rng)
+    t_collect = time.perf_counter() - t0
+    ws = WindowSurrogate(voxels, hidden=hidden, max_batch=min(256, max(32, E)), seed=seed)
+    ws.add(X, Y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = ws.fit(epochs=epochs)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+
+    def reset_hp():
+        hp = da.hp_random_initialize(low, high, rng=np.random.default_rng(seed + 1))
+        da.mul_property_by_subblk(da._hp_index_updating, hp.float().reshape(-1))
+        da.timings = []
+
+    reset_hp()
+    obs = []
+
+    def observe(t):
+        y = nb.bold_state("bold")[truth_vox].copy()
+        obs.append(y)
+        return torch.from_numpy(y)
+
+    w_sim, _, t_sim = da.da_rest_run(observe, "/tmp", observation_times=windows, steps=steps, save=False)
+    obs = np.stack(obs)
+    reset_hp()
+    sda = SurrogateDA(da, ws, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs)
+    res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
+    rmse = lambda w: np.sqrt(((w[:, :, :, -1].mean(1) - obs) ** 2).mean(1))
+    out = dict(members=E, voxels=voxels, windows=windows, steps_per_window=steps, build_s=t_build,
+               train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect, fit_s=t_fit,
+               fit_loss_first=hist[0], fit_loss_last=hist[-1], sim_window_s_mean=float(t_sim.mean()),
+               sur_window_s_mean=float(res["times"].mean()),
+               gain=float(t_sim.sum() / res["times"].sum()), simulated_windows=int(res["simulated"].sum()),
+               nrmse=[round(e, 5) for _, e in res["nrmse"]],
+               bold_rmse_sim_last=float(rmse(w_sim)[-1]), bold_rmse_sur_last=float(rmse(res["forecasts"])[-1]))
+    if log:
+        log(f"C5 surrogate DA: {out}")
+    return out, dict(w_sim=w_sim, sur=res, obs=obs, hist=hist, truth_hp=truth_hp)

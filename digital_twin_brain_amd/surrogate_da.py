@@ -1,0 +1,140 @@
+"""Surrogate-assisted data assimilation (no counterpart in the reference: new
+in this library).
+
+A window of `DataAssimilation` maps, per member, the hyper-parameters of its
+voxels and the BOLD state at the window start to the end-of-window hidden
+state tail (act, s, q, v, f_in, bold per voxel).  `WindowSurrogate` learns
+that map with `MLPSurrogate`; `SurrogateDA` runs the window loop of
+data_assimilation/DataAssimilation.py:402-453 and, in most windows, replaces
+the network simulation by one surrogate evaluation.
+
+Fallback policy: the simulator runs every `every`-th window, and also after
+any simulated window whose surrogate NRMSE exceeds `nrmse_max`, where
+NRMSE = ||BOLD_sur - BOLD_sim||_2 / (sqrt(n) * range(BOLD_sim)) over the
+window's members and voxels.  Each simulated window adds its samples to the
+training set and fine-tunes the surrogate.  Hyper-parameters are pushed into
+the network only in simulated windows (the push is absolute: original value
+times the current factor, so skipped windows lose nothing).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from .surrogate import MLPSurrogate
+
+
+def window_inputs(da):
+    """[E, 8V]: per voxel the 4 hyper-parameter logits of its hidden state and
+    the BOLD state (s, q, v, f_in) at the window start."""
+    E = da.ensemble_number
+    Vb = da._num_voxel_pblk
+    idx = da._index_da_voxel_pblk.numpy()
+    hp = da._hp_log.reshape(E, -1)[:, da.for_hidden_state].reshape(E, len(idx), -1).double()
+    nv = da._da_voxels
+    st = np.stack([da.block_model.nb.bold_state(w)[:nv] for w in ("s", "q", "v", "f_in")], axis=1)
+    st = torch.from_numpy(st.reshape(E, Vb, 4)[:, idx])
+    return torch.cat((hp, st), dim=2).reshape(E, -1)
+
+
+def nrmse(sur, sim):
+    sur = np.asarray(sur, np.float64)
+    sim = np.asarray(sim, np.float64)
+    rng = float(sim.max() - sim.min())
+    return float(np.linalg.norm(sur - sim) / np.sqrt(sim.size) / max(rng, 1e-300))
+
+
+class WindowSurrogate:
+    """MLP from window_inputs -> hidden-state tail [E, V*6]."""
+
+    def __init__(self, voxels, hp_per_voxel=4, hidden=(512, 512, 512), max_batch=256, lr=1e-3, seed=0, device=0):
+        self.V = int(voxels)
+        self.n_in = self.V * (hp_per_voxel + 4)
+        self.n_out = self.V * 6
+        self.mlp = MLPSurrogate(self.n_in, self.n_out, hidden=hidden, max_batch=max_batch, lr=lr, seed=seed,
+                                device=device)
+        self.X = []
+        self.Y = []
+
+    def add(self, X, Y):
+        self.X.append(torch.as_tensor(X, dtype=torch.float64).cpu())
+        self.Y.append(torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], -1).cpu())
+
+    def fit(self, epochs=200, batch=None, renormalize=True):
+        X = torch.cat(self.X)
+        Y = torch.cat(self.Y)
+        if renormalize:
+            self.mlp.set_normalization(X, Y)
+        return self.mlp.fit(X, Y, epochs=epochs, batch=batch)
+
+    def predict(self, X):
+        return self.mlp.predict(torch.as_tensor(X)).double().reshape(X.shape[0], self.V, 6)
+
+
+class SurrogateDA:
+    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=20):
+        self.da = da
+        self.sur = surrogate
+        self.every = int(every)
+        self.nrmse_max = float(nrmse_max)
+        self.finetune_epochs = int(finetune_epochs)
+
+    def _advance_hp(self, hp_sigma):
+        da = self.da
+        da._hp_log += torch.normal(0, hp_sigma ** 0.5, size=da._hp_log.shape).float()
+        da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
+
+    def run(self, observe, windows, steps=800, bold_sigma=1e-8, solo_rate=0.8, hp_sigma=0.25):
+        """`observe(t)` -> observed BOLD [voxels per member].  Returns a dict with
+        per-window forecasts, flags, NRMSE of simulated windows and times."""
+        da = self.da
+        E = da.ensemble_number
+        forecasts, flags, errs, times = [], [], [], []
+        force = False
+        for t in range(windows):
+            t0 = time.perf_counter()
+            self._advance_hp(hp_sigma)
+            X = window_inputs(da)
+            simulate = force or (t % self.every == 0)
+            pred = self.sur.predict(X)
+            if simulate:
+                da.mul_property_by_subblk(da._hp_index_updating, da._hp.float().reshape(-1))
+                da.get_hidden_state(steps)
+                tail = da._hidden_state[:, :, -6:].double().cpu()
+                e = nrmse(pred[:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
+                errs.append((t, e))
+                force = e > self.nrmse_max
+                self.sur.add(X, tail.reshape(E, -1))
+                if self.finetune_epochs:
+                    self.sur.fit(epochs=self.finetune_epochs, renormalize=False)
+            else:
+                hpl = da._hp_log.reshape(E, -1)[:, da.for_hidden_state].reshape(E, pred.shape[1], -1)
+                da._hidden_state = torch.cat((hpl.double().to(pred.device), pred), dim=2)
+            forecasts.append(da._hidden_state.cpu().numpy())
+            flags.append(simulate)
+            y = observe(t)
+            da.da_filter(torch.as_tensor(y).reshape(1, -1), bold_sigma=bold_sigma, solo_rate=solo_rate)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        return dict(forecasts=np.stack(forecasts), simulated=np.array(flags), nrmse=errs, times=np.array(times))
+
+
+def collect_windows(da, windows, steps, logit_range=10.0, rng=None):
+    """Simulator windows without analysis, each with fresh hyper-parameter
+    logits drawn uniformly from [-logit_range, logit_range] (independent
+    coverage of the range the assimilation explores); returns
+    (X [windows*E, 8V], Y [windows*E, 6V]) for training."""
+    rng = np.random.default_rng(0) if rng is None else rng
+    Xs, Ys = [], []
+    E = da.ensemble_number
+    for _ in range(windows):
+        da._hp_log = torch.from_numpy(rng.uniform(-logit_range, logit_range, size=tuple(da._hp_log.shape))).float()
+        da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
+        X = window_inputs(da)
+        da.mul_property_by_subblk(da._hp_index_updating, da._hp.float().reshape(-1))
+        da.get_hidden_state(steps)
+        Xs.append(X)
+        Ys.append(da._hidden_state[:, :, -6:].double().cpu().reshape(E, -1))
+    return torch.cat(Xs), torch.cat(Ys)

@@ -234,6 +234,39 @@ int dtb_bold_step_f64(int64_t n, const double* par, double* s, double* q, double
 int dtb_enkf_analysis_f64(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y, double rho,
                           const double* xi, void* stream);
 
+/* Learned surrogate kernels (no counterpart in the reference: the surrogate
+ * is new in this library).  bf16 operands are raw uint16 bit patterns on the
+ * device.  All calls are asynchronous on `stream` (hipStream_t or NULL). */
+
+/* C[M,N] = act(A[M,K] * B[N,K]^T + bias[N]) with bf16 A, B and FP32
+ * accumulation on the matrix cores.  M and N multiples of 32, K a multiple
+ * of 16, lda/ldb multiples of 8 (elements), A and B 16-byte aligned.
+ * act: 0 identity, 1 ReLU.  bias may be NULL.  Writes C (FP32, ldc) and/or
+ * Cb (bf16, ldcb); either may be NULL but not both. */
+int dtb_mlp_gemm_bf16_nt(int32_t M, int32_t N, int32_t K, const void* A, int64_t lda, const void* B, int64_t ldb,
+                         const float* bias, int32_t act, float* C, int64_t ldc, void* Cb, int64_t ldcb, void* stream);
+/* dZ = dY where Z > 0, else 0 (FP32 and/or bf16 output) */
+int dtb_mlp_relu_backward(int64_t n, const float* dY, const float* Z, float* dZ, void* dZb, void* stream);
+/* db[j] = sum_i dZ[i*N + j] */
+int dtb_mlp_bias_grad(int32_t M, int32_t N, const float* dZ, float* db, void* stream);
+/* grad = scale * mask * (out - target); lsum[b] = sum of (mask*(out-target))^2
+ * over the 256 elements of block b (ceil(n/256) entries; may be NULL). */
+int dtb_mlp_mse_grad(int64_t n, const float* out, const float* target, const float* mask, float scale, float* grad,
+                     void* gradb, float* lsum, void* stream);
+/* one Adam step (t >= 1) on FP32 weights w with moments m, v; refreshes the
+ * bf16 copy wb when not NULL */
+int dtb_mlp_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                 int32_t t, void* wb, void* stream);
+/* out[N,M] = in[M,N]^T (bf16) */
+int dtb_mlp_transpose_bf16(int32_t M, int32_t N, const void* in, void* out, void* stream);
+/* out[r][c] = in[r][c]*scale[c] + shift[c] for c < cin and 0 for cin <= c < cout
+ * (normalise and zero-pad features); FP32 and/or bf16 output with row
+ * stride ldo; scale/shift may be NULL (1 and 0). */
+int dtb_mlp_affine(int32_t rows, int32_t cin, int32_t cout, const float* in, int64_t ldi, const float* scale,
+                   const float* shift, float* out, void* outb, int64_t ldo, void* stream);
+/* round-to-nearest-even FP32 -> bf16 */
+int dtb_mlp_f32_to_bf16(int64_t n, const float* in, void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

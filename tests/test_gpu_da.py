@@ -160,3 +160,24 @@ def test_c4_runner_small():
     assert out["windows"] == 3 and np.isfinite(out["window_s_mean"])
     assert rec["w_save"].shape == (3, 6, 4, 2 * 2 + 6)
     assert np.all(np.isfinite(rec["obs"]))
+
+
+def test_c5_surrogate_da_small():
+    """Surrogate-assisted DA: fallback every 5th window (and after any window
+    over the NRMSE limit), finite forecasts, surrogate windows faster than
+    simulated ones."""
+    from digital_twin_brain_amd.da_runner import run_c5
+
+    out, rec = run_c5(E=8, voxels=4, train_windows=12, windows=7, window_ms=50.0, hidden=(128, 128), epochs=60,
+                      finetune_epochs=5, n_lo=500, n_hi=3000, k_lo=40, k_hi=80, truth_n=1000, truth_k=60)
+    sim = rec["sur"]["simulated"]
+    assert sim[0] and sim[5]
+    assert out["simulated_windows"] == int(sim.sum()) and len(out["nrmse"]) == out["simulated_windows"]
+    for (t, e), nxt in zip(rec["sur"]["nrmse"], range(7)):
+        if e > 0.02 and t + 1 < 7:
+            assert sim[t + 1]
+    assert np.all(np.isfinite(rec["sur"]["forecasts"]))
+    times = rec["sur"]["times"]
+    if (~sim).any():
+        assert times[~sim].mean() < times[sim].mean()
+    assert out["fit_loss_last"] < out["fit_loss_first"]
