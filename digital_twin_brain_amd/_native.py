@@ -104,6 +104,8 @@ def lib():
         "dtb_mlp_transpose_bf16": [_i32, _i32, _vp, _vp, _vp],
         "dtb_mlp_f32_to_bf16": [_i64, _vp, _vp, _vp],
         "dtb_mlp_affine": [_i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
+        "dtb_gru_forward": [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
+        "dtb_gru_backward": [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -124,7 +126,8 @@ def exported_symbols():
             "dtb_bold_step_f32", "dtb_bold_step_f64",
             "dtb_enkf_analysis_f64",
             "dtb_mlp_gemm_bf16_nt", "dtb_mlp_relu_backward", "dtb_mlp_bias_grad", "dtb_mlp_mse_grad",
-            "dtb_mlp_adam", "dtb_mlp_transpose_bf16", "dtb_mlp_f32_to_bf16", "dtb_mlp_affine"]
+            "dtb_mlp_adam", "dtb_mlp_transpose_bf16", "dtb_mlp_f32_to_bf16", "dtb_mlp_affine",
+            "dtb_gru_forward", "dtb_gru_backward"]
 
 
 def check(status):

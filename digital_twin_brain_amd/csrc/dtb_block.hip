@@ -353,8 +353,9 @@ static void launch_deliver(const Block* b, const DeliverArgs& da, int32_t k)
 // ---------------------------------------------------------------------------
 // Bucketed delivery.  Random float atomics on MI355X saturate near 2e10/s
 // whatever the footprint, scope or XCD placement, so a large step's events
-// are instead sorted by destination bucket (BucketCfg::kSize consecutive
-// neurons, one 128 KB LDS slice) and accumulated in LDS:
+// are instead sorted by destination bucket (bsize consecutive neurons, one
+// LDS slice of up to 160 KB; bsize is chosen at set-up so that the buckets
+// fill the CUs in as few full waves as the LDS allows) and accumulated in LDS:
 //   phase 1 (deliver_stage_kernel): each workgroup expands 256 (row, delay)
 //     segments at a time into rounds of 256*kEPT events held in registers;
 //     an LDS histogram ranks every event inside its bucket, one global atomic
@@ -366,19 +367,19 @@ static void launch_deliver(const Block* b, const DeliverArgs& da, int32_t k)
 // global atomics into acc2, which phase 2 then folds in.
 template <bool PAIR>
 struct BucketCfg {
-    static constexpr int kBits = PAIR ? 13 : 14;     // 8192 / 16384 neurons per bucket
-    static constexpr int kSize = 1 << kBits;
     static constexpr int kChan = PAIR ? 4 : 2;       // floats per neuron in acc
     static constexpr int kEPT = PAIR ? 16 : 32;      // events per lane per round
-    static constexpr size_t kLds = (size_t)kSize * kChan * sizeof(float); // 128 KB
 };
-constexpr int kSegPerChunk = 256;
+constexpr int kMaxBucketBits = 15;   // neurons per bucket < 2^15 (local index in staged words)
 constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
 constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
 
 struct StageArgs {
     DeliverArgs d;
     int32_t nbk;      // buckets
+    int32_t bsize;    // neurons per bucket (< 2^15)
+    uint64_t bmagic;  // bucket = (post * bmagic) >> bshift == post / bsize for post < 2^26
+    int32_t bshift;
     int64_t cap;      // staged events per bucket
     uint32_t* cursor; // [nbk]
     void* stage;      // [nbk][cap] uint2 (compact) / uint4 (pair)
@@ -405,15 +406,18 @@ __device__ inline uint32_t block_inclusive_scan_256(uint32_t v, uint32_t* wtot)
     return v + add;
 }
 
-template <bool PAIR>
+// SEG (64, 128 or 256) segments per chunk: fewer give more, smaller chunks
+// (parallelism when few neurons spike), more amortise the chunk prologue.
+template <bool PAIR, int SEG>
 __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t k)
 {
     using Cfg = BucketCfg<PAIR>;
     constexpr int EPT = Cfg::kEPT;
+    static_assert(SEG >= 64 && SEG <= 256, "segment index is packed in 8 bits");
     __shared__ uint32_t cum[DTB_MAX_DELAY + 1];
     __shared__ int32_t slot_of[DTB_MAX_DELAY + 1];
-    __shared__ uint32_t segpre[kSegPerChunk + 1];
-    __shared__ int64_t segbeg[kSegPerChunk];
+    __shared__ uint32_t segpre[SEG + 1];
+    __shared__ int64_t segbeg[SEG];
     __shared__ uint32_t hist[kMaxBucketsLds];
     __shared__ uint32_t wtot[4];
     const DeliverArgs& a = s.d;
@@ -433,14 +437,14 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     __syncthreads();
     const uint32_t total = cum[a.D];
     if (blockIdx.x == 0 && t == 0 && a.rec_nspk) a.rec_nspk[k] = cum[1];
-    const uint32_t nchunks = (total + kSegPerChunk - 1) / kSegPerChunk;
+    const uint32_t nchunks = (total + SEG - 1) / SEG;
     uint64_t ev_wg = 0;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         // segment lengths of this chunk -> LDS prefix
-        const uint32_t g = c * kSegPerChunk + t;
+        const uint32_t g = c * SEG + t;
         uint32_t len = 0;
         int64_t beg = 0;
-        if (g < total) {
+        if (t < SEG && g < total) {
             int32_t d = 1;
             while (g >= cum[d]) ++d;
             const int32_t j = a.spk[(int64_t)slot_of[d] * a.n + (g - cum[d - 1])];
@@ -456,11 +460,14 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
             }
             len = (uint32_t)(end - beg);
         }
-        segbeg[t] = beg;
-        segpre[t + 1] = block_inclusive_scan_256(len, wtot);
+        const uint32_t inc = block_inclusive_scan_256(len, wtot);
+        if (t < SEG) {
+            segbeg[t] = beg;
+            segpre[t + 1] = inc;
+        }
         if (t == 0) segpre[0] = 0;
         __syncthreads();
-        const uint32_t E = segpre[kSegPerChunk];
+        const uint32_t E = segpre[SEG];
         ev_wg += E;
         for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
             uint32_t br[EPT], pk_[EPT]; // br = bucket << kRankBits | rank
@@ -502,8 +509,8 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                 br[m] = 0xffffffffu;
                 if (loc[m] != 0xffffffffu) {
                     const uint32_t post = cw[m] & a.pk.idx_mask;
-                    const uint32_t bk = post >> Cfg::kBits;
-                    pk_[m] = (post & (Cfg::kSize - 1)) | (cw[m] & 0x80000000u);
+                    const uint32_t bk = (uint32_t)(((uint64_t)post * s.bmagic) >> s.bshift);
+                    pk_[m] = (post - bk * (uint32_t)s.bsize) | (cw[m] & 0x80000000u);
                     br[m] = (bk << kRankBits) | atomicAdd(&hist[bk], 1u);
                 }
             }
@@ -527,7 +534,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                     else
                         reinterpret_cast<uint2*>(s.stage)[at] = make_uint2(pk_[m], __float_as_uint(w0[m]));
                 } else {
-                    const int64_t post = ((int64_t)bk << Cfg::kBits) + (pk_[m] & (Cfg::kSize - 1));
+                    const int64_t post = (int64_t)bk * s.bsize + (pk_[m] & 0x7fffffffu);
                     const uint32_t kind = pk_[m] >> 31;
                     if (PAIR) {
                         float* dst = reinterpret_cast<float*>(s.acc2) + (post << 2) + (kind << 1);
@@ -548,15 +555,26 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
 }
 
 template <bool PAIR>
+static void launch_stage(const Block* b, const StageArgs& sa, int32_t k)
+{
+    const dim3 g(b->deliver_grid), t(256);
+    switch (b->stage_segs) {
+    case 64: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 64>), g, t, 0, b->stream, sa, k); break;
+    case 128: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 128>), g, t, 0, b->stream, sa, k); break;
+    default: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 256>), g, t, 0, b->stream, sa, k); break;
+    }
+}
+
+template <bool PAIR>
 __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 {
     using Cfg = BucketCfg<PAIR>;
     constexpr int C = Cfg::kChan;
-    extern __shared__ float slice[]; // [kSize][C]
+    extern __shared__ float slice[]; // [bsize][C]
     const DeliverArgs& a = s.d;
     const int32_t b = blockIdx.x;
     const int t = threadIdx.x;
-    for (int x = t; x < Cfg::kSize * C; x += blockDim.x) slice[x] = 0.0f;
+    for (int x = t; x < s.bsize * C; x += blockDim.x) slice[x] = 0.0f;
     const uint32_t cnt = (uint32_t)min((uint64_t)s.cursor[b], (uint64_t)s.cap);
     const bool ovf = s.ovf[b] != 0u;
     __syncthreads();
@@ -573,7 +591,7 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (base + (uint32_t)u * blockDim.x >= cnt) break;
-                const uint32_t o = (v[u].x & (Cfg::kSize - 1)) * 4 + ((v[u].x >> 31) << 1);
+                const uint32_t o = (v[u].x & 0x7fffffffu) * 4 + ((v[u].x >> 31) << 1);
                 atomicAdd(&slice[o], __uint_as_float(v[u].y));
                 atomicAdd(&slice[o + 1], __uint_as_float(v[u].z));
             }
@@ -587,13 +605,13 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (base + (uint32_t)u * blockDim.x >= cnt) break;
-                atomicAdd(&slice[(v[u].x & (Cfg::kSize - 1)) * 2 + (v[u].x >> 31)], __uint_as_float(v[u].y));
+                atomicAdd(&slice[(v[u].x & 0x7fffffffu) * 2 + (v[u].x >> 31)], __uint_as_float(v[u].y));
             }
         }
     }
     __syncthreads();
-    const int64_t i0 = (int64_t)b << Cfg::kBits;
-    const int32_t nloc = (int32_t)min((int64_t)Cfg::kSize, a.n - i0);
+    const int64_t i0 = (int64_t)b * s.bsize;
+    const int32_t nloc = (int32_t)min((int64_t)s.bsize, a.n - i0);
     if (PAIR) {
         float4* acc = reinterpret_cast<float4*>(a.acc) + i0;
         float4* acc2 = reinterpret_cast<float4*>(s.acc2) + i0;
@@ -1132,16 +1150,28 @@ static int ensure_bucketed(Block* b)
     if (b->stage) return DTB_OK;
     DTB_REQUIRE(b->n <= (1ll << 26), "bucketed delivery supports n <= 2^26");
     DTB_REQUIRE(b->max_row < (1ll << 24), "bucketed delivery supports rows of fewer than 2^24 synapses");
-    static bool attr_done = false;
-    if (!attr_done) {
-        DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)BucketCfg<true>::kLds));
-        DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)BucketCfg<false>::kLds));
-        attr_done = true;
-    }
-    const int64_t bsz = b->pair ? BucketCfg<true>::kSize : BucketCfg<false>::kSize;
-    b->nbk = (int32_t)((b->n + bsz - 1) / bsz);
+    // bucket size: the largest LDS slice the device allows (at most 160 KB),
+    // then as few full waves of one bucket per CU as that permits
+    int ncu = 0, lds = 0;
+    DTB_TRY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+    DTB_TRY_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
+    lds = std::min(std::max(lds, 131072), 163840);
+    const int64_t bpn = b->pair ? 16 : 8; // LDS bytes per neuron
+    const int64_t max_bs = std::min<int64_t>((1 << kMaxBucketBits) - 64, lds / bpn / 64 * 64);
+    const int64_t per_wave = (int64_t)std::max(ncu, 1) * max_bs;
+    const int64_t waves = (b->n + per_wave - 1) / per_wave;
+    const int64_t per = (b->n + waves * std::max(ncu, 1) - 1) / (waves * std::max(ncu, 1));
+    b->bsize = (int32_t)std::min<int64_t>(max_bs, std::max<int64_t>(64, (per + 63) / 64 * 64));
+    b->bshift = 26;
+    while ((1ll << (b->bshift - 26)) < b->bsize) ++b->bshift;
+    b->bmagic = ((1ull << b->bshift) + (uint64_t)b->bsize - 1) / (uint64_t)b->bsize;
+    b->nbk = (int32_t)((b->n + b->bsize - 1) / b->bsize);
+    DTB_REQUIRE(b->nbk <= kMaxBucketsLds, "too many delivery buckets");
+    // the attribute is per kernel and process-wide: allow the largest slice any block may use
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     // room for 1% of all synapses per step (a mean rate of 100 Hz at 0.1 ms),
     // at least 8192 events per bucket; fuller buckets fall back to atomics
     b->stage_cap = std::max<int64_t>(8192, (b->nnz / 100 + b->nbk - 1) / b->nbk);
@@ -1420,14 +1450,14 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         }
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
         if (b->deliver_mode == 1) {
-            StageArgs sa{da, b->nbk, b->stage_cap, b->cursor, b->stage, b->acc2, b->ovf};
+            StageArgs sa{da, b->nbk, b->bsize, b->bmagic, b->bshift, b->stage_cap, b->cursor, b->stage, b->acc2, b->ovf};
             if (b->pair) {
-                hipLaunchKernelGGL(deliver_stage_kernel<true>, dim3(b->deliver_grid), dim3(256), 0, b->stream, sa, k);
-                hipLaunchKernelGGL(deliver_accum_kernel<true>, dim3(b->nbk), dim3(1024), BucketCfg<true>::kLds,
+                launch_stage<true>(b, sa, k);
+                hipLaunchKernelGGL(deliver_accum_kernel<true>, dim3(b->nbk), dim3(1024), (size_t)b->bsize * 16,
                                    b->stream, sa);
             } else {
-                hipLaunchKernelGGL(deliver_stage_kernel<false>, dim3(b->deliver_grid), dim3(256), 0, b->stream, sa, k);
-                hipLaunchKernelGGL(deliver_accum_kernel<false>, dim3(b->nbk), dim3(1024), BucketCfg<false>::kLds,
+                launch_stage<false>(b, sa, k);
+                hipLaunchKernelGGL(deliver_accum_kernel<false>, dim3(b->nbk), dim3(1024), (size_t)b->bsize * 8,
                                    b->stream, sa);
             }
         } else if (b->pair) {
@@ -2076,6 +2106,9 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
         DTB_REQUIRE(value == 0 || value == 1, "parameter sets must be 0 (off) or 1 (auto)");
         b->pset_off = value == 0;
         DTB_TRY(build_psets(b));
+    } else if (what == DTB_LAUNCH_STAGE_SEGS) {
+        DTB_REQUIRE(value == 64 || value == 128 || value == 256, "segments per chunk must be 64, 128 or 256");
+        b->stage_segs = value;
     } else if (what == DTB_LAUNCH_DELIVER_MODE) {
         DTB_REQUIRE(value == 0 || value == 1, "delivery mode must be 0 (atomics) or 1 (bucketed)");
         if (value == 1) DTB_TRY(ensure_bucketed(b));
@@ -2096,6 +2129,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     else if (what == DTB_LAUNCH_DELIVER_LANES) *value = b->deliver_lanes;
     else if (what == DTB_LAUNCH_DELIVER_MODE) *value = b->deliver_mode;
     else if (what == DTB_LAUNCH_NEURON_PSETS) *value = b->npsets;
+    else if (what == DTB_LAUNCH_STAGE_SEGS) *value = b->stage_segs;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

@@ -98,6 +98,10 @@ struct Block {
     int32_t deliver_grid = 64;  // workgroups of the delivery kernel
     int32_t deliver_lanes = 64; // lanes per (row, delay) segment
     int32_t deliver_mode = 0;   // 0: global atomics, 1: bucketed (staged + LDS accumulation)
+    int32_t stage_segs = 256;   // bucketed: (row, delay) segments per staging chunk
+    int32_t bsize = 0;          // bucketed: neurons per destination bucket
+    uint64_t bmagic = 0;        // bucket = (post * bmagic) >> bshift
+    int32_t bshift = 0;
     int32_t nbk = 0;            // destination buckets
     int64_t stage_cap = 0;      // staged events per bucket
     uint32_t* cursor = nullptr; // [nbk]

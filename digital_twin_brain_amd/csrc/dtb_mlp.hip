@@ -166,6 +166,71 @@ __global__ void affine_kernel(int32_t rows, int32_t cin, int32_t cout, const flo
     if (outb) outb[(int64_t)r * ldo + c] = f2bf(v);
 }
 
+__device__ inline float sigm(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// GRU cell (gate order r, z, n as torch.nn.GRU):
+//   r = s(gi_r + gh_r), z = s(gi_z + gh_z), n = tanh(gi_n + r * gh_n), h' = (1 - z) n + z h
+// gi = x W_ih^T + b_ih and gh = h W_hh^T + b_hh come from the GEMM, [B][3H] each.
+__global__ void gru_forward_kernel(int32_t B, int32_t H, const float* __restrict__ gi, const float* __restrict__ gh,
+                                   const float* __restrict__ h, float* __restrict__ hn, uint16_t* __restrict__ hnb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * H) return;
+    const int64_t b = i / H, j = i % H;
+    const float* a = gi + b * 3 * H;
+    const float* c = gh + b * 3 * H;
+    const float r = sigm(a[j] + c[j]);
+    const float z = sigm(a[H + j] + c[H + j]);
+    const float n = tanhf(a[2 * H + j] + r * c[2 * H + j]);
+    const float v = (1.0f - z) * n + z * h[i];
+    hn[i] = v;
+    if (hnb) hnb[i] = f2bf(v);
+}
+
+// Backward of one GRU step with the gates recomputed from gi, gh, h.
+// dh = dh_a + dh_b + dh_c (readout gradient, direct and GEMM parts of the
+// next step's recurrent gradient; dh_b/dh_c may be NULL).  Writes
+// dgi = [dr', dz', dn'] and dgh = [dr', dz', dn' * r] (FP32 and bf16) and the
+// direct recurrent part dh_z = dh * z.
+__global__ void gru_backward_kernel(int32_t B, int32_t H, const float* __restrict__ gi, const float* __restrict__ gh,
+                                    const float* __restrict__ h, const float* __restrict__ dh_a,
+                                    const float* __restrict__ dh_b, const float* __restrict__ dh_c,
+                                    float* __restrict__ dgi, uint16_t* __restrict__ dgib, float* __restrict__ dgh,
+                                    uint16_t* __restrict__ dghb, float* __restrict__ dh_z)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * H) return;
+    const int64_t b = i / H, j = i % H;
+    const int64_t o = b * 3 * H;
+    const float r = sigm(gi[o + j] + gh[o + j]);
+    const float z = sigm(gi[o + H + j] + gh[o + H + j]);
+    const float ghn = gh[o + 2 * H + j];
+    const float n = tanhf(gi[o + 2 * H + j] + r * ghn);
+    float d = dh_a[i];
+    if (dh_b) d += dh_b[i];
+    if (dh_c) d += dh_c[i];
+    const float dn = d * (1.0f - z) * (1.0f - n * n);
+    const float dz = d * (h[i] - n) * z * (1.0f - z);
+    const float dr = dn * ghn * r * (1.0f - r);
+    dh_z[i] = d * z;
+    dgi[o + j] = dr;
+    dgi[o + H + j] = dz;
+    dgi[o + 2 * H + j] = dn;
+    dgh[o + j] = dr;
+    dgh[o + H + j] = dz;
+    dgh[o + 2 * H + j] = dn * r;
+    if (dgib) {
+        dgib[o + j] = f2bf(dr);
+        dgib[o + H + j] = f2bf(dz);
+        dgib[o + 2 * H + j] = f2bf(dn);
+    }
+    if (dghb) {
+        dghb[o + j] = f2bf(dr);
+        dghb[o + H + j] = f2bf(dz);
+        dghb[o + 2 * H + j] = f2bf(dn * r);
+    }
+}
+
 static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 } // namespace dtb
@@ -256,6 +321,27 @@ int dtb_mlp_affine(int32_t rows, int32_t cin, int32_t cout, const float* in, int
     if ((int64_t)rows * cout == 0) return DTB_OK;
     hipLaunchKernelGGL(affine_kernel, g1((int64_t)rows * cout), dim3(256), 0, (hipStream_t)stream, rows, cin, cout,
                        in, ldi, scale, shift, out, (uint16_t*)outb, ldo);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_gru_forward(int32_t B, int32_t H, const float* gi, const float* gh, const float* h, float* hn, void* hnb,
+                    void* stream)
+{
+    DTB_REQUIRE(B > 0 && H > 0 && gi && gh && h && hn, "bad arguments");
+    hipLaunchKernelGGL(gru_forward_kernel, g1((int64_t)B * H), dim3(256), 0, (hipStream_t)stream, B, H, gi, gh, h, hn,
+                       (uint16_t*)hnb);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
+int dtb_gru_backward(int32_t B, int32_t H, const float* gi, const float* gh, const float* h, const float* dh_a,
+                     const float* dh_b, const float* dh_c, float* dgi, void* dgib, float* dgh, void* dghb, float* dh_z,
+                     void* stream)
+{
+    DTB_REQUIRE(B > 0 && H > 0 && gi && gh && h && dh_a && dgi && dgh && dh_z, "bad arguments");
+    hipLaunchKernelGGL(gru_backward_kernel, g1((int64_t)B * H), dim3(256), 0, (hipStream_t)stream, B, H, gi, gh, h,
+                       dh_a, dh_b, dh_c, dgi, (uint16_t*)dgib, dgh, (uint16_t*)dghb, dh_z);
     DTB_TRY_HIP(hipGetLastError());
     return DTB_OK;
 }

@@ -162,7 +162,7 @@ class NativeBlock:
         nat.check(nat.lib().dtb_block_bold_state(self._h, self.BOLD_FIELDS[which], 1, nat.ptr(v)))
 
     # -------------------------------------------------------- launch shape
-    LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2, neuron_psets=3)
+    LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2, neuron_psets=3, stage_segs=4)
 
     def set_launch(self, **kw):
         for k, v in kw.items():

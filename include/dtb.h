@@ -216,6 +216,7 @@ int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx);
 #define DTB_LAUNCH_DELIVER_MODE 2  /* 0: global atomics, 1: bucketed LDS accumulation */
 #define DTB_LAUNCH_NEURON_PSETS 3  /* set: 0 per-neuron constants, 1 shared parameter sets when <= 64
                                       distinct; get: number of sets in use (0 = per-neuron) */
+#define DTB_LAUNCH_STAGE_SEGS 4    /* bucketed delivery: 64, 128 or 256 (row, delay) segments per chunk */
 int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
 int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 
@@ -264,6 +265,18 @@ int dtb_mlp_transpose_bf16(int32_t M, int32_t N, const void* in, void* out, void
  * stride ldo; scale/shift may be NULL (1 and 0). */
 int dtb_mlp_affine(int32_t rows, int32_t cin, int32_t cout, const float* in, int64_t ldi, const float* scale,
                    const float* shift, float* out, void* outb, int64_t ldo, void* stream);
+/* GRU cell, gate order (r, z, n) as torch.nn.GRU.  gi = x W_ih^T + b_ih and
+ * gh = h W_hh^T + b_hh are [B][3H] FP32 (from dtb_mlp_gemm_bf16_nt);
+ * forward: hn = (1-z) n + z h with r = sigmoid(gi_r+gh_r),
+ * z = sigmoid(gi_z+gh_z), n = tanh(gi_n + r gh_n); FP32 and optional bf16. */
+int dtb_gru_forward(int32_t B, int32_t H, const float* gi, const float* gh, const float* h, float* hn, void* hnb,
+                    void* stream);
+/* backward of one step, gates recomputed: with dh = dh_a + dh_b + dh_c
+ * (dh_b, dh_c may be NULL) writes the gradients of gi and gh (FP32, optional
+ * bf16) and dh_z = dh * z; the full gradient of h is dh_z + dgh W_hh. */
+int dtb_gru_backward(int32_t B, int32_t H, const float* gi, const float* gh, const float* h, const float* dh_a,
+                     const float* dh_b, const float* dh_c, float* dgi, void* dgib, float* dgh, void* dghb, float* dh_z,
+                     void* stream);
 /* round-to-nearest-even FP32 -> bf16 */
 int dtb_mlp_f32_to_bf16(int64_t n, const float* in, void* out, void* stream);
 

@@ -195,3 +195,51 @@ def test_surrogate_predict_matches_oracle_and_training_fits():
     np.testing.assert_allclose(pred, want, rtol=1e-3, atol=1e-3 * np.abs(want).max())
     nrmse = np.linalg.norm(pred - Y[:300]) / np.sqrt(pred.size) / (Y.max() - Y.min())
     assert nrmse < 0.05, nrmse
+
+
+def test_gru_gradients_match_fp64_oracle():
+    """BPTT of the GRU surrogate on the GPU against the FP64 oracle (itself
+    pinned to torch.nn.GRU) with bf16 rounding at the same points."""
+    from digital_twin_brain_amd.surrogate import GRUSurrogate
+    from oracle import mlp_oracle as mo
+
+    rng = np.random.default_rng(11)
+    T, r, n_in, n_out, H = 6, 40, 50, 30, 64
+    sur = GRUSurrogate(n_in, n_out, hidden=H, max_batch=64, max_steps=8, seed=2)
+    sur.p["b_o"].copy_(torch.from_numpy(rng.normal(size=sur.O) * 0.1).float())
+    X = rng.normal(size=(T, r, n_in))
+    Y = rng.normal(size=(T, r, n_out))
+    h0 = rng.normal(size=(r, H)) * 0.3
+    ls = sur.gradients(torch.from_numpy(X), torch.from_numpy(Y), h0=torch.from_numpy(h0))
+    torch.cuda.synchronize()
+    P = sur.params_numpy()
+    Xp = np.zeros((T, r, sur.I)); Xp[:, :, :n_in] = X
+    Tp = np.zeros((T, r, sur.O)); Tp[:, :, :n_out] = Y
+    Mp = np.zeros((T, r, sur.O)); Mp[:, :, :n_out] = 1.0
+    h0f = h0.astype(np.float32).astype(np.float64)
+    Yo, cache = mo.gru_forward(P, Xp, h0f, bf16=True)
+    loss, gY = mo.loss_and_grad(Yo, Tp, Mp, 1.0 / (T * r))
+    g = mo.gru_backward(P, cache, gY, bf16=True)
+    B = 64  # 40 rows padded to a multiple of 32
+    Yg = sur.Y[:T * B].cpu().double().numpy().reshape(T, B, -1)[:, :r]
+    np.testing.assert_allclose(Yg, Yo, rtol=1e-3, atol=1e-3)
+    assert abs(float(ls.sum()) / (T * r) / 2 - loss) <= 1e-3 * loss
+    for k in ("W_ih", "b_ih", "W_hh", "b_hh", "W_o", "b_o"):
+        assert _rel(sur.grad[k].cpu().double().numpy(), g[k]) < 5e-3, k
+
+
+def test_gru_training_fits_a_delayed_sequence_task():
+    """y_t = x_{t-1} (a one-step memory): only the recurrence can fit it."""
+    from digital_twin_brain_amd.surrogate import GRUSurrogate
+
+    rng = np.random.default_rng(12)
+    T, n, d = 8, 512, 8
+    X = rng.normal(size=(T, n, d))
+    Y = np.concatenate([np.zeros((1, n, d)), X[:-1]], axis=0)
+    sur = GRUSurrogate(d, d, hidden=64, max_batch=64, max_steps=T, seed=1, lr=3e-3)
+    hist = sur.fit(torch.from_numpy(X), torch.from_numpy(Y), epochs=30, batch=64)
+    assert hist[-1] < 0.2 * hist[0], hist
+    pred, hT = sur.predict(torch.from_numpy(X[:, :64]))
+    err = pred.cpu().double().numpy()[1:] - Y[1:, :64]
+    assert np.sqrt((err ** 2).mean()) < 0.4
+    assert hT.shape == (64, 64)

@@ -91,3 +91,68 @@ def test_bf16_round_matches_torch():
                         [0.0, -0.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, np.inf, -np.inf]])
     want = torch.from_numpy(x.astype(np.float32)).to(torch.bfloat16).double().numpy()
     assert np.array_equal(mo.bf16_round(x), want)
+
+
+def _gru_params(rng, I, H, O):
+    s = 1 / np.sqrt(H)
+    return dict(W_ih=rng.uniform(-s, s, (3 * H, I)), b_ih=rng.uniform(-s, s, 3 * H),
+                W_hh=rng.uniform(-s, s, (3 * H, H)), b_hh=rng.uniform(-s, s, 3 * H),
+                W_o=rng.normal(size=(O, H)) * s, b_o=rng.normal(size=O) * 0.1)
+
+
+def _torch_gru(p, X, h0):
+    I, H = p["W_ih"].shape[1], p["W_hh"].shape[1]
+    gru = torch.nn.GRU(I, H, dtype=torch.float64)
+    lin = torch.nn.Linear(H, p["W_o"].shape[0], dtype=torch.float64)
+    with torch.no_grad():
+        gru.weight_ih_l0.copy_(torch.from_numpy(p["W_ih"]))
+        gru.bias_ih_l0.copy_(torch.from_numpy(p["b_ih"]))
+        gru.weight_hh_l0.copy_(torch.from_numpy(p["W_hh"]))
+        gru.bias_hh_l0.copy_(torch.from_numpy(p["b_hh"]))
+        lin.weight.copy_(torch.from_numpy(p["W_o"]))
+        lin.bias.copy_(torch.from_numpy(p["b_o"]))
+    out, _ = gru(torch.from_numpy(X), torch.from_numpy(h0)[None])
+    return lin(out), gru, lin
+
+
+def test_gru_forward_and_backward_match_torch_gru_fp64():
+    rng = np.random.default_rng(9)
+    T, B, I, H, O = 5, 3, 4, 6, 2
+    p = _gru_params(rng, I, H, O)
+    X = rng.normal(size=(T, B, I))
+    h0 = rng.normal(size=(B, H)) * 0.5
+    Tg = rng.normal(size=(T, B, O))
+    Y, cache = mo.gru_forward(p, X, h0)
+    Yt, gru, lin = _torch_gru(p, X, h0)
+    np.testing.assert_allclose(Y, Yt.detach().numpy(), rtol=1e-12, atol=1e-13)
+    loss, gY = mo.loss_and_grad(Y, Tg, None, 0.3)
+    g = mo.gru_backward(p, cache, gY)
+    (0.5 * 0.3 * ((Yt - torch.from_numpy(Tg)) ** 2).sum()).backward()
+    pairs = dict(W_ih=gru.weight_ih_l0, b_ih=gru.bias_ih_l0, W_hh=gru.weight_hh_l0, b_hh=gru.bias_hh_l0,
+                 W_o=lin.weight, b_o=lin.bias)
+    for k, t in pairs.items():
+        np.testing.assert_allclose(g[k], t.grad.numpy(), rtol=1e-10, atol=1e-12, err_msg=k)
+
+
+def test_gru_backward_central_differences():
+    rng = np.random.default_rng(10)
+    T, B, I, H, O = 4, 2, 3, 5, 2
+    p = _gru_params(rng, I, H, O)
+    X = rng.normal(size=(T, B, I))
+    Tg = rng.normal(size=(T, B, O))
+
+    def loss():
+        return mo.loss_and_grad(mo.gru_forward(p, X)[0], Tg)[0]
+
+    Y, cache = mo.gru_forward(p, X)
+    g = mo.gru_backward(p, cache, mo.loss_and_grad(Y, Tg)[1])
+    for k in ("W_hh", "b_ih", "W_o"):
+        arr = p[k]
+        for idx in list(np.ndindex(arr.shape))[:12]:
+            old = arr[idx]
+            arr[idx] = old + 1e-6
+            lp = loss()
+            arr[idx] = old - 1e-6
+            lm = loss()
+            arr[idx] = old
+            assert abs((lp - lm) / 2e-6 - g[k][idx]) <= 1e-6 * max(1.0, abs(g[k][idx])), (k, idx)
