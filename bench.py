@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--da-windows", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true", help="launch step batches directly (for counter runs)")
+    ap.add_argument("--launch", default="", metavar="MODE,GRID,SEGS",
+                    help="fixed delivery launch shape instead of the autotuner (e.g. 1,1024,128)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -135,7 +137,11 @@ def main():
     if W:
         nb.run(W, noise_mode=1, graph=graph)
     log(f"warm-up {W} steps done")
-    if not a.no_tune:
+    if a.launch:
+        mode, grid, segs = (int(x) for x in a.launch.split(","))
+        nb.set_launch(deliver_mode=mode, deliver_grid=grid, stage_segs=segs)
+        log(f"fixed delivery launch: mode={mode} grid={grid} segs={segs}")
+    elif not a.no_tune:
         from digital_twin_brain_amd.autotune import tune_delivery
 
         best, table = tune_delivery(nb, dict(noise_mode=1), steps=20, use_cache=False)
