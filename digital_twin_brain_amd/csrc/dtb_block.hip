@@ -371,6 +371,9 @@ struct BucketCfg {
     static constexpr int kEPT = PAIR ? 16 : 32;      // events per lane per round
 };
 constexpr int kMaxBucketBits = 15;   // neurons per bucket < 2^15 (local index in staged words)
+constexpr int kCursorStride = 32;    // one 128-byte line per bucket cursor / overflow flag: every
+                                     // workgroup round hits every cursor, and atomics on one line
+                                     // serialise
 constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
 constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
 
@@ -381,7 +384,7 @@ struct StageArgs {
     uint64_t bmagic;  // bucket = (post * bmagic) >> bshift == post / bsize for post < 2^26
     int32_t bshift;
     int64_t cap;      // staged events per bucket
-    uint32_t* cursor; // [nbk]
+    uint32_t* cursor; // [nbk * kCursorStride]
     void* stage;      // [nbk][cap] uint2 (compact) / uint4 (pair)
     void* acc2;       // overflow accumulator [n]
     uint32_t* ovf;    // [nbk] overflow flags
@@ -518,7 +521,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
             // reserve space in each touched bucket; hist[b] becomes the base
             for (int b = t; b < s.nbk; b += 256) {
                 const uint32_t h = hist[b];
-                hist[b] = h ? atomicAdd(&s.cursor[b], h) : 0u;
+                hist[b] = h ? atomicAdd(&s.cursor[(size_t)b * kCursorStride], h) : 0u;
             }
             __syncthreads();
 #pragma unroll
@@ -543,7 +546,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                     } else {
                         unsafeAtomicAdd(reinterpret_cast<float*>(s.acc2) + (post << 1) + kind, w0[m]);
                     }
-                    s.ovf[bk] = 1u;
+                    s.ovf[(size_t)bk * kCursorStride] = 1u;
                 }
             }
             __syncthreads();
@@ -575,8 +578,8 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     const int32_t b = blockIdx.x;
     const int t = threadIdx.x;
     for (int x = t; x < s.bsize * C; x += blockDim.x) slice[x] = 0.0f;
-    const uint32_t cnt = (uint32_t)min((uint64_t)s.cursor[b], (uint64_t)s.cap);
-    const bool ovf = s.ovf[b] != 0u;
+    const uint32_t cnt = (uint32_t)min((uint64_t)s.cursor[(size_t)b * kCursorStride], (uint64_t)s.cap);
+    const bool ovf = s.ovf[(size_t)b * kCursorStride] != 0u;
     __syncthreads();
     constexpr int U = 8; // staged events in flight per lane
     const int64_t sb = (int64_t)b * s.cap;
@@ -639,8 +642,8 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     }
     __syncthreads();
     if (t == 0) {
-        s.cursor[b] = 0u;
-        s.ovf[b] = 0u;
+        s.cursor[(size_t)b * kCursorStride] = 0u;
+        s.ovf[(size_t)b * kCursorStride] = 0u;
     }
 }
 
@@ -1176,12 +1179,12 @@ static int ensure_bucketed(Block* b)
     // at least 8192 events per bucket; fuller buckets fall back to atomics
     b->stage_cap = std::max<int64_t>(8192, (b->nnz / 100 + b->nbk - 1) / b->nbk);
     const size_t esz = b->pair ? 16 : 8;
-    DTB_TRY(alloc_n(b, &b->cursor, b->nbk));
-    DTB_TRY(alloc_n(b, &b->ovf, b->nbk));
+    DTB_TRY(alloc_n(b, &b->cursor, (int64_t)b->nbk * kCursorStride));
+    DTB_TRY(alloc_n(b, &b->ovf, (int64_t)b->nbk * kCursorStride));
     DTB_TRY(block_alloc(b, &b->stage, esz * (size_t)b->nbk * (size_t)b->stage_cap));
     DTB_TRY(block_alloc(b, &b->acc2, (b->pair ? 16 : 8) * (size_t)b->n));
-    DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)b->nbk, b->stream));
-    DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)b->nbk, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)b->nbk * kCursorStride, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)b->nbk * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->acc2, 0, (b->pair ? 16 : 8) * (size_t)b->n, b->stream));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     return DTB_OK;

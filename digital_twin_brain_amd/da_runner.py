@@ -19,9 +19,10 @@ from .data_assimilation import DataAssimilation
 HP_COLUMNS = (10, 12)  # g_AMPA and g_GABAa of every sub-population
 
 
-def _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw):
+def _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw, tune=True):
     """Block of E members + truth, DataAssimilation over the members, random
-    initial hyper-parameters pushed, truth factors drawn and pushed."""
+    initial hyper-parameters pushed, truth factors drawn and pushed, delivery
+    launch shape autotuned on the live network."""
     nb, props, mem = networks.build_c4(E=E, voxels=voxels, **member_kw)
     P = voxels * 2
     sps = int(round(1.0 / dt))  # BOLD once per ms
@@ -38,14 +39,19 @@ def _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw):
     truth_hp = rng.uniform(hp_lo, hp_hi, size=(P, H)).astype(np.float32)
     nb.mul_props_by_pop(np.repeat(truth_pops, H).astype(np.uint32), np.tile(HP_COLUMNS, P).astype(np.uint32),
                         truth_hp.reshape(-1))
+    if tune:
+        from .autotune import tune_delivery
+
+        nb.run(10 * sps, noise_mode=1, graph=False)  # leave the initial transient before measuring
+        tune_delivery(nb, dict(noise_mode=1), steps=2 * sps)
     return nb, da, truth_hp, (low, high)
 
 
-def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, log=None,
+def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, log=None, tune=True,
            **member_kw):
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
-    nb, da, truth_hp, _ = _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw)
+    nb, da, truth_hp, _ = _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw, tune)
     t_build = time.perf_counter() - t0
     steps = int(round(window_ms / dt))
     truth_vox = slice(E * voxels, (E + 1) * voxels)
@@ -62,7 +68,11 @@ def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_h
     bold_da = w_save[:, :, :, -1]  # [T, E, V] forecast BOLD
     obs = np.stack(obs)
     err = bold_da.mean(1) - obs
+    spk = np.zeros(100, np.uint32)
+    ev = np.zeros(100, np.uint64)
+    nb.run(100, noise_mode=1, graph=True, spikes=spk, events=ev)
     out = dict(windows=windows, steps_per_window=steps, window_s_mean=float(times.mean()),
+               events_per_step=float(ev.mean()), spikes_per_step=float(spk.mean()), launch=nb.get_launch(),
                window_s_median=float(np.median(times)), build_s=t_build, neurons=int(nb.n), synapses=int(nb.nnz),
                members=E, voxels=voxels, ensemble_bold_rmse_first=float(np.sqrt((err[0] ** 2).mean())),
                ensemble_bold_rmse_last=float(np.sqrt((err[-1] ** 2).mean())))
@@ -72,7 +82,8 @@ def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_h
 
 
 def run_c5(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
-           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=200, finetune_epochs=20, log=None, **member_kw):
+           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=200, finetune_epochs=20, log=None, tune=True,
+           **member_kw):
     """Surrogate-assisted DA against simulator-only DA on the same ensemble.
 
     1. `train_windows` simulator windows with fresh random hyper-parameters
@@ -89,7 +100,7 @@ def run_c5(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.
 
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
-    nb, da, truth_hp, (low, high) = _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw)
+    nb, da, truth_hp, (low, high) = _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw, tune)
     t_build = time.perf_counter() - t0
     steps = int(round(window_ms / dt))
     truth_vox = slice(E * voxels, (E + 1) * voxels)
@@ -125,6 +136,7 @@ def run_c5(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.
     res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
     rmse = lambda w: np.sqrt(((w[:, :, :, -1].mean(1) - obs) ** 2).mean(1))
     out = dict(members=E, voxels=voxels, windows=windows, steps_per_window=steps, build_s=t_build,
+               neurons=int(nb.n), synapses=int(nb.nnz), surrogate="MLP " + "x".join(str(h) for h in hidden),
                train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect, fit_s=t_fit,
                fit_loss_first=hist[0], fit_loss_last=hist[-1], sim_window_s_mean=float(t_sim.mean()),
                sur_window_s_mean=float(res["times"].mean()),
@@ -133,4 +145,4 @@ def run_c5(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.
                bold_rmse_sim_last=float(rmse(w_sim)[-1]), bold_rmse_sur_last=float(rmse(res["forecasts"])[-1]))
     if log:
         log(f"C5 surrogate DA: {out}")
-    return out, dict(w_sim=w_sim, sur=res, obs=obs, hist=hist, truth_hp=truth_hp)
+    return out, dict(w_sim=w_sim, sur=res, obs=obs, hist=hist, truth_hp=truth_hp, da=da)
