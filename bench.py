@@ -66,7 +66,7 @@ def cpu_baseline(n_sample, in_degree, steps_warm, steps_timed):
     from digital_twin_brain_amd import networks
     from oracle import oracle as orc
 
-    nb, props = networks.build_c2(n=n_sample, in_degree=in_degree)
+    nb, props = networks.build_large(n=n_sample, in_degree=in_degree)
     csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
                w0=nb.get("w0"), w1=None)
     nb.close()
@@ -129,8 +129,8 @@ def main():
         return
     graph = not a.no_graph
     t0 = time.perf_counter()
-    nb, props = networks.build_c2(n=a.n, in_degree=a.in_degree)
-    log(f"built C2 network n={nb.n} nnz={nb.nnz} S={nb.S} D={nb.max_delay} in {time.perf_counter() - t0:.1f} s")
+    nb, props = networks.build_large(n=a.n, in_degree=a.in_degree)
+    log(f"built large block n={nb.n} nnz={nb.nnz} S={nb.S} D={nb.max_delay} in {time.perf_counter() - t0:.1f} s")
     nb.attach_bold(steps_per_sample=10, delta_t=1e-3)
     S = nb.S
     # warm-up (also captures the step-batch graph)
@@ -196,15 +196,15 @@ def main():
 
     da = None
     if a.da_windows > 0:
-        from digital_twin_brain_amd.da_runner import run_c5
+        from digital_twin_brain_amd.da_runner import run_surrogate_da
 
         t0 = time.perf_counter()
-        c5, rec = run_c5(E=64, voxels=90, train_windows=a.train_windows, windows=a.da_windows, window_ms=800.0,
+        c5, rec = run_surrogate_da(E=64, voxels=90, train_windows=a.train_windows, windows=a.da_windows, window_ms=800.0,
                          dt=0.1, log=log)
         rec["da"].block_model.nb.close()
         da = {"metric": "DA window seconds (surrogate vs full sim)", "unit": "s", "higher_is_better": False,
               "full_sim_window_s": c5["sim_window_s_mean"], "surrogate_window_s": c5["sur_window_s_mean"],
-              "gain": c5["gain"], "config": "64 uneven members (C4 ensemble), 90 voxels, 800 ms windows, EnKF FP64",
+              "gain": c5["gain"], "config": "64 uneven members (2k-200k neurons, in-degree 100-500) + truth, 90 voxels, 800 ms windows, EnKF FP64",
               "detail": c5, "total_s": time.perf_counter() - t0}
         del rec
 
@@ -220,7 +220,7 @@ def main():
         "metric": "synaptic events/s", "value": value, "unit": "events/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "C2 single large block", "neurons": a.n, "in_degree": a.in_degree,
+        "config": {"workload": "single large block", "neurons": a.n, "in_degree": a.in_degree,
                    "synapses": a.n * a.in_degree, "delays": "1-10 steps", "dt_ms": 0.1,
                    "noise_hz": "5-20 per neuron", "bold": "per 1000-neuron voxel, 1 ms", "weights": "compact fp32"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -1,4 +1,4 @@
-"""Batched data assimilation over an uneven ensemble (the C4 workload).
+"""Batched data assimilation over an uneven ensemble.
 
 One block holds E members of uneven size plus a hidden "truth" member whose
 conductance factors are drawn once and kept fixed.  Every window advances the
@@ -23,7 +23,7 @@ def _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw, tune=True):
     """Block of E members + truth, DataAssimilation over the members, random
     initial hyper-parameters pushed, truth factors drawn and pushed, delivery
     launch shape autotuned on the live network."""
-    nb, props, mem = networks.build_c4(E=E, voxels=voxels, **member_kw)
+    nb, props, mem = networks.build_ensemble(E=E, voxels=voxels, **member_kw)
     P = voxels * 2
     sps = int(round(1.0 / dt))  # BOLD once per ms
     da = DataAssimilation(network=nb, column=False, ensemble=E, dt=dt, bold_steps=sps, da_populations=E * P,
@@ -47,7 +47,7 @@ def _setup(E, voxels, dt, hp_lo, hp_hi, rng, member_kw, tune=True):
     return nb, da, truth_hp, (low, high)
 
 
-def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, log=None, tune=True,
+def run_ensemble_da(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, log=None, tune=True,
            **member_kw):
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
@@ -77,11 +77,11 @@ def run_c4(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_h
                members=E, voxels=voxels, ensemble_bold_rmse_first=float(np.sqrt((err[0] ** 2).mean())),
                ensemble_bold_rmse_last=float(np.sqrt((err[-1] ** 2).mean())))
     if log:
-        log(f"C4 DA: {out}")
+        log(f"ensemble DA: {out}")
     return out, da, dict(w_save=w_save, w_fix=w_fix, obs=obs, truth_hp=truth_hp)
 
 
-def run_c5(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
+def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
            every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=200, finetune_epochs=20, log=None, tune=True,
            **member_kw):
     """Surrogate-assisted DA against simulator-only DA on the same ensemble.
@@ -144,5 +144,5 @@ def run_c5(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.
                nrmse=[round(e, 5) for _, e in res["nrmse"]],
                bold_rmse_sim_last=float(rmse(w_sim)[-1]), bold_rmse_sur_last=float(rmse(res["forecasts"])[-1]))
     if log:
-        log(f"C5 surrogate DA: {out}")
+        log(f"surrogate DA: {out}")
     return out, dict(w_sim=w_sim, sur=res, obs=obs, hist=hist, truth_hp=truth_hp, da=da)

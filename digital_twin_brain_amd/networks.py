@@ -1,10 +1,14 @@
-"""Synthetic workloads of the benchmark configurations (seeded).
+"""Synthetic benchmark workloads (seeded).
 
-C1  plain block: N=10,000 (80% E), g_ui ~ U[0.001, 0.05] per neuron and
-    channel, in-degree 100, weights U[0,1], delay 1, noise 10 Hz, 8,000 steps.
-C2  large block: N=5e6, in-degree 1,000 (5e9 synapses, compact int32 index +
-    fp32 weight), delays U{1..10}, per-neuron noise 5-20 Hz, firing 2-20 Hz,
-    10,000 steps, BOLD per 1,000-neuron voxel once per ms.
+plain block     N=10,000 (80% E), g_ui ~ U[0.001, 0.05] per neuron and
+                channel, in-degree 100, weights U[0,1], delay 1, noise 10 Hz,
+                8,000 steps.
+large block     N=5e6, in-degree 1,000 (5e9 synapses, compact int32 index +
+                fp32 weight), delays U{1..10}, per-neuron noise 5-20 Hz,
+                firing 2-20 Hz, 10,000 steps, BOLD per 1,000-neuron voxel once
+                per ms.
+ensemble        E members of log-uniform size and uniform in-degree packed in
+                one block, plus a hidden "truth" member.
 Property columns follow generation/make_block.py:394-434.
 """
 from __future__ import annotations
@@ -30,7 +34,7 @@ def base_props(n, dt=0.1):
     return p
 
 
-def c1_props(n=10_000, seed=2, dt=0.1):
+def plain_props(n=10_000, seed=2, dt=0.1):
     rng = np.random.default_rng(seed)
     p = base_props(n, dt)
     n_exc = int(round(0.8 * n))
@@ -40,8 +44,8 @@ def c1_props(n=10_000, seed=2, dt=0.1):
     return p, n_exc
 
 
-def c2_props(n=5_000_000, seed=2, dt=0.1, voxel=1000):
-    """C2 parameters: inhibition-dominated coupling (0.1 x / 0.05 x the
+def large_props(n=5_000_000, seed=2, dt=0.1, voxel=1000):
+    """Large-block parameters: inhibition-dominated coupling (0.1 x / 0.05 x the
     calibrated E / I conductances, +-20% per neuron) with I_ext = 5 gives an
     asynchronous state at ~8 Hz under in-degree 1,000."""
     rng = np.random.default_rng(seed)
@@ -55,20 +59,20 @@ def c2_props(n=5_000_000, seed=2, dt=0.1, voxel=1000):
     return p, n_exc
 
 
-def build_c1(n=10_000, device=0, conn_seed=1, noise_seed=3):
-    p, ne = c1_props(n)
+def build_plain(n=10_000, device=0, conn_seed=1, noise_seed=3):
+    p, ne = plain_props(n)
     return NativeBlock.fixed_indegree(p, ne, 100, max_delay=1, w_lo=0.0, w_hi=1.0, pair_weights=False,
                                       conn_seed=conn_seed, noise_seed=noise_seed, device=device), p
 
 
-def build_c2(n=5_000_000, in_degree=1000, device=0, conn_seed=1, noise_seed=3):
-    p, ne = c2_props(n)
+def build_large(n=5_000_000, in_degree=1000, device=0, conn_seed=1, noise_seed=3):
+    p, ne = large_props(n)
     return NativeBlock.fixed_indegree(p, ne, in_degree, max_delay=10, w_lo=0.0, w_hi=1.0, pair_weights=False,
                                       conn_seed=conn_seed, noise_seed=noise_seed, device=device), p
 
 
-def c4_members(E=64, seed=4, n_lo=2000, n_hi=200_000, k_lo=100, k_hi=500, truth_n=20_000, truth_k=300):
-    """C4: E members with N log-uniform in [n_lo, n_hi] and in-degree uniform in
+def ensemble_members(E=64, seed=4, n_lo=2000, n_hi=200_000, k_lo=100, k_hi=500, truth_n=20_000, truth_k=300):
+    """Ensemble: E members with N log-uniform in [n_lo, n_hi] and in-degree uniform in
     [k_lo, k_hi], plus one hidden "truth" member that produces the observed BOLD."""
     rng = np.random.default_rng(seed)
     ns = np.round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), size=E))).astype(np.int64)
@@ -79,10 +83,10 @@ def c4_members(E=64, seed=4, n_lo=2000, n_hi=200_000, k_lo=100, k_hi=500, truth_
     return mem
 
 
-def c4_props(members, voxels=90, seed=5, dt=0.1, member_jitter=0.3):
+def ensemble_props(members, voxels=90, seed=5, dt=0.1, member_jitter=0.3):
     """Per member: `voxels` voxels of an E and an I sub-population
-    (sub_idx = (member*voxels + v)*2 + {0: E, 1: I}); conductances of the C2
-    regime rescaled by 1000/in-degree, perturbed +-member_jitter per member
+    (sub_idx = (member*voxels + v)*2 + {0: E, 1: I}); conductances of the
+    large-block regime rescaled by 1000/in-degree, perturbed +-member_jitter per member
     and channel, +-20% per neuron."""
     rng = np.random.default_rng(seed)
     n = sum(m[0] for m in members)
@@ -104,9 +108,9 @@ def c4_props(members, voxels=90, seed=5, dt=0.1, member_jitter=0.3):
     return p
 
 
-def build_c4(E=64, voxels=90, device=0, conn_seed=1, noise_seed=3, max_delay=1, **kw):
-    mem = c4_members(E=E, **kw)
-    p = c4_props(mem, voxels=voxels)
+def build_ensemble(E=64, voxels=90, device=0, conn_seed=1, noise_seed=3, max_delay=1, **kw):
+    mem = ensemble_members(E=E, **kw)
+    p = ensemble_props(mem, voxels=voxels)
     nb = NativeBlock.members(p, mem, max_delay=max_delay, w_lo=0.0, w_hi=1.0, conn_seed=conn_seed,
                              noise_seed=noise_seed, stats_exclude_blocked=True, device=device)
     return nb, p, mem

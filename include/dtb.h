@@ -89,7 +89,7 @@ int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_sp
                                     int32_t device);
 
 /* A network of members packed in one block (a data-assimilation ensemble of
- * uneven sizes, SURVEY C4): member m owns the next n neurons, the first
+ * uneven sizes): member m owns the next n neurons, the first
  * n_exc excitatory, each receiving in_degree synapses from inside the member
  * with the draws of dtb_block_create_fixed_indegree.  No reference
  * counterpart: the reference replicates equal members inside one network

@@ -157,7 +157,7 @@ def main(ref_root):
     props = make_props(n, ne, rng, 0.01, 0.08)
     run_block_case(ref_block, "dense64", n, ne, csr, props, steps=400, noise_rate=2.0 ** -5, isolated=False,
                    dt=0.1, seed=5, dense=True, snap_steps=(1, 2, 50, 100, 200, 400))
-    # 2) sparse W, 400 neurons, compact weights, 4 sub-populations, C1-like parameters
+    # 2) sparse W, 400 neurons, compact weights, 4 sub-populations, plain-block parameters
     n, ne = 400, 320
     csr = orc.gen_fixed_indegree(n, ne, 100, 1, 0.0, 1.0, False, seed=21)
     props = make_props(n, ne, rng, 0.001, 0.05, n_sub=4)

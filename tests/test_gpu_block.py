@@ -140,7 +140,7 @@ def test_gpu_generator_csr_bit_exact(pair, D):
     assert np.array_equal(nb.get("w1"), ref["w1"] if pair else ref["w0"])
 
 
-def _c1_like_props(n, n_exc, rng, g_scale=3.0):
+def _plain_like_props(n, n_exc, rng, g_scale=3.0):
     p = np.zeros((n, 22), np.float32)
     p[:, 3] = (np.arange(n) >= n_exc).astype(np.float32)
     p[:, 4] = 1; p[:, 5] = 2; p[:, 6] = 0.03; p[:, 7] = -75; p[:, 8] = -50; p[:, 9] = -65
@@ -161,7 +161,7 @@ def test_compaction_and_delivery_bit_exact_on_fixed_spike_vector():
     ref = orc.gen_fixed_indegree(n, ne, K, 1, 0.0, 1.0, True, seed=12)
     for k in ("w0", "w1"):
         ref[k] = (np.floor(ref[k] * 256) / 256).astype(np.float32)
-    props = _c1_like_props(n, ne, rng)
+    props = _plain_like_props(n, ne, rng)
     nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], ref["w1"], dt=0.1)
     ob = orc.OracleBlock(props, ref, 0.1)
     for s in range(1, 6):
@@ -181,13 +181,13 @@ def test_compaction_and_delivery_bit_exact_on_fixed_spike_vector():
 
 @pytest.mark.parametrize("pair,D,graph", [(False, 1, True), (True, 1, False), (False, 6, True), (True, 4, True)])
 def test_network_run_matches_oracle(pair, D, graph):
-    """C1-like recurrent network (fixed in-degree, delays 1..D) against the
+    """Plain-block-like recurrent network (fixed in-degree, delays 1..D) against the
     FP64 push oracle on the same CSR and the same Philox noise."""
     from digital_twin_brain_amd import NativeBlock
 
     n, ne, K, steps = 4000, 3200, 100, 600
     rng = np.random.default_rng(11)
-    props = _c1_like_props(n, ne, rng)
+    props = _plain_like_props(n, ne, rng)
     props[:, 3] = np.arange(n) // 500  # 8 sub-populations
     nb = NativeBlock.fixed_indegree(props, ne, K, max_delay=D, pair_weights=pair, conn_seed=3, noise_seed=9)
     csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
@@ -239,7 +239,7 @@ def test_graph_and_direct_launch_agree_bitwise():
 
     n, ne, K = 3000, 2400, 50
     rng = np.random.default_rng(2)
-    props = _c1_like_props(n, ne, rng)
+    props = _plain_like_props(n, ne, rng)
     ref = orc.gen_fixed_indegree(n, ne, K, 3, 0.0, 1.0, False, seed=4)
     ref["w0"] = (np.floor(ref["w0"] * 256) / 256).astype(np.float32)
     a = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"], dt=0.1,
@@ -262,7 +262,7 @@ def test_property_edits_match_reference_semantics():
 
     n = 1000
     rng = np.random.default_rng(3)
-    props = _c1_like_props(n, 800, rng)
+    props = _plain_like_props(n, 800, rng)
     props[:, 3] = np.arange(n) // 250
     csr = orc.gen_fixed_indegree(n, 800, 20, 1, 0.0, 1.0, False, seed=5)
     nb = NativeBlock.from_csr(props, csr["rowptr"], csr["post"], csr["kind"], csr["w0"], None, dt=0.1)
@@ -333,7 +333,7 @@ def test_invalid_inputs_raise_cleanly():
     from digital_twin_brain_amd import NativeBlock
     from digital_twin_brain_amd._native import DTBError
 
-    props = _c1_like_props(10, 8, np.random.default_rng(0))
+    props = _plain_like_props(10, 8, np.random.default_rng(0))
     with pytest.raises(DTBError):
         NativeBlock.from_csr(props, np.array([0] * 10 + [1], np.int64), np.array([10], np.int32),
                              np.array([0], np.uint8), np.array([1.0], np.float32))
@@ -356,7 +356,7 @@ def test_attached_bold_matches_oracle_on_same_counts():
 
     n, ne = 6000, 4800
     rng = np.random.default_rng(4)
-    props = _c1_like_props(n, ne, rng)
+    props = _plain_like_props(n, ne, rng)
     props[:, 3] = np.arange(n) // 1000
     props[:2000, 1] = 1.0  # blocked_in_stat: excluded when stats_exclude_blocked
     nb = NativeBlock.fixed_indegree(props, ne, 80, max_delay=3, conn_seed=2, noise_seed=5,
@@ -388,7 +388,7 @@ def test_results_independent_of_delivery_launch_shape():
 
     n, ne, K, D = 3000, 2400, 120, 8
     rng = np.random.default_rng(6)
-    props = _c1_like_props(n, ne, rng)
+    props = _plain_like_props(n, ne, rng)
     ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, False, seed=8)
     ref["w0"] = (np.floor(ref["w0"] * 256) / 256).astype(np.float32)
     outs = []
@@ -417,7 +417,7 @@ def test_bucketed_delivery_matches_atomics_including_overflow(pair):
 
     n, ne, K, D = 20000, 16000, 400, 3
     rng = np.random.default_rng(9)
-    props = _c1_like_props(n, ne, rng, g_scale=0.2)
+    props = _plain_like_props(n, ne, rng, g_scale=0.2)
     ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, pair, seed=10)
     for key in ("w0", "w1"):
         if ref.get(key) is not None:

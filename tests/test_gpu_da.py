@@ -130,8 +130,8 @@ def test_da_window_analysis_matches_oracle_enkf():
     from digital_twin_brain_amd.data_assimilation import DataAssimilation
 
     E, V = 4, 3
-    mem = networks.c4_members(E=E, seed=1, n_lo=400, n_hi=1500, k_lo=30, k_hi=60, truth_n=600, truth_k=40)
-    props = networks.c4_props(mem, voxels=V)
+    mem = networks.ensemble_members(E=E, seed=1, n_lo=400, n_hi=1500, k_lo=30, k_hi=60, truth_n=600, truth_k=40)
+    props = networks.ensemble_props(mem, voxels=V)
     nb = networks.NativeBlock.members(props, mem, max_delay=2, stats_exclude_blocked=True)
     da = DataAssimilation(network=nb, column=False, ensemble=E, dt=0.1, bold_steps=10, da_populations=E * 2 * V,
                           da_voxels=E * V, write_path="/tmp")
@@ -152,23 +152,23 @@ def test_da_window_analysis_matches_oracle_enkf():
                                rtol=1e-12)
 
 
-def test_c4_runner_small():
-    from digital_twin_brain_amd.da_runner import run_c4
+def test_ensemble_da_runner_small():
+    from digital_twin_brain_amd.da_runner import run_ensemble_da
 
-    out, da, rec = run_c4(E=6, voxels=4, windows=3, window_ms=50.0, n_lo=500, n_hi=3000, k_lo=40, k_hi=80,
+    out, da, rec = run_ensemble_da(E=6, voxels=4, windows=3, window_ms=50.0, n_lo=500, n_hi=3000, k_lo=40, k_hi=80,
                           truth_n=1000, truth_k=60)
     assert out["windows"] == 3 and np.isfinite(out["window_s_mean"])
     assert rec["w_save"].shape == (3, 6, 4, 2 * 2 + 6)
     assert np.all(np.isfinite(rec["obs"]))
 
 
-def test_c5_surrogate_da_small():
+def test_surrogate_da_small():
     """Surrogate-assisted DA: fallback every 5th window (and after any window
     over the NRMSE limit), finite forecasts, surrogate windows faster than
     simulated ones."""
-    from digital_twin_brain_amd.da_runner import run_c5
+    from digital_twin_brain_amd.da_runner import run_surrogate_da
 
-    out, rec = run_c5(E=8, voxels=4, train_windows=12, windows=7, window_ms=50.0, hidden=(128, 128), epochs=60,
+    out, rec = run_surrogate_da(E=8, voxels=4, train_windows=12, windows=7, window_ms=50.0, hidden=(128, 128), epochs=60,
                       finetune_epochs=5, n_lo=500, n_hi=3000, k_lo=40, k_hi=80, truth_n=1000, truth_k=60)
     sim = rec["sur"]["simulated"]
     assert sim[0] and sim[5]
