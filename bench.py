@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", choices=("large", "regions"), default="large",
+                    help="large: one 5e6-neuron block; regions: 90 regions, 2e7 neurons, connectome, delays 1-40")
     ap.add_argument("--n", type=int, default=5_000_000)
     ap.add_argument("--in-degree", type=int, default=1000)
     ap.add_argument("--kernel-window", type=int, default=200)
@@ -129,9 +131,23 @@ def main():
         return
     graph = not a.no_graph
     t0 = time.perf_counter()
-    nb, props = networks.build_large(n=a.n, in_degree=a.in_degree)
-    log(f"built large block n={nb.n} nnz={nb.nnz} S={nb.S} D={nb.max_delay} in {time.perf_counter() - t0:.1f} s")
-    nb.attach_bold(steps_per_sample=10, delta_t=1e-3)
+    if a.workload == "regions":
+        nb, props, spec, _ = networks.build_regions()
+        vop = (np.arange(nb.S) // 2).astype(np.int32)  # one BOLD voxel per region
+        nb.attach_bold(steps_per_sample=10, delta_t=1e-3, voxel_of_pop=vop,
+                       voxel_size=np.array([s[0] for s in spec], np.float64), nvox=len(spec))
+        config = {"workload": "90 regions, one GPU's share of a whole-brain model", "neurons": int(nb.n),
+                  "synapses": int(nb.nnz), "in_degree": "400 intra + 100 inter (Exp(1) connectome)",
+                  "region_sizes": "lognormal sigma 0.8", "delays": "1-40 steps", "dt_ms": 0.1,
+                  "external": "Poisson, 20% of input", "bold": "per region, 1 ms", "weights": "compact fp32"}
+    else:
+        nb, props = networks.build_large(n=a.n, in_degree=a.in_degree)
+        nb.attach_bold(steps_per_sample=10, delta_t=1e-3)
+        config = {"workload": "single large block", "neurons": a.n, "in_degree": a.in_degree,
+                  "synapses": a.n * a.in_degree, "delays": "1-10 steps", "dt_ms": 0.1,
+                  "noise_hz": "5-20 per neuron", "bold": "per 1000-neuron voxel, 1 ms", "weights": "compact fp32"}
+    log(f"built {a.workload} network n={nb.n} nnz={nb.nnz} S={nb.S} D={nb.max_delay} in "
+        f"{time.perf_counter() - t0:.1f} s")
     S = nb.S
     # warm-up (also captures the step-batch graph)
     W = a.warmup - a.warmup % 10
@@ -195,7 +211,7 @@ def main():
     torch.cuda.empty_cache()
 
     da = None
-    if a.da_windows > 0:
+    if a.da_windows > 0 and a.workload == "large":
         from digital_twin_brain_amd.da_runner import run_surrogate_da
 
         t0 = time.perf_counter()
@@ -209,20 +225,18 @@ def main():
         del rec
 
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and a.workload == "large":
         t0 = time.perf_counter()
         cpu = cpu_baseline(a.cpu_sample, a.in_degree, 200, a.cpu_steps)
         log(f"cpu baseline {cpu['value']:.3e} events/s ({time.perf_counter() - t0:.1f} s)")
 
     value = tot_ev / wall
-    traffic = pmc_traffic(prof, "c2")
+    traffic = pmc_traffic(prof, a.workload)
     out = {
         "metric": "synaptic events/s", "value": value, "unit": "events/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "single large block", "neurons": a.n, "in_degree": a.in_degree,
-                   "synapses": a.n * a.in_degree, "delays": "1-10 steps", "dt_ms": 0.1,
-                   "noise_hz": "5-20 per neuron", "bold": "per 1000-neuron voxel, 1 ms", "weights": "compact fp32"},
+        "config": config,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernels": kernels,

@@ -53,7 +53,7 @@ class FixedIndegreeSpec(ctypes.Structure):
 
 
 class MemberSpec(ctypes.Structure):
-    _fields_ = [("n", _i64), ("n_exc", _i64), ("in_degree", _i32), ("reserved", _i32)]
+    _fields_ = [("n", _i64), ("n_exc", _i64), ("in_degree", _i32), ("inter_degree", _i32)]
 
 
 class RunArgs(ctypes.Structure):
@@ -80,6 +80,8 @@ def lib():
         "dtb_block_create_fixed_indegree": [_vp, _vp, _vp, _f64, _u64, _i32, _i32],
         "dtb_block_destroy": [_vp],
         "dtb_block_create_members": [_vp, _vp, _i32, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32, _i32],
+        "dtb_block_create_regions": [_vp, _vp, _i32, _vp, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32,
+                                     _i32],
         "dtb_block_run": [_vp, _i64, _vp],
         "dtb_block_get": [_vp, _i32, _vp, _i64, _vp],
         "dtb_block_info": [_vp, _vp],
@@ -91,6 +93,7 @@ def lib():
         "dtb_block_set_samples": [_vp, _i64, _vp],
         "dtb_block_attach_bold": [_vp, _vp, _i32, _i32, _vp, _vp],
         "dtb_block_set_launch": [_vp, _i32, _i32],
+        "dtb_block_set_external": [_vp, _i32, _vp, _f32],
         "dtb_block_get_launch": [_vp, _i32, _vp],
         "dtb_block_bold_state": [_vp, _i32, _i32, _vp],
         "dtb_bold_step_f32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -118,11 +121,11 @@ def lib():
 def exported_symbols():
     """Names the header declares, for the load test."""
     return ["dtb_last_error", "dtb_abi_version", "dtb_block_create", "dtb_block_create_fixed_indegree",
-            "dtb_block_create_members",
+            "dtb_block_create_members", "dtb_block_create_regions",
             "dtb_block_destroy", "dtb_block_run", "dtb_block_get", "dtb_block_info", "dtb_block_time",
             "dtb_block_set_props", "dtb_block_mul_props_by_pop", "dtb_block_gamma_props_by_pop",
             "dtb_block_replace_props", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",
-            "dtb_block_set_launch", "dtb_block_get_launch",
+            "dtb_block_set_launch", "dtb_block_get_launch", "dtb_block_set_external",
             "dtb_bold_step_f32", "dtb_bold_step_f64",
             "dtb_enkf_analysis_f64",
             "dtb_mlp_gemm_bf16_nt", "dtb_mlp_relu_backward", "dtb_mlp_bias_grad", "dtb_mlp_mse_grad",

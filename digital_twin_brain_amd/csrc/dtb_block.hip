@@ -111,6 +111,7 @@ struct NeuronArgs {
     int32_t isolated;
     uint32_t* __restrict__ rec_freq;
     float* __restrict__ rec_vsum;
+    ExtDrive ext;
 };
 
 // Steps until the accumulated time reaches t_spike + T_ref: the same double
@@ -202,7 +203,14 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
             u = noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
         const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
         spike = (u < pr) || (act && !a.isolated);
-        a.Jp[i] = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
+        float4 jn = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
+        if (a.ext.n) {
+            // external arrivals of this step: part of J at the next step, like this step's deliveries
+            const float x = (float)ext_count(a.ext, a.seed, (uint64_t)step, (uint64_t)i) * a.ext.w;
+            jn.x += x;
+            jn.y += x;
+        }
+        a.Jp[i] = jn;
         a.flags[i] = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
         p = a.pop[i];
     }
@@ -1404,6 +1412,7 @@ static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* unif
     x.noise_mode = a->noise_mode; x.noise_rate = a->noise_rate; x.uniforms = uniforms; x.isolated = a->isolated;
     x.rec_freq = freq ? b->rec_freq : nullptr;
     x.rec_vsum = vsum ? b->rec_vsum : nullptr;
+    x.ext = b->ext;
     return x;
 }
 
@@ -1841,22 +1850,26 @@ int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_sp
     return DTB_OK;
 }
 
-int dtb_block_create_members(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
-                             int32_t max_delay, float w_lo, float w_hi, int32_t pair_weights, uint64_t seed,
-                             const float* props, double dt, uint64_t noise_seed, int32_t stats_exclude_blocked,
-                             int32_t device)
+static int create_members_impl(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
+                               const uint64_t* source_cdf, int32_t max_delay, float w_lo, float w_hi,
+                               int32_t pair_weights, uint64_t seed, const float* props, double dt,
+                               uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device, bool regions)
 {
+    DTB_REQUIRE(out != nullptr, "out handle pointer is NULL");
     DTB_REQUIRE(members != nullptr && props != nullptr && n_members >= 1, "members and props are required");
     DTB_REQUIRE(max_delay >= 1 && max_delay <= DTB_MAX_DELAY, "max_delay must be in 1..64");
     DTB_REQUIRE(dt > 0.0, "dt must be > 0");
+    DTB_REQUIRE(!regions || source_cdf != nullptr, "source_cdf is required");
     std::vector<dtb_member_spec> hm((size_t)n_members);
     DTB_TRY_HIP(hipMemcpy(hm.data(), members, sizeof(dtb_member_spec) * (size_t)n_members, hipMemcpyDefault));
     int64_t n = 0, nnz = 0;
     for (const auto& m : hm) {
-        DTB_REQUIRE(m.n >= 1 && m.n_exc >= 0 && m.n_exc <= m.n && m.in_degree >= 0, "bad member spec");
+        DTB_REQUIRE(m.n >= 1 && m.n_exc >= 0 && m.n_exc <= m.n && m.in_degree >= 0 && m.inter_degree >= 0,
+                    "bad member spec");
+        DTB_REQUIRE(regions || m.inter_degree == 0, "inter_degree must be 0 (use dtb_block_create_regions)");
         DTB_REQUIRE(m.in_degree == 0 || m.n >= 2, "a member with synapses needs at least 2 neurons");
         n += m.n;
-        nnz += m.n * (int64_t)m.in_degree;
+        nnz += m.n * (int64_t)(m.in_degree + m.inter_degree);
     }
     Block* b = nullptr;
     DTB_TRY(begin(&b, device, n));
@@ -1870,7 +1883,8 @@ int dtb_block_create_members(dtb_block** out, const dtb_member_spec* members, in
         if ((st = make_pack(b, max_delay))) break;
         b->max_row = 8192;
         if ((st = alloc_state(b))) break;
-        if ((st = gen_members(b, hm.data(), n_members, max_delay, w_lo, w_hi, seed))) break;
+        if ((st = gen_members(b, hm.data(), n_members, max_delay, w_lo, w_hi, seed, regions ? source_cdf : nullptr)))
+            break;
         st = finish_create(b, props);
     } while (0);
     if (st) {
@@ -1879,6 +1893,24 @@ int dtb_block_create_members(dtb_block** out, const dtb_member_spec* members, in
     }
     *out = static_cast<dtb_block*>(b);
     return DTB_OK;
+}
+
+int dtb_block_create_members(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
+                             int32_t max_delay, float w_lo, float w_hi, int32_t pair_weights, uint64_t seed,
+                             const float* props, double dt, uint64_t noise_seed, int32_t stats_exclude_blocked,
+                             int32_t device)
+{
+    return create_members_impl(out, members, n_members, nullptr, max_delay, w_lo, w_hi, pair_weights, seed, props,
+                               dt, noise_seed, stats_exclude_blocked, device, false);
+}
+
+int dtb_block_create_regions(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
+                             const uint64_t* source_cdf, int32_t max_delay, float w_lo, float w_hi,
+                             int32_t pair_weights, uint64_t seed, const float* props, double dt,
+                             uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device)
+{
+    return create_members_impl(out, members, n_members, source_cdf, max_delay, w_lo, w_hi, pair_weights, seed, props,
+                               dt, noise_seed, stats_exclude_blocked, device, true);
 }
 
 int dtb_block_destroy(dtb_block* b)
@@ -2093,6 +2125,25 @@ int dtb_block_bold_state(dtb_block* b, int32_t which, int32_t dir, double* data)
     else
         DTB_TRY_HIP(hipMemcpyAsync(src, data, sizeof(double) * (size_t)b->nvox, hipMemcpyDefault, b->stream));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    return DTB_OK;
+}
+
+int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* thresholds, float weight)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    DTB_REQUIRE(n_thresholds >= 0 && n_thresholds <= kMaxExtThr, "at most 16 thresholds");
+    DTB_REQUIRE(n_thresholds == 0 || thresholds != nullptr, "thresholds are required");
+    DTB_REQUIRE(std::isfinite(weight), "weight must be finite");
+    ExtDrive e{};
+    e.n = n_thresholds;
+    e.w = weight;
+    for (int32_t j = 0; j < n_thresholds; ++j) {
+        e.thr[j] = thresholds[j];
+        DTB_REQUIRE(j == 0 || e.thr[j] >= e.thr[j - 1], "thresholds must be non-decreasing");
+    }
+    b->ext = e;
+    for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
     return DTB_OK;
 }
 

@@ -99,6 +99,7 @@ struct Block {
     int32_t deliver_lanes = 64; // lanes per (row, delay) segment
     int32_t deliver_mode = 0;   // 0: global atomics, 1: bucketed (staged + LDS accumulation)
     int32_t stage_segs = 256;   // bucketed: (row, delay) segments per staging chunk
+    ExtDrive ext{};             // external Poisson drive (n = 0: off)
     int32_t bsize = 0;          // bucketed: neurons per destination bucket
     uint64_t bmagic = 0;        // bucket = (post * bmagic) >> bshift
     int32_t bshift = 0;
@@ -117,6 +118,6 @@ int block_alloc(Block* b, void** p, size_t bytes);
 int derive_props(Block* b, bool psets_dirty = true);
 int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec);
 int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_delay, float w_lo, float w_hi,
-                uint64_t seed);
+                uint64_t seed, const uint64_t* cdf);
 
 } // namespace dtb

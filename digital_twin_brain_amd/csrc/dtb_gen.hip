@@ -2,12 +2,16 @@
 // in the presynaptic-major packed CSR the step kernels read.
 //
 // The network is a list of members (one member for a single block; an
-// uneven data-assimilation ensemble packs many): member m owns neurons
-// [off_m, off_m + n_m), the first n_exc_m of them excitatory, and every one of
-// its neurons receives K_m synapses from inside the member.  For post i of
-// member m and slot k, one Philox4x32-10 call (ctr = (k, i_lo, i_hi, CONN),
-// key = seed) gives r0..r3:
-//   pre   = off_m + floor(r0 (n_m-1) / 2^32), shifted past i (no self-connection)
+// uneven data-assimilation ensemble or the regions of a whole-brain share
+// pack many): member m owns neurons [off_m, off_m + n_m), the first n_exc_m
+// of them excitatory, and every one of its neurons receives Ki_m synapses
+// from inside the member and K_m - Ki_m from other members chosen through a
+// source table.  For post i of member m and slot k, one Philox4x32-10 call
+// (ctr = (k, i_lo, i_hi, CONN), key = seed) gives r0..r3:
+//   pre   = off_m + floor(r0 (n_m-1) / 2^32), shifted past i (no self-connection),
+//           for k < Ki_m; for k >= Ki_m a second call (ctr = (k, i_lo, i_hi,
+//           REGION)) picks the source s = smallest s with q0 < cdf[m][s] and
+//           pre = off_s + floor(r0 n_s / 2^32)
 //   w0    = fmaf(w_hi - w_lo, u24(r1), w_lo)
 //   delay = 1 + floor(r2 D / 2^32)
 //   w1    = fmaf(w_hi - w_lo, u24(r3), w_lo)      (pair weights)
@@ -27,7 +31,7 @@ namespace dtb {
 
 struct GenMember {
     int64_t off, n, n_exc, syn_off; // syn_off: synapses of the members before this one
-    int32_t K, pad;
+    int32_t K, Ki;                   // in-degree, of which the first Ki slots are intra-member
 };
 
 struct GenSpec {
@@ -37,7 +41,10 @@ struct GenSpec {
     float w_lo, span;
     uint32_t k0, k1;
     const GenMember* __restrict__ mem;
+    const uint64_t* __restrict__ cdf; // [M][M] source tables (scaled to 2^32), or null
 };
+
+constexpr uint32_t kStreamRegion = 0x52454749u; // ctr = (k, post_lo, post_hi, REGION)
 
 __device__ inline u32x4 conn_draw(const GenSpec& s, int64_t i, int32_t k)
 {
@@ -85,6 +92,23 @@ __device__ inline uint32_t conn_pre(const GenMember& g, int64_t i, uint32_t r0)
     return (uint32_t)(g.off + pre);
 }
 
+__device__ inline uint32_t syn_pre(const GenSpec& s, int32_t m, int64_t i, int32_t k, uint32_t r0)
+{
+    const GenMember& g = s.mem[m];
+    if (k < g.Ki) return conn_pre(g, i, r0);
+    const u32x4 q =
+        philox4x32_10(u32x4{(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), kStreamRegion}, s.k0, s.k1);
+    const uint64_t* row = s.cdf + (int64_t)m * s.M;
+    int32_t lo = 0, hi = s.M - 1; // smallest source with q0 < row[source]; row[M-1] = 2^32
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)q.x < row[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    const GenMember& h = s.mem[lo];
+    return (uint32_t)(h.off + (int64_t)(((uint64_t)r0 * (uint64_t)h.n) >> 32));
+}
+
 __global__ void gen_count_kernel(GenSpec s, uint32_t* __restrict__ outdeg)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -93,7 +117,7 @@ __global__ void gen_count_kernel(GenSpec s, uint32_t* __restrict__ outdeg)
         int32_t k, m;
         syn_coords(s, x, i, k, m);
         const u32x4 r = conn_draw(s, i, k);
-        atomicAdd(&outdeg[conn_pre(s.mem[m], i, r.x)], 1u);
+        atomicAdd(&outdeg[syn_pre(s, m, i, k, r.x)], 1u);
     }
 }
 
@@ -183,7 +207,7 @@ __global__ void gen_fill_kernel(GenSpec s, unsigned long long* __restrict__ curs
         int32_t k, m;
         syn_coords(s, x, i, k, m);
         const u32x4 r = conn_draw(s, i, k);
-        const uint32_t pre = conn_pre(s.mem[m], i, r.x);
+        const uint32_t pre = syn_pre(s, m, i, k, r.x);
         const uint64_t d = ((uint64_t)r.z * (uint64_t)s.D) >> 32; // delay - 1
         const unsigned long long pos = atomicAdd(&cursor[pre], 1ull);
         keys[pos] = (d << 58) | ((uint64_t)i << 32) | (uint32_t)k;
@@ -347,7 +371,7 @@ static int gen_run(Block* b, const GenSpec& s, int32_t max_delay)
 }
 
 int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_delay, float w_lo, float w_hi,
-                uint64_t seed)
+                uint64_t seed, const uint64_t* cdf)
 {
     const int64_t n = b->n;
     const int64_t nnz = b->nnz;
@@ -355,24 +379,52 @@ int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_del
     DTB_REQUIRE(max_delay <= 63, "generated networks are limited to delays <= 63");
     std::vector<GenMember> hm((size_t)M);
     int64_t off = 0, so = 0;
+    bool inter = false;
     for (int32_t m = 0; m < M; ++m) {
-        hm[(size_t)m] = GenMember{off, mem[m].n, mem[m].n_exc, so, mem[m].in_degree, 0};
+        const int32_t K = mem[m].in_degree + mem[m].inter_degree;
+        hm[(size_t)m] = GenMember{off, mem[m].n, mem[m].n_exc, so, K, mem[m].in_degree};
+        inter = inter || mem[m].inter_degree > 0;
         off += mem[m].n;
-        so += mem[m].n * (int64_t)mem[m].in_degree;
+        so += mem[m].n * (int64_t)K;
     }
+    DTB_REQUIRE(!inter || cdf != nullptr, "inter-member synapses need a source table");
     GenMember* dmem = nullptr;
+    uint64_t* dcdf = nullptr;
     DTB_TRY_HIP(hipMalloc(&dmem, sizeof(GenMember) * (size_t)M));
     DTB_TRY_HIP(hipMemcpy(dmem, hm.data(), sizeof(GenMember) * (size_t)M, hipMemcpyHostToDevice));
-    GenSpec s{n, nnz, M, max_delay, w_lo, w_hi - w_lo, (uint32_t)seed, (uint32_t)(seed >> 32), dmem};
+    if (inter) {
+        std::vector<uint64_t> hc((size_t)M * (size_t)M);
+        hipError_t e = hipMemcpy(hc.data(), cdf, sizeof(uint64_t) * hc.size(), hipMemcpyDefault);
+        int bad = e != hipSuccess;
+        for (int32_t m = 0; m < M && !bad; ++m) {
+            const uint64_t* row = hc.data() + (size_t)m * M;
+            for (int32_t j = 1; j < M; ++j) bad |= row[j] < row[j - 1];
+            bad |= row[M - 1] != (1ull << 32);
+        }
+        if (!bad) e = hipMalloc(&dcdf, sizeof(uint64_t) * hc.size());
+        if (!bad && e == hipSuccess) e = hipMemcpy(dcdf, hc.data(), sizeof(uint64_t) * hc.size(), hipMemcpyHostToDevice);
+        if (bad || e != hipSuccess) {
+            hipFree(dmem);
+            if (dcdf) hipFree(dcdf);
+            if (bad) {
+                set_error("source tables must be non-decreasing rows ending at 2^32");
+                return DTB_INVALID_PARAMETER;
+            }
+            set_error("source table upload failed: %s", hipGetErrorString(e));
+            return DTB_RUNTIME_ERROR;
+        }
+    }
+    GenSpec s{n, nnz, M, max_delay, w_lo, w_hi - w_lo, (uint32_t)seed, (uint32_t)(seed >> 32), dmem, dcdf};
     int st = gen_run(b, s, max_delay);
     hipFree(dmem);
+    if (dcdf) hipFree(dcdf);
     return st;
 }
 
 int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
 {
     dtb_member_spec m{spec->n, spec->n_exc, spec->in_degree, 0};
-    return gen_members(b, &m, 1, spec->max_delay, spec->w_lo, spec->w_hi, spec->seed);
+    return gen_members(b, &m, 1, spec->max_delay, spec->w_lo, spec->w_hi, spec->seed, nullptr);
 }
 
 } // namespace dtb

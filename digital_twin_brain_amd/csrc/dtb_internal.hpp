@@ -71,6 +71,27 @@ __host__ __device__ inline float u24(uint32_t x) { return (float)(x >> 8) * (1.0
 constexpr uint32_t kStreamNoise = 0x4E4F4953u; // ctr = (i_lo, i_hi, step, NOISE)
 constexpr uint32_t kStreamConn = 0x434F4E4Eu;  // ctr = (k, post_lo, post_hi, CONN)
 constexpr uint32_t kStreamGamma = 0x47414D4Du; // ctr = (neuron_lo, neuron_hi, draw, GAMMA)
+constexpr uint32_t kStreamExt = 0x45585420u;   // ctr = (i_lo, i_hi, step, EXT)
+
+// External Poisson drive: per neuron and step, count = #{j < n : r >= thr[j]}
+// with r one 32-bit Philox word and thr[j] = 2^32 P(X <= j) (host-computed,
+// so the count is integer-exact everywhere); the count times w arrives on the
+// excitatory channels.
+constexpr int kMaxExtThr = 16;
+struct ExtDrive {
+    int32_t n;
+    float w;
+    uint32_t thr[kMaxExtThr];
+};
+
+__host__ __device__ inline uint32_t ext_count(const ExtDrive& e, uint64_t seed, uint64_t step, uint64_t i)
+{
+    const u32x4 c{(uint32_t)i, (uint32_t)(i >> 32), (uint32_t)step, kStreamExt};
+    const uint32_t r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32)).x;
+    uint32_t k = 0;
+    for (int j = 0; j < e.n; ++j) k += r >= e.thr[j] ? 1u : 0u;
+    return k;
+}
 
 __host__ __device__ inline float noise_uniform(uint64_t seed, uint64_t step, uint64_t i)
 {

@@ -7,6 +7,7 @@ provides, or into freshly allocated ones.
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 
@@ -19,6 +20,20 @@ def _as_np(x, dtype):
     if hasattr(x, "detach"):
         x = x.detach().cpu().numpy()
     return np.ascontiguousarray(x, dtype=dtype)
+
+
+def poisson_thresholds(lam, kmax=16):
+    """thr[j] = floor(2^32 P(X <= j)) for X ~ Poisson(lam), j < kmax (uint32):
+    a 32-bit uniform r gives the count #{j : r >= thr[j]}."""
+    lam = float(lam)
+    p = math.exp(-lam)
+    cdf = 0.0
+    thr = np.empty(kmax, np.uint32)
+    for j in range(kmax):
+        cdf += p
+        thr[j] = min(int(cdf * 4294967296.0), 0xFFFFFFFF)
+        p *= lam / (j + 1)
+    return thr
 
 
 class NativeBlock:
@@ -93,6 +108,26 @@ class NativeBlock:
                                                      int(bool(stats_exclude_blocked)), int(device)))
         return cls(h.value)
 
+    @classmethod
+    def regions(cls, props, regions, source_cdf, max_delay=1, w_lo=0.0, w_hi=1.0, pair_weights=False, conn_seed=1,
+                dt=0.1, noise_seed=3, stats_exclude_blocked=False, device=0):
+        """Regions with inter-region synapses; regions = [(n, n_exc, k_intra, k_inter), ...],
+        source_cdf = uint64 [R, R] cumulative source probabilities scaled to 2^32."""
+        nat.require_gpu()
+        props = np.ascontiguousarray(_as_np(props, np.float32)[:, :nat.NPROP])
+        cdf = np.ascontiguousarray(source_cdf, dtype=np.uint64)
+        if cdf.shape != (len(regions), len(regions)):
+            raise ValueError("source_cdf must be [regions, regions]")
+        arr = (nat.MemberSpec * len(regions))(*[nat.MemberSpec(int(n), int(ne), int(k), int(kx))
+                                                for n, ne, k, kx in regions])
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().dtb_block_create_regions(ctypes.byref(h), arr, len(regions), nat.ptr(cdf),
+                                                     int(max_delay), float(w_lo), float(w_hi),
+                                                     int(bool(pair_weights)), int(conn_seed), nat.ptr(props),
+                                                     float(dt), int(noise_seed), int(bool(stats_exclude_blocked)),
+                                                     int(device)))
+        return cls(h.value)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             nat.lib().dtb_block_destroy(self._h)
@@ -160,6 +195,14 @@ class NativeBlock:
         if v.shape[0] != self.nvox:
             raise ValueError(f"need {self.nvox} values")
         nat.check(nat.lib().dtb_block_bold_state(self._h, self.BOLD_FIELDS[which], 1, nat.ptr(v)))
+
+    # ---------------------------------------------------- external drive
+    def set_external(self, rate_per_step, weight, kmax=16):
+        """Poisson(rate_per_step) external spikes per neuron and step, each adding
+        `weight` to the excitatory channels (rate 0 turns the drive off)."""
+        thr = poisson_thresholds(rate_per_step, kmax) if rate_per_step > 0 else np.zeros(0, np.uint32)
+        nat.check(nat.lib().dtb_block_set_external(self._h, int(thr.shape[0]), nat.ptr(thr), float(weight)))
+        return thr
 
     # -------------------------------------------------------- launch shape
     LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2, neuron_psets=3, stage_segs=4)

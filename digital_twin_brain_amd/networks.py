@@ -114,3 +114,76 @@ def build_ensemble(E=64, voxels=90, device=0, conn_seed=1, noise_seed=3, max_del
     nb = NativeBlock.members(p, mem, max_delay=max_delay, w_lo=0.0, w_hi=1.0, conn_seed=conn_seed,
                              noise_seed=noise_seed, stats_exclude_blocked=True, device=device)
     return nb, p, mem
+
+
+# --------------------------------------------------------------------------
+# Regions: one GPU's share of a whole-brain model.  R regions with lognormal
+# sizes, each neuron receiving k_intra synapses from its own region and
+# k_inter from other regions picked through a seeded connectome (Exp(1)
+# weights, no self-loops), delays U{1..max_delay}, plus an external Poisson
+# drive standing for the regions on other GPUs.
+
+def region_sizes(R=90, total=20_000_000, sigma=0.8, seed=11, min_size=1000):
+    """Lognormal(0, sigma) sizes scaled to `total` neurons (exact sum)."""
+    rng = np.random.default_rng(seed)
+    x = rng.lognormal(0.0, sigma, size=R)
+    sizes = np.maximum(np.floor(x / x.sum() * total).astype(np.int64), min_size)
+    sizes[np.argmax(sizes)] += total - int(sizes.sum())
+    return sizes
+
+
+def connectome(R=90, seed=12):
+    """R x R weights, Exp(1) off the diagonal, 0 on it."""
+    rng = np.random.default_rng(seed)
+    W = rng.exponential(1.0, size=(R, R))
+    np.fill_diagonal(W, 0.0)
+    return W
+
+
+def source_cdf(W):
+    """Per target region, cumulative source probabilities scaled to 2^32
+    (uint64; rows non-decreasing and ending at exactly 2^32)."""
+    W = np.asarray(W, np.float64)
+    P = W / W.sum(axis=1, keepdims=True)
+    cdf = np.floor(np.cumsum(P, axis=1) * 4294967296.0).astype(np.uint64)
+    cdf = np.maximum.accumulate(np.minimum(cdf, np.uint64(1 << 32)), axis=1)
+    cdf[:, -1] = np.uint64(1 << 32)
+    return cdf
+
+
+def region_props(sizes, k_total=500, seed=13, dt=0.1):
+    """Per region an E (80%) and an I population (sub_idx = 2 region + {0, 1});
+    conductances of the large-block regime rescaled by 1000/k_total."""
+    rng = np.random.default_rng(seed)
+    n = int(np.sum(sizes))
+    p = base_props(n, dt)
+    p[:, 0] = rng.uniform(5.0, 20.0, size=n).astype(np.float32) * np.float32(dt * 1e-3)
+    p[:, 2] = 5.0
+    scale = np.array([0.1, 0.1, 0.05, 0.05], np.float32) * np.float32(1000.0 / k_total)
+    p[:, 10:14] = G_CAL * scale * rng.uniform(0.8, 1.2, size=(n, 4)).astype(np.float32)
+    off = 0
+    regions = []
+    for r, nr in enumerate(sizes):
+        ne = int(round(0.8 * nr))
+        p[off:off + ne, 3] = 2 * r
+        p[off + ne:off + nr, 3] = 2 * r + 1
+        regions.append((int(nr), ne))
+        off += nr
+    return p, regions
+
+
+def build_regions(R=90, total=20_000_000, k_intra=400, k_inter=100, max_delay=40, ext_fraction=0.2,
+                  target_hz=8.0, dt=0.1, device=0, conn_seed=1, noise_seed=3, sizes=None, W=None):
+    """Region network with external drive: the external Poisson stream is
+    `ext_fraction` of all input events at `target_hz` (each event the mean
+    synaptic weight, 0.5).  Returns (block, props, regions, cdf)."""
+    sizes = region_sizes(R, total) if sizes is None else np.asarray(sizes, np.int64)
+    W = connectome(len(sizes)) if W is None else W
+    cdf = source_cdf(W)
+    p, regs = region_props(sizes, k_intra + k_inter, dt=dt)
+    spec = [(n, ne, k_intra, k_inter) for n, ne in regs]
+    nb = NativeBlock.regions(p, spec, cdf, max_delay=max_delay, w_lo=0.0, w_hi=1.0, conn_seed=conn_seed, dt=dt,
+                             noise_seed=noise_seed, stats_exclude_blocked=True, device=device)
+    internal = (k_intra + k_inter) * target_hz * dt * 1e-3  # input events per neuron per step
+    nb.set_external(ext_fraction / (1.0 - ext_fraction) * internal, 0.5)
+    return nb, p, spec, cdf

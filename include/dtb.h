@@ -89,22 +89,37 @@ int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_sp
                                     int32_t device);
 
 /* A network of members packed in one block (a data-assimilation ensemble of
- * uneven sizes): member m owns the next n neurons, the first
- * n_exc excitatory, each receiving in_degree synapses from inside the member
- * with the draws of dtb_block_create_fixed_indegree.  No reference
- * counterpart: the reference replicates equal members inside one network
- * (data_assimilation/DataAssimilation.py:97). */
+ * uneven sizes, or the regions of one GPU's share of a whole-brain model):
+ * member m owns the next n neurons, the first n_exc excitatory, each
+ * receiving in_degree synapses from inside the member with the draws of
+ * dtb_block_create_fixed_indegree, and inter_degree more from other members
+ * (dtb_block_create_regions).  No reference counterpart for the ensemble:
+ * the reference replicates equal members inside one network
+ * (data_assimilation/DataAssimilation.py:97); regions follow the
+ * population/region layout of generation/make_block.py:206-434. */
 typedef struct {
     int64_t n;
     int64_t n_exc;
     int32_t in_degree;
-    int32_t reserved;
+    int32_t inter_degree; /* 0 for dtb_block_create_members */
 } dtb_member_spec;
 
 int dtb_block_create_members(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
                              int32_t max_delay, float w_lo, float w_hi, int32_t pair_weights, uint64_t seed,
                              const float* props, double dt, uint64_t noise_seed, int32_t stats_exclude_blocked,
                              int32_t device);
+
+/* Members with inter-member synapses: inter slot k of a neuron of member m
+ * draws its source member s as the smallest s with q < source_cdf[m*M + s]
+ * (q a 32-bit Philox draw, ctr = (k, post_lo, post_hi, 0x52454749)), then a
+ * presynaptic neuron uniformly inside s.  source_cdf: [M][M] cumulative
+ * source probabilities scaled to 2^32, rows non-decreasing and ending at
+ * exactly 2^32 (host or device memory).  The reference draws inter-region
+ * partners from a connectome in generation/make_block.py:206-330. */
+int dtb_block_create_regions(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
+                             const uint64_t* source_cdf, int32_t max_delay, float w_lo, float w_hi,
+                             int32_t pair_weights, uint64_t seed, const float* props, double dt,
+                             uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device);
 
 int dtb_block_destroy(dtb_block* b);
 
@@ -205,6 +220,16 @@ int dtb_block_replace_props(dtb_block* b, const float* props);
 /* Sampled neurons for sample_spike / sample_v (BlockWrapper.set_samples,
  * cuda/python/dist_blockwrapper_pytorch.py:505-528). */
 int dtb_block_set_samples(dtb_block* b, int64_t m, const uint32_t* idx);
+
+/* External Poisson drive (spikes from regions outside the block, no
+ * reference counterpart in block.py): at every step each neuron receives
+ * count * weight on its excitatory channels (AMPA and NMDA), as part of J from
+ * the next step on, like that step's synaptic arrivals.  count =
+ * #{j : r >= thresholds[j]}, r a 32-bit Philox draw keyed by the noise seed
+ * (ctr = (i_lo, i_hi, step, 0x45585420)); thresholds[j] = 2^32 P(X <= j) of
+ * the wanted count distribution (a Poisson CDF for a Poisson stream),
+ * non-decreasing, at most 16.  n_thresholds = 0 turns the drive off. */
+int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* thresholds, float weight);
 
 /* Launch shape of the delivery kernel (no counterpart in the reference,
  * whose launch shapes are fixed, cuda/include/common.hpp:70-71): the

@@ -108,40 +108,66 @@ void orc_noise_uniforms(uint64_t seed, uint64_t step, int64_t n, float* out)
  * the first nexc_m excitatory, each receiving K_m synapses from inside the
  * member; draws as above with pre = off_m + floor(r0 (n_m-1) / 2^32) shifted
  * past i, and kind = (pre - off_m >= nexc_m). */
-void orc_gen_members(int32_t M, const int64_t* ns, const int64_t* nexcs, const int32_t* Ks, int32_t D, float w_lo,
-                     float w_hi, int32_t pair, uint64_t seed, int64_t* rowptr, int32_t* post, uint8_t* kind,
-                     uint8_t* delay, float* w0, float* w1)
+/* Regions (or members) with a source-region table: region m owns the next
+ * ns[m] neurons, the first nexcs[m] excitatory; each of its neurons receives
+ * Ks[m] synapses from inside the region (draws as orc_gen_members) and then
+ * Kx[m] synapses from other regions.  Inter slot k: the source region is the
+ * smallest s with r < cdf[m*M + s], r from a second Philox call
+ * (ctr = (k, post_lo, post_hi, REGION)), and the presynaptic neuron is
+ * off_s + floor(r0 n_s / 2^32).  cdf holds cumulative probabilities scaled to
+ * 2^32 (the last entry of a row is 2^32). */
+#define DTB_STREAM_REGION 0x52454749u /* ctr = (k, post_lo, post_hi, REGION) */
+
+void orc_gen_regions(int32_t M, const int64_t* ns, const int64_t* nexcs, const int32_t* Ks, const int32_t* Kx,
+                     const uint64_t* cdf, int32_t D, float w_lo, float w_hi, int32_t pair, uint64_t seed,
+                     int64_t* rowptr, int32_t* post, uint8_t* kind, uint8_t* delay, float* w0, float* w1)
 {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     float span = w_hi - w_lo;
     int64_t n = 0;
-    for (int32_t m = 0; m < M; ++m) n += ns[m];
+    int64_t* offs = (int64_t*)calloc((size_t)M + 1, sizeof(int64_t));
+    for (int32_t m = 0; m < M; ++m) {
+        offs[m] = n;
+        n += ns[m];
+    }
+    offs[M] = n;
     int64_t* cursor = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
     memset(rowptr, 0, sizeof(int64_t) * (size_t)(n + 1));
     for (int pass = 0; pass < 2; ++pass) {
-        int64_t off = 0;
         for (int32_t m = 0; m < M; ++m) {
+            const int64_t off = offs[m];
+            const int32_t kx = Kx ? Kx[m] : 0;
             for (int64_t i = off; i < off + ns[m]; ++i) {
-                for (int32_t k = 0; k < Ks[m]; ++k) {
+                for (int32_t k = 0; k < Ks[m] + kx; ++k) {
                     uint32_t ctr[4] = {(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), DTB_STREAM_CONN};
                     uint32_t r[4];
                     orc_philox4x32_10(ctr, key, r);
-                    int64_t pl = (int64_t)(((uint64_t)r[0] * (uint64_t)(ns[m] - 1)) >> 32);
-                    if (pl >= i - off) pl += 1;
-                    int64_t pre = off + pl;
+                    int32_t src = m;
+                    int64_t pl;
+                    if (k < Ks[m]) {
+                        pl = (int64_t)(((uint64_t)r[0] * (uint64_t)(ns[m] - 1)) >> 32);
+                        if (pl >= i - off) pl += 1;
+                    } else {
+                        uint32_t c2[4] = {(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), DTB_STREAM_REGION};
+                        uint32_t q[4];
+                        orc_philox4x32_10(c2, key, q);
+                        src = 0;
+                        while (src < M - 1 && (uint64_t)q[0] >= cdf[(int64_t)m * M + src]) ++src;
+                        pl = (int64_t)(((uint64_t)r[0] * (uint64_t)ns[src]) >> 32);
+                    }
+                    int64_t pre = offs[src] + pl;
                     if (pass == 0) {
                         rowptr[pre + 1] += 1;
                         continue;
                     }
                     int64_t e = cursor[pre]++;
                     post[e] = (int32_t)i;
-                    kind[e] = (uint8_t)(pl >= nexcs[m]);
+                    kind[e] = (uint8_t)(pl >= nexcs[src]);
                     delay[e] = (uint8_t)(1 + (uint32_t)(((uint64_t)r[2] * (uint64_t)D) >> 32));
                     w0[e] = fmaf(span, u24(r[1]), w_lo);
                     if (w1) w1[e] = pair ? fmaf(span, u24(r[3]), w_lo) : w0[e];
                 }
             }
-            off += ns[m];
         }
         if (pass == 0) {
             for (int64_t j = 0; j < n; ++j) rowptr[j + 1] += rowptr[j];
@@ -149,6 +175,14 @@ void orc_gen_members(int32_t M, const int64_t* ns, const int64_t* nexcs, const i
         }
     }
     free(cursor);
+    free(offs);
+}
+
+void orc_gen_members(int32_t M, const int64_t* ns, const int64_t* nexcs, const int32_t* Ks, int32_t D, float w_lo,
+                     float w_hi, int32_t pair, uint64_t seed, int64_t* rowptr, int32_t* post, uint8_t* kind,
+                     uint8_t* delay, float* w0, float* w1)
+{
+    orc_gen_regions(M, ns, nexcs, Ks, NULL, NULL, D, w_lo, w_hi, pair, seed, rowptr, post, kind, delay, w0, w1);
 }
 
 void orc_gen_fixed_indegree(int64_t n, int64_t n_exc, int32_t K, int32_t D, float w_lo, float w_hi,
@@ -193,6 +227,10 @@ typedef struct {
     uint8_t* shist;                      /* [D][n] spike history (pull) */
     uint8_t *active, *spike;
     int32_t* sub;
+    /* external Poisson drive: count = #{j : r >= ext_thr[j]}, times ext_w on channels 0 and 1 */
+    int32_t ext_n;
+    float ext_w;
+    uint32_t ext_thr[16];
 } orc_block;
 
 #define P(b, i, c) ((REAL)(b)->props[(int64_t)(i) * 22 + (c)])
@@ -295,6 +333,27 @@ void RSUFFIX(orc_block_destroy)(void* h)
  * Outputs (may be NULL): sum_act[S] (threshold spikes per sub_idx),
  * mean_v[S] (mean V per sub_idx, NaN for an empty sub_idx).
  */
+#define DTB_STREAM_EXT 0x45585420u /* external drive: ctr = (i_lo, i_hi, step, EXT) */
+
+uint32_t RSUFFIX(orc_ext_count)(uint64_t seed, uint64_t step, uint64_t i, int32_t n, const uint32_t* thr)
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), (uint32_t)step, DTB_STREAM_EXT};
+    uint32_t r[4];
+    orc_philox4x32_10(ctr, key, r);
+    uint32_t c = 0;
+    for (int32_t j = 0; j < n; ++j) c += r[0] >= thr[j];
+    return c;
+}
+
+void RSUFFIX(orc_block_set_external)(void* h, int32_t n, const uint32_t* thr, float w)
+{
+    orc_block* b = (orc_block*)h;
+    b->ext_n = n;
+    b->ext_w = w;
+    for (int32_t j = 0; j < n && j < 16; ++j) b->ext_thr[j] = thr[j];
+}
+
 void RSUFFIX(orc_block_step)(void* h, int32_t noise_mode, float noise_rate, const float* uniforms,
                              uint64_t seed, int32_t isolated, int32_t pull, REAL* sum_act, REAL* mean_v)
 {
@@ -342,10 +401,13 @@ void RSUFFIX(orc_block_step)(void* h, int32_t noise_mode, float noise_rate, cons
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
             REAL* r = b->ring + ((int64_t)slot_now * 4) * n;
+            REAL ext = b->ext_n ? (REAL)RSUFFIX(orc_ext_count)(seed, (uint64_t)step, (uint64_t)i, b->ext_n,
+                                                              b->ext_thr) * (REAL)b->ext_w : 0;
             for (int k = 0; k < 4; ++k) {
                 REAL tau = P(b, i, 18 + k);
                 REAL J = b->J[(int64_t)k * n + i] * (REAL)exp(-(double)dt / (double)tau);
                 J += r[(int64_t)k * n + i];
+                if (k < 2) J += ext;
                 r[(int64_t)k * n + i] = 0;
                 b->J[(int64_t)k * n + i] = J;
             }
@@ -364,9 +426,12 @@ void RSUFFIX(orc_block_step)(void* h, int32_t noise_mode, float noise_rate, cons
                 acc[c0] += (REAL)b->pw0[e] * s;
                 acc[c0 + 1] += (REAL)b->pw1[e] * s;
             }
+            REAL ext = b->ext_n ? (REAL)RSUFFIX(orc_ext_count)(seed, (uint64_t)step, (uint64_t)i, b->ext_n,
+                                                              b->ext_thr) * (REAL)b->ext_w : 0;
             for (int k = 0; k < 4; ++k) {
                 REAL tau = P(b, i, 18 + k);
-                b->J[(int64_t)k * n + i] = b->J[(int64_t)k * n + i] * (REAL)exp(-(double)dt / (double)tau) + acc[k];
+                b->J[(int64_t)k * n + i] = b->J[(int64_t)k * n + i] * (REAL)exp(-(double)dt / (double)tau) + acc[k] +
+                                           (k < 2 ? ext : 0);
             }
         }
     }

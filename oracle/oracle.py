@@ -47,6 +47,8 @@ def _lib(real="f64"):
                                            _vp, _vp, _vp, _vp, _vp, _vp]
     lib.orc_gen_members.argtypes = [_c_i32, _vp, _vp, _vp, _c_i32, _c_f32, _c_f32, _c_i32, _c_u64,
                                     _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.orc_gen_regions.argtypes = [_c_i32, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_f32, _c_f32, _c_i32, _c_u64,
+                                    _vp, _vp, _vp, _vp, _vp, _vp]
     f = getattr(lib, "orc_block_create" + sfx)
     f.argtypes = [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_f64, _c_i32]
     f.restype = _vp
@@ -55,6 +57,9 @@ def _lib(real="f64"):
     getattr(lib, "orc_block_num_sub" + sfx).argtypes = [_vp]
     getattr(lib, "orc_block_num_sub" + sfx).restype = _c_i32
     getattr(lib, "orc_block_get" + sfx).argtypes = [_vp, _c_i32, _vp]
+    getattr(lib, "orc_block_set_external" + sfx).argtypes = [_vp, _c_i32, _vp, _c_f32]
+    getattr(lib, "orc_ext_count" + sfx).argtypes = [_c_u64, _c_u64, _c_u64, _c_i32, _vp]
+    getattr(lib, "orc_ext_count" + sfx).restype = ctypes.c_uint32
     lib.orc_compact.argtypes = [_vp, _c_i64, _vp]
     lib.orc_compact.restype = _c_i64
     lib.orc_bold_step.argtypes = [_c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
@@ -121,6 +126,34 @@ def gen_members(members, max_delay, w_lo, w_hi, pair, seed):
     return dict(rowptr=rowptr, post=post, kind=kind, delay=delay, w0=w0, w1=w1 if pair else None)
 
 
+def gen_regions(regions, cdf, max_delay, w_lo, w_hi, pair, seed):
+    """Regions with a source-region table; regions = [(n, n_exc, k_intra, k_inter), ...],
+    cdf = uint64 [R, R] cumulative source probabilities scaled to 2^32."""
+    ns = np.array([r[0] for r in regions], np.int64)
+    nexc = np.array([r[1] for r in regions], np.int64)
+    ks = np.array([r[2] for r in regions], np.int32)
+    kx = np.array([r[3] for r in regions], np.int32)
+    cdf = np.ascontiguousarray(cdf, dtype=np.uint64)
+    assert cdf.shape == (len(regions), len(regions))
+    n = int(ns.sum())
+    nnz = int((ns * (ks + kx)).sum())
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    post = np.zeros(nnz, dtype=np.int32)
+    kind = np.zeros(nnz, dtype=np.uint8)
+    delay = np.zeros(nnz, dtype=np.uint8)
+    w0 = np.zeros(nnz, dtype=np.float32)
+    w1 = np.zeros(nnz, dtype=np.float32)
+    _lib().orc_gen_regions(len(regions), _p(ns), _p(nexc), _p(ks), _p(kx), _p(cdf), max_delay, w_lo, w_hi,
+                           int(pair), seed, _p(rowptr), _p(post), _p(kind), _p(delay), _p(w0), _p(w1))
+    return dict(rowptr=rowptr, post=post, kind=kind, delay=delay, w0=w0, w1=w1 if pair else None)
+
+
+def ext_count(seed, step, i, thresholds):
+    """External-drive count of neuron i at `step`: #{j : r >= thresholds[j]}."""
+    thr = np.ascontiguousarray(thresholds, dtype=np.uint32)
+    return int(_lib().orc_ext_count_f64(seed, step, i, thr.shape[0], _p(thr)))
+
+
 class OracleBlock:
     """CPU restatement of models/block.py:5-177 (block.__init__ / run)."""
 
@@ -163,6 +196,12 @@ class OracleBlock:
         getattr(self.lib, "orc_block_step" + self.sfx)(self.h, int(noise_mode), float(noise_rate), _p(u), seed,
                                                        int(bool(isolated)), int(self.pull), _p(s), _p(m))
         return s, m
+
+    def set_external(self, thresholds, weight):
+        """External Poisson drive, as dtb_block_set_external."""
+        thr = np.ascontiguousarray(thresholds, dtype=np.uint32)
+        self._keep["ext"] = thr
+        getattr(self.lib, "orc_block_set_external" + self.sfx)(self.h, int(thr.shape[0]), _p(thr), float(weight))
 
     def get(self, name):
         f = self.FIELDS[name]

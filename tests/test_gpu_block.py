@@ -452,3 +452,69 @@ def test_gpu_members_generator_csr_bit_exact():
         for k in ("rowptr", "post", "kind", "delay", "w0"):
             assert np.array_equal(nb.get(k), ref[k]), k
         assert np.array_equal(nb.get("w1"), ref["w1"] if pair else ref["w0"])
+
+
+def _region_case():
+    from digital_twin_brain_amd import networks
+
+    sizes = np.array([700, 1500, 400, 1200, 900])
+    W = networks.connectome(5, seed=4)
+    cdf = networks.source_cdf(W)
+    props, regs = networks.region_props(sizes, k_total=60)
+    spec = [(n, ne, 40, 20) for n, ne in regs]
+    return props, spec, cdf
+
+
+def test_gpu_regions_generator_csr_bit_exact():
+    """Regions with inter-region synapses drawn through the source table."""
+    from digital_twin_brain_amd import NativeBlock
+
+    props, spec, cdf = _region_case()
+    for pair, D in ((False, 40), (True, 7)):
+        nb = NativeBlock.regions(props, spec, cdf, max_delay=D, pair_weights=pair, conn_seed=17)
+        ref = rows_by_delay(orc.gen_regions(spec, cdf, D, 0.0, 1.0, pair, seed=17))
+        for k in ("rowptr", "post", "kind", "delay", "w0"):
+            assert np.array_equal(nb.get(k), ref[k]), k
+        assert np.array_equal(nb.get("w1"), ref["w1"] if pair else ref["w0"])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_region_network_with_external_drive_matches_oracle(mode):
+    """Region network, delays 1..40, external Poisson drive: the GPU against
+    the FP64 push oracle with the same thresholds and the same Philox draws."""
+    from digital_twin_brain_amd import NativeBlock
+    from digital_twin_brain_amd.engine import poisson_thresholds
+
+    props, spec, cdf = _region_case()
+    nb = NativeBlock.regions(props, spec, cdf, max_delay=40, conn_seed=5, noise_seed=8)
+    nb.set_launch(deliver_mode=mode)
+    thr = nb.set_external(0.15, 0.5)
+    assert np.array_equal(thr, poisson_thresholds(0.15))
+    csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
+               w0=nb.get("w0"), w1=None)
+    ob = orc.OracleBlock(props, csr, 0.1)
+    ob.set_external(thr, 0.5)
+    steps, B = 400, 50
+    freqs = np.zeros((B, nb.S), np.uint32)
+    events = np.zeros(B, np.uint64)
+    diverged = None
+    ospk = []
+    for b0 in range(0, steps, B):
+        nb.run(B, noise_mode=1, graph=True, freqs=freqs, events=events)
+        for k in range(B):
+            sa, _ = ob.run(noise_mode=1, seed=8)
+            ospk.append(ob.get("spike"))
+            if diverged is None and not np.array_equal(freqs[k].astype(np.float64), sa):
+                diverged = b0 + k + 1
+        if diverged is not None:
+            break
+        np.testing.assert_allclose(nb.get("V"), ob.get("V"), rtol=RTOL)
+        Jo = ob.get("J")
+        np.testing.assert_allclose(nb.get("J"), Jo, rtol=RTOL, atol=RTOL * np.abs(Jo).max())
+        arr = arrivals_per_step(csr, ospk, 40)[b0:b0 + B]
+        assert np.array_equal(events.astype(np.int64), arr)
+    print(f"regions mode={mode}: first divergence {diverged}, spikes {int(np.sum(ospk))}")
+    assert diverged is None or diverged > 200
+    assert np.sum(ospk) > 0
+    # the external drive is what keeps J_AMPA of an otherwise silent network up
+    assert ob.get("J")[0].mean() > 0

@@ -341,3 +341,87 @@ def test_enkf_invariants():
                                rtol=1e-12, atol=1e-12)
     # the global term couples voxels: it differs from the local one for V > 1
     assert np.abs(w0 - w1).max() > 1e-6
+
+
+def test_gen_regions_structure_and_source_frequencies():
+    """Every neuron gets k_intra synapses from its own region (no self-loops)
+    and k_inter from other regions; the sources of inter synapses follow the
+    connectome rows (chi-square); kinds follow the presynaptic E/I split."""
+    from scipy import stats
+
+    from digital_twin_brain_amd import networks
+
+    sizes = np.array([300, 500, 200, 400, 600])
+    R = len(sizes)
+    W = networks.connectome(R, seed=3)
+    cdf = networks.source_cdf(W)
+    spec = [(int(n), int(0.8 * n), 20, 10) for n in sizes]
+    D = 12
+    g = orc.gen_regions(spec, cdf, D, 0.0, 1.0, False, seed=5)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(off[-1])
+    reg = np.repeat(np.arange(R), sizes)
+    pre = np.repeat(np.arange(n), np.diff(g["rowptr"]))
+    post = g["post"].astype(np.int64)
+    assert not np.any(pre == post)
+    same = reg[pre] == reg[post]
+    assert np.array_equal(np.bincount(post[same], minlength=n), np.full(n, 20))
+    assert np.array_equal(np.bincount(post[~same], minlength=n), np.full(n, 10))
+    assert g["delay"].min() >= 1 and g["delay"].max() <= D
+    local = pre - off[reg[pre]]
+    ne = np.array([s[1] for s in spec])
+    assert np.array_equal(g["kind"], (local >= ne[reg[pre]]).astype(np.uint8))
+    P = W / W.sum(1, keepdims=True)
+    for r in range(R):
+        m = (~same) & (reg[post] == r)
+        obs = np.bincount(reg[pre[m]], minlength=R)
+        exp = P[r] * m.sum()
+        keep = exp > 0
+        assert obs[~keep].sum() == 0  # never the own region
+        chi2 = ((obs[keep] - exp[keep]) ** 2 / exp[keep]).sum()
+        assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4, (r, obs, exp)
+
+
+def test_poisson_thresholds_match_scipy_and_counts_are_poisson():
+    from scipy import stats
+
+    from digital_twin_brain_amd.engine import poisson_thresholds
+
+    for lam in (0.05, 0.3, 1.7):
+        thr = poisson_thresholds(lam)
+        want = np.minimum(np.floor(stats.poisson.cdf(np.arange(16), lam) * 2.0 ** 32), 2 ** 32 - 1)
+        assert np.all(np.abs(thr.astype(np.float64) - want) <= 1), lam
+    thr = poisson_thresholds(0.3)
+    c = np.array([orc.ext_count(3, 7, i, thr) for i in range(100_000)])
+    obs = np.bincount(c, minlength=6)[:6]
+    exp = stats.poisson.pmf(np.arange(6), 0.3) * c.size
+    chi2 = ((obs[:4] - exp[:4]) ** 2 / exp[:4]).sum()
+    assert stats.chi2.sf(chi2, 3) > 1e-4
+    assert abs(c.mean() - 0.3) < 0.01
+
+
+def test_external_drive_stationary_mean_of_J():
+    """Silent network (no synapses, no noise, V held below threshold) under the
+    external drive: J follows J_s = a J_{s-1} + c_s w with a = e^{-dt/tau} on
+    AMPA and NMDA, so from J_0 = 0 its mean over neurons after N steps is
+    lam w (1 - a^N) / (1 - a); GABA channels stay 0."""
+    from digital_twin_brain_amd.engine import poisson_thresholds
+
+    n = 20000
+    props = np.zeros((n, 22), np.float32)
+    props[:, 4] = 1; props[:, 5] = 2; props[:, 6] = 0.03; props[:, 7] = -75; props[:, 8] = 1e6; props[:, 9] = -65
+    props[:, 18:22] = (2, 40, 10, 50)
+    csr = dict(rowptr=np.zeros(n + 1, np.int64), post=np.zeros(0, np.int32), kind=np.zeros(0, np.uint8),
+               delay=None, w0=np.zeros(0, np.float32), w1=None)
+    ob = orc.OracleBlock(props, csr, 0.1)
+    lam, w = 0.2, 0.5
+    ob.set_external(poisson_thresholds(lam), w)
+    N = 600
+    for _ in range(N):
+        ob.run(noise_rate=0.0, seed=1)
+    J = ob.get("J")
+    for ch, tau in ((0, 2.0), (1, 40.0)):
+        a = np.exp(-0.1 / tau)
+        want = lam * w * (1 - a ** N) / (1 - a)
+        assert abs(J[ch].mean() / want - 1) < 0.03, (ch, J[ch].mean(), want)
+    assert np.all(J[2:] == 0)
