@@ -375,13 +375,10 @@ static void launch_deliver(const Block* b, const DeliverArgs& da, int32_t k)
 // global atomics into acc2, which phase 2 then folds in.
 template <bool PAIR>
 struct BucketCfg {
-    static constexpr int kChan = PAIR ? 4 : 2;       // floats per neuron in acc
-    static constexpr int kEPT = PAIR ? 16 : 32;      // events per lane per round
+    static constexpr int kChan = PAIR ? 4 : 2; // floats per neuron in acc
 };
 constexpr int kMaxBucketBits = 15;   // neurons per bucket < 2^15 (local index in staged words)
-constexpr int kCursorStride = 32;    // one 128-byte line per bucket cursor / overflow flag: every
-                                     // workgroup round hits every cursor, and atomics on one line
-                                     // serialise
+constexpr int kCursorStride = 32;    // one 128-byte line per bucket cursor / overflow flag
 constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
 constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
 
@@ -419,17 +416,20 @@ __device__ inline uint32_t block_inclusive_scan_256(uint32_t v, uint32_t* wtot)
 
 // SEG (64, 128 or 256) segments per chunk: fewer give more, smaller chunks
 // (parallelism when few neurons spike), more amortise the chunk prologue.
-template <bool PAIR, int SEG>
+// EPT events per lane per round (16 or 32): fewer use fewer registers (more
+// resident waves), more take fewer staging reservations.  The bucket
+// histogram is dynamic LDS of nbk words, so small bucket counts leave room for
+// more resident workgroups.
+template <bool PAIR, int SEG, int EPT>
 __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t k)
 {
-    using Cfg = BucketCfg<PAIR>;
-    constexpr int EPT = Cfg::kEPT;
     static_assert(SEG >= 64 && SEG <= 256, "segment index is packed in 8 bits");
+    static_assert(256 * EPT <= (1 << 13), "rank inside a round must fit kRankBits");
     __shared__ uint32_t cum[DTB_MAX_DELAY + 1];
     __shared__ int32_t slot_of[DTB_MAX_DELAY + 1];
     __shared__ uint32_t segpre[SEG + 1];
     __shared__ int64_t segbeg[SEG];
-    __shared__ uint32_t hist[kMaxBucketsLds];
+    extern __shared__ uint32_t hist[]; // [nbk]
     __shared__ uint32_t wtot[4];
     const DeliverArgs& a = s.d;
     const int64_t step = a.ctl->nbase + k + 1;
@@ -565,15 +565,23 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     if (t == 0 && ev_wg && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev_wg);
 }
 
+template <bool PAIR, int EPT>
+static void launch_stage_ept(const Block* b, const StageArgs& sa, int32_t k)
+{
+    const dim3 g(b->deliver_grid), t(256);
+    const size_t lds = sizeof(uint32_t) * (size_t)b->nbk;
+    switch (b->stage_segs) {
+    case 64: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 64, EPT>), g, t, lds, b->stream, sa, k); break;
+    case 128: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 128, EPT>), g, t, lds, b->stream, sa, k); break;
+    default: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 256, EPT>), g, t, lds, b->stream, sa, k); break;
+    }
+}
+
 template <bool PAIR>
 static void launch_stage(const Block* b, const StageArgs& sa, int32_t k)
 {
-    const dim3 g(b->deliver_grid), t(256);
-    switch (b->stage_segs) {
-    case 64: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 64>), g, t, 0, b->stream, sa, k); break;
-    case 128: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 128>), g, t, 0, b->stream, sa, k); break;
-    default: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 256>), g, t, 0, b->stream, sa, k); break;
-    }
+    if (b->stage_ept == 16) launch_stage_ept<PAIR, 16>(b, sa, k);
+    else launch_stage_ept<PAIR, 32>(b, sa, k);
 }
 
 template <bool PAIR>
@@ -1147,6 +1155,7 @@ static int alloc_ring_and_records(Block* b)
     b->deliver_lanes = avg_seg >= 48 ? 64 : avg_seg >= 24 ? 32 : 16;
     const double groups_per_wg = 4.0 * (64 / b->deliver_lanes);
     b->deliver_grid = (int32_t)std::min<double>(16384.0, std::max(4.0, std::ceil(segs / groups_per_wg)));
+    b->stage_ept = b->pair ? 16 : 32;
     if (b->n >= (1 << 17) && b->n <= (1ll << 26)) {
         b->deliver_mode = 1;
         b->deliver_grid = 1024;
@@ -2163,6 +2172,9 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
     } else if (what == DTB_LAUNCH_STAGE_SEGS) {
         DTB_REQUIRE(value == 64 || value == 128 || value == 256, "segments per chunk must be 64, 128 or 256");
         b->stage_segs = value;
+    } else if (what == DTB_LAUNCH_STAGE_EPT) {
+        DTB_REQUIRE(value == 16 || value == 32, "events per lane per round must be 16 or 32");
+        b->stage_ept = value;
     } else if (what == DTB_LAUNCH_DELIVER_MODE) {
         DTB_REQUIRE(value == 0 || value == 1, "delivery mode must be 0 (atomics) or 1 (bucketed)");
         if (value == 1) DTB_TRY(ensure_bucketed(b));
@@ -2184,6 +2196,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     else if (what == DTB_LAUNCH_DELIVER_MODE) *value = b->deliver_mode;
     else if (what == DTB_LAUNCH_NEURON_PSETS) *value = b->npsets;
     else if (what == DTB_LAUNCH_STAGE_SEGS) *value = b->stage_segs;
+    else if (what == DTB_LAUNCH_STAGE_EPT) *value = b->stage_ept;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

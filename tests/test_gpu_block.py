@@ -392,13 +392,16 @@ def test_results_independent_of_delivery_launch_shape():
     ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, False, seed=8)
     ref["w0"] = (np.floor(ref["w0"] * 256) / 256).astype(np.float32)
     outs = []
-    for mode, lanes, grid, sg in ((0, 64, 4, 256), (0, 32, 37, 256), (0, 16, 300, 256), (0, 64, 2048, 256),
-                                  (1, 64, 1, 256), (1, 64, 300, 256), (1, 64, 7, 64), (1, 64, 1000, 128)):
+    for mode, lanes, grid, sg, ept in ((0, 64, 4, 256, 32), (0, 32, 37, 256, 32), (0, 16, 300, 256, 32),
+                                       (0, 64, 2048, 256, 32), (1, 64, 1, 256, 32), (1, 64, 300, 256, 16),
+                                       (1, 64, 7, 64, 16), (1, 64, 1000, 128, 32)):
         nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"],
                                   dt=0.1, noise_seed=2)
-        nb.set_launch(deliver_lanes=lanes, deliver_grid=grid, deliver_mode=mode, neuron_psets=mode, stage_segs=sg)
+        nb.set_launch(deliver_lanes=lanes, deliver_grid=grid, deliver_mode=mode, neuron_psets=mode, stage_segs=sg,
+                      stage_ept=ept)
         assert nb.get_launch() == {"deliver_grid": grid, "deliver_lanes": lanes, "deliver_mode": mode,
-                                   "neuron_psets": mode, "stage_segs": sg}  # one shared parameter set when enabled
+                                   "neuron_psets": mode, "stage_segs": sg,
+                                   "stage_ept": ept}  # one shared parameter set when enabled
         ev = np.zeros(300, np.uint64)
         nb.run(300, noise_rate=0.02, graph=True, events=ev)
         outs.append((nb.get("V"), nb.get("J"), ev))
