@@ -72,6 +72,22 @@ __device__ inline float wave_sum(float v)
 }
 
 // ---------------------------------------------------------------- kernels
+// Spike lists.  The list of a step is kSubLists sub-lists: workgroup b of the
+// neuron kernel appends to sub-list b % kSubLists (capacity spk_cap = the
+// neurons of those workgroups, an exact bound), so each sub-list counter takes
+// ~1/kSubLists of the workgroups' reservation atomics: returning atomics on
+// one address serialise at ~11 ns each, which made one shared counter the
+// neuron kernel's bottleneck.  Counters sit kCntStride words apart.
+constexpr int kSubLists = 32;
+constexpr int kCntStride = 32;
+constexpr int kMaxEntries = DTB_MAX_DELAY * kSubLists; // (delay, sub-list) entries of a delivery
+
+__host__ __device__ inline int64_t spk_cap_of(int64_t n)
+{
+    const int64_t nwg = (n + 255) / 256;
+    return (nwg + kSubLists - 1) / kSubLists * 256;
+}
+
 struct NeuronArgs {
     int64_t n;
     int32_t D, S;
@@ -97,8 +113,9 @@ struct NeuronArgs {
     double* __restrict__ t_last;
     uint8_t* __restrict__ flags;
     void* __restrict__ acc;
-    int32_t* __restrict__ spk;
-    uint32_t* __restrict__ spk_cnt;
+    int32_t* __restrict__ spk;     // [L][kSubLists][spk_cap]
+    uint32_t* __restrict__ spk_cnt; // [L][kSubLists] (kCntStride apart)
+    int64_t spk_cap;
     int32_t L;         // spike-list slots (D + 1)
     int32_t clear_acc; // direct delivery accumulates into acc, so it is cleared after reading
     const StepCtl* __restrict__ ctl;
@@ -148,7 +165,8 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     const double t = a.ttab[k];
     const int32_t slot = (int32_t)(step % a.L);
     // the list of step+1 reuses the slot of step+1-L, whose last delivery was step
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.spk_cnt[(step + 1) % a.L] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kSubLists)
+        a.spk_cnt[((int64_t)((step + 1) % a.L) * kSubLists + threadIdx.x) * kCntStride] = 0;
 
     bool act = false, spike = false;
     float vout = 0.0f;
@@ -222,15 +240,16 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     const uint32_t pre = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) s_wcnt[wid] = (uint32_t)__popcll(m);
     __syncthreads();
+    const int64_t sl = (int64_t)slot * kSubLists + blockIdx.x % kSubLists;
     if (threadIdx.x == 0) {
         const uint32_t tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
-        s_base = tot ? atomicAdd(&a.spk_cnt[slot], tot) : 0u;
+        s_base = tot ? atomicAdd(&a.spk_cnt[sl * kCntStride], tot) : 0u;
     }
     __syncthreads();
     if (spike) {
         uint32_t off = s_base;
         for (uint32_t w = 0; w < wid; ++w) off += s_wcnt[w];
-        a.spk[(int64_t)slot * a.n + off + pre] = (int32_t)i;
+        a.spk[sl * a.spk_cap + off + pre] = (int32_t)i;
     }
 
     // per sub-population statistics (threshold spikes, V sum)
@@ -266,43 +285,105 @@ struct DeliverArgs {
     void* __restrict__ acc;
     const int32_t* __restrict__ spk;
     const uint32_t* __restrict__ spk_cnt;
+    int64_t spk_cap;
     const StepCtl* __restrict__ ctl;
     uint64_t* __restrict__ rec_events;
     uint32_t* __restrict__ rec_nspk;
 };
+
+// Inclusive scan over the 256 lanes of a workgroup (shuffles within each
+// wavefront, then the four wavefront totals through LDS).
+__device__ inline uint32_t block_inclusive_scan_256(uint32_t v, uint32_t* wtot)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += y;
+    }
+    if (lane == kWave - 1) wtot[wid] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wid; ++w) add += wtot[w];
+    __syncthreads();
+    return v + add;
+}
+
+// where sub-list entry e = (d - 1) * kSubLists + q of step `step` starts in spk
+__device__ inline int64_t entry_base(const DeliverArgs& a, int64_t step, int32_t e)
+{
+    const int32_t d = e / kSubLists + 1, q = e % kSubLists;
+    return ((int64_t)((step - d + 1 + a.L) % a.L) * kSubLists + q) * a.spk_cap;
+}
+
+// Segments to deliver at batch step k: entry e = (d - 1) * kSubLists + q holds
+// sub-list q of the step that spiked d - 1 steps ago; cum[e + 1] = segments in
+// entries <= e.  Workgroups of 256 threads; returns the number of entries.
+__device__ inline int32_t delivery_entries(const DeliverArgs& a, int32_t k, uint32_t* cum, uint32_t* wtot)
+{
+    const int64_t step = a.ctl->nbase + k + 1;
+    const int32_t E = a.D * kSubLists;
+    constexpr int kPer = kMaxEntries / 256;
+    uint32_t c[kPer];
+    uint32_t sum = 0;
+    const int32_t e0 = threadIdx.x * kPer;
+#pragma unroll
+    for (int x = 0; x < kPer; ++x) {
+        const int32_t e = e0 + x;
+        c[x] = 0;
+        if (e < E) {
+            const int32_t d = e / kSubLists + 1, q = e % kSubLists;
+            const int64_t sl = (int64_t)((step - d + 1 + a.L) % a.L) * kSubLists + q;
+            c[x] = min(a.spk_cnt[sl * kCntStride], (uint32_t)a.spk_cap);
+        }
+        sum += c[x];
+    }
+    uint32_t run = block_inclusive_scan_256(sum, wtot) - sum;
+    if (threadIdx.x == 0) cum[0] = 0;
+#pragma unroll
+    for (int x = 0; x < kPer; ++x) {
+        run += c[x];
+        if (e0 + x < E) cum[e0 + x + 1] = run;
+    }
+    __syncthreads();
+    return E;
+}
+
+// entry holding segment g (cum[E] > g): the last e with cum[e] <= g
+__device__ inline int32_t entry_of(const uint32_t* cum, int32_t E, uint32_t g)
+{
+    int32_t lo = 0, hi = E - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (cum[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // G lanes per (row, delay) segment (64, 32 or 16): a wavefront works on 64/G
 // segments at once, each lane keeps two synapse loads in flight.
 template <bool PAIR, int G>
 __global__ __launch_bounds__(256) void deliver_kernel(DeliverArgs a, int32_t k)
 {
-    // cum[d] = number of (row, delay) segments with delay <= d
-    __shared__ uint32_t cum[DTB_MAX_DELAY + 1];
-    __shared__ int32_t slot_of[DTB_MAX_DELAY + 1];
+    __shared__ uint32_t cum[kMaxEntries + 1];
+    __shared__ uint32_t wtot[4];
     const int64_t step = a.ctl->nbase + k + 1;
-    if (threadIdx.x == 0) {
-        uint32_t c = 0;
-        cum[0] = 0;
-        for (int32_t d = 1; d <= a.D; ++d) {
-            const int32_t sl = (int32_t)((step - d + 1 + a.L) % a.L);
-            slot_of[d] = sl;
-            c += min(a.spk_cnt[sl], (uint32_t)a.n);
-            cum[d] = c;
-        }
-    }
-    __syncthreads();
-    const uint32_t total = cum[a.D];
+    const int32_t E = delivery_entries(a, k, cum, wtot);
+    const uint32_t total = cum[E];
     const uint32_t lane = __lane_id();
     const uint32_t sub = lane % G;
     constexpr uint32_t kGroups = kWave / G;
     const uint32_t gid = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kGroups + lane / G;
     const uint32_t ngroups = ((gridDim.x * blockDim.x) >> 6) * kGroups;
-    if (gid == 0 && sub == 0 && a.rec_nspk) a.rec_nspk[k] = cum[1];
+    if (gid == 0 && sub == 0 && a.rec_nspk) a.rec_nspk[k] = cum[kSubLists];
     uint64_t ev = 0;
-    int32_t d = 1;
+    int32_t e = 0;
     for (uint32_t r = gid; r < total; r += ngroups) {
-        while (r >= cum[d]) ++d; // r only grows, so d only moves forward
-        const int32_t j = a.spk[(int64_t)slot_of[d] * a.n + (r - cum[d - 1])];
+        while (r >= cum[e + 1]) ++e; // r only grows, so e only moves forward
+        const int32_t d = e / kSubLists + 1;
+        const int32_t j = a.spk[entry_base(a, step, e) + (r - cum[e])];
         const int64_t row = a.rowptr[j];
         int64_t beg, end;
         if (a.D == 1) {
@@ -395,24 +476,6 @@ struct StageArgs {
     uint32_t* ovf;    // [nbk] overflow flags
 };
 
-// Inclusive scan over the 256 lanes of a workgroup (shuffles within each
-// wavefront, then the four wavefront totals through LDS).
-__device__ inline uint32_t block_inclusive_scan_256(uint32_t v, uint32_t* wtot)
-{
-    const uint32_t lane = __lane_id();
-    const uint32_t wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t y = __shfl_up(v, off);
-        if (lane >= (uint32_t)off) v += y;
-    }
-    if (lane == kWave - 1) wtot[wid] = v;
-    __syncthreads();
-    uint32_t add = 0;
-    for (uint32_t w = 0; w < wid; ++w) add += wtot[w];
-    __syncthreads();
-    return v + add;
-}
 
 // SEG (64, 128 or 256) segments per chunk: fewer give more, smaller chunks
 // (parallelism when few neurons spike), more amortise the chunk prologue.
@@ -425,8 +488,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
 {
     static_assert(SEG >= 64 && SEG <= 256, "segment index is packed in 8 bits");
     static_assert(256 * EPT <= (1 << 13), "rank inside a round must fit kRankBits");
-    __shared__ uint32_t cum[DTB_MAX_DELAY + 1];
-    __shared__ int32_t slot_of[DTB_MAX_DELAY + 1];
+    __shared__ uint32_t cum[kMaxEntries + 1];
     __shared__ uint32_t segpre[SEG + 1];
     __shared__ int64_t segbeg[SEG];
     extern __shared__ uint32_t hist[]; // [nbk]
@@ -434,20 +496,10 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     const DeliverArgs& a = s.d;
     const int64_t step = a.ctl->nbase + k + 1;
     const int t = threadIdx.x;
-    if (t == 0) {
-        uint32_t c = 0;
-        cum[0] = 0;
-        for (int32_t d = 1; d <= a.D; ++d) {
-            const int32_t sl = (int32_t)((step - d + 1 + a.L) % a.L);
-            slot_of[d] = sl;
-            c += min(a.spk_cnt[sl], (uint32_t)a.n);
-            cum[d] = c;
-        }
-    }
     for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
-    __syncthreads();
-    const uint32_t total = cum[a.D];
-    if (blockIdx.x == 0 && t == 0 && a.rec_nspk) a.rec_nspk[k] = cum[1];
+    const int32_t NE = delivery_entries(a, k, cum, wtot);
+    const uint32_t total = cum[NE];
+    if (blockIdx.x == 0 && t == 0 && a.rec_nspk) a.rec_nspk[k] = cum[kSubLists];
     const uint32_t nchunks = (total + SEG - 1) / SEG;
     uint64_t ev_wg = 0;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -456,9 +508,9 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
         uint32_t len = 0;
         int64_t beg = 0;
         if (t < SEG && g < total) {
-            int32_t d = 1;
-            while (g >= cum[d]) ++d;
-            const int32_t j = a.spk[(int64_t)slot_of[d] * a.n + (g - cum[d - 1])];
+            const int32_t e = entry_of(cum, NE, g);
+            const int32_t d = e / kSubLists + 1;
+            const int32_t j = a.spk[entry_base(a, step, e) + (g - cum[e])];
             const int64_t row = a.rowptr[j];
             int64_t end;
             if (a.D == 1) {
@@ -1122,8 +1174,9 @@ static int alloc_state(Block* b)
     DTB_TRY(alloc_n(b, &b->last_step, n));
     DTB_TRY(alloc_n(b, &b->t_last, n));
     DTB_TRY(alloc_n(b, &b->flags, n));
-    DTB_TRY(alloc_n(b, &b->spk, (int64_t)(b->D + 1) * n));
-    DTB_TRY(alloc_n(b, &b->spk_cnt, b->D + 1));
+    b->spk_cap = spk_cap_of(n);
+    DTB_TRY(alloc_n(b, &b->spk, (int64_t)(b->D + 1) * kSubLists * b->spk_cap));
+    DTB_TRY(alloc_n(b, &b->spk_cnt, (int64_t)(b->D + 1) * kSubLists * kCntStride));
     DTB_TRY(alloc_n(b, &b->ctl, 1));
     b->kmax = 1000;
     DTB_TRY(alloc_n(b, &b->ttab, b->kmax));
@@ -1133,7 +1186,8 @@ static int alloc_state(Block* b)
     DTB_TRY_HIP(hipMemsetAsync(b->last_step, 0, sizeof(int32_t) * (size_t)n, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->t_last, 0, sizeof(double) * (size_t)n, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->flags, 0, (size_t)n, b->stream));
-    DTB_TRY_HIP(hipMemsetAsync(b->spk_cnt, 0, (size_t)(b->D + 1) * sizeof(uint32_t), b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->spk_cnt, 0, (size_t)(b->D + 1) * kSubLists * kCntStride * sizeof(uint32_t),
+                               b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->ctl, 0, sizeof(StepCtl), b->stream));
     return DTB_OK;
 }
@@ -1414,7 +1468,8 @@ static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* unif
     x.vrst = b->vrst; x.iext = b->iext; x.tref = b->tref; x.noisep = b->noisep; x.pop = b->pop;
     x.pset = b->pset; x.ptab = b->ptab; x.npsets = b->npsets;
     x.V = b->V; x.Jp = b->Jp; x.ref_end = b->ref_end; x.last_step = b->last_step; x.t_last = b->t_last;
-    x.flags = b->flags; x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.L = b->D + 1; x.ctl = b->ctl;
+    x.flags = b->flags; x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.spk_cap = b->spk_cap;
+    x.L = b->D + 1; x.ctl = b->ctl;
     x.clear_acc = b->deliver_mode == 0;
     x.ttab = b->ttab;
     x.dt = b->dt; x.seed = b->seed;
@@ -1430,7 +1485,8 @@ static DeliverArgs deliver_args(Block* b)
     DeliverArgs x;
     x.n = b->n; x.D = b->D; x.L = b->D + 1; x.pk = b->pk; x.rowptr = b->rowptr; x.dend = b->dend; x.col = b->col;
     x.w = b->w; x.w2 = b->w2;
-    x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.ctl = b->ctl; x.rec_events = b->rec_events;
+    x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.spk_cap = b->spk_cap; x.ctl = b->ctl;
+    x.rec_events = b->rec_events;
     x.rec_nspk = b->rec_nspk;
     return x;
 }
@@ -1704,17 +1760,27 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
         st = copy_out(b, dst, tmp, (size_t)n);
         break;
     case DTB_FIELD_SPIKE_LIST: {
+        // the sub-lists of the last step, concatenated and sorted
         const int64_t slot = b->steps_done % (b->D + 1);
-        uint32_t cnt = 0;
+        std::vector<uint32_t> c((size_t)kSubLists * kCntStride, 0u);
         if (b->steps_done > 0) {
-            DTB_TRY_HIP(hipMemcpyAsync(&cnt, b->spk_cnt + slot, 4, hipMemcpyDeviceToHost, b->stream));
+            DTB_TRY_HIP(hipMemcpyAsync(c.data(), b->spk_cnt + slot * kSubLists * kCntStride, 4 * c.size(),
+                                       hipMemcpyDeviceToHost, b->stream));
             DTB_TRY_HIP(hipStreamSynchronize(b->stream));
         }
-        DTB_TRY(need(4 * (int64_t)cnt));
+        int64_t cnt = 0;
+        for (int q = 0; q < kSubLists; ++q) cnt += std::min<int64_t>(c[(size_t)q * kCntStride], b->spk_cap);
+        DTB_TRY(need(4 * cnt));
         if (cnt) {
-            std::vector<int32_t> l(cnt);
-            DTB_TRY_HIP(hipMemcpyAsync(l.data(), b->spk + slot * n, 4 * (size_t)cnt, hipMemcpyDeviceToHost,
-                                       b->stream));
+            std::vector<int32_t> l((size_t)cnt);
+            int64_t at = 0;
+            for (int q = 0; q < kSubLists; ++q) {
+                const int64_t m = std::min<int64_t>(c[(size_t)q * kCntStride], b->spk_cap);
+                if (m)
+                    DTB_TRY_HIP(hipMemcpyAsync(l.data() + at, b->spk + (slot * kSubLists + q) * b->spk_cap,
+                                               4 * (size_t)m, hipMemcpyDeviceToHost, b->stream));
+                at += m;
+            }
             DTB_TRY_HIP(hipStreamSynchronize(b->stream));
             std::sort(l.begin(), l.end());
             DTB_TRY_HIP(hipMemcpy(dst, l.data(), 4 * (size_t)cnt, hipMemcpyDefault));

@@ -60,8 +60,9 @@ struct Block {
     float2* w2 = nullptr;    // pair weights
 
     // spike lists of the last D+1 steps (slot = step mod (D+1))
-    int32_t* spk = nullptr;      // [D+1][n]
-    uint32_t* spk_cnt = nullptr; // [D+1]
+    int32_t* spk = nullptr;      // [D+1][sub-lists][spk_cap]
+    uint32_t* spk_cnt = nullptr; // [D+1][sub-lists], padded
+    int64_t spk_cap = 0;         // entries per sub-list
 
     StepCtl* ctl = nullptr;
     double* ttab = nullptr; // [kmax] time of each step of the batch
