@@ -460,6 +460,9 @@ struct BucketCfg {
 };
 constexpr int kMaxBucketBits = 15;   // neurons per bucket < 2^15 (local index in staged words)
 constexpr int kCursorStride = 32;    // one 128-byte line per bucket cursor / overflow flag
+constexpr int kStageParts = 8;       // staging partitions per bucket: workgroup w reserves in part
+                                     // w % kStageParts, so each cursor takes 1/8 of the (serialising)
+                                     // same-address reservation atomics
 constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
 constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
 
@@ -469,9 +472,9 @@ struct StageArgs {
     int32_t bsize;    // neurons per bucket (< 2^15)
     uint64_t bmagic;  // bucket = (post * bmagic) >> bshift == post / bsize for post < 2^26
     int32_t bshift;
-    int64_t cap;      // staged events per bucket
-    uint32_t* cursor; // [nbk * kCursorStride]
-    void* stage;      // [nbk][cap] uint2 (compact) / uint4 (pair)
+    int64_t cap;      // staged events per bucket partition
+    uint32_t* cursor; // [nbk][kStageParts] (kCursorStride apart)
+    void* stage;      // [nbk][kStageParts][cap] uint2 (compact) / uint4 (pair)
     void* acc2;       // overflow accumulator [n]
     uint32_t* ovf;    // [nbk] overflow flags
 };
@@ -496,6 +499,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     const DeliverArgs& a = s.d;
     const int64_t step = a.ctl->nbase + k + 1;
     const int t = threadIdx.x;
+    const int32_t part = (int32_t)(blockIdx.x % kStageParts);
     for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
     const int32_t NE = delivery_entries(a, k, cum, wtot);
     const uint32_t total = cum[NE];
@@ -581,7 +585,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
             // reserve space in each touched bucket; hist[b] becomes the base
             for (int b = t; b < s.nbk; b += 256) {
                 const uint32_t h = hist[b];
-                hist[b] = h ? atomicAdd(&s.cursor[(size_t)b * kCursorStride], h) : 0u;
+                hist[b] = h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
             }
             __syncthreads();
 #pragma unroll
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                 const uint32_t bk = br[m] >> kRankBits;
                 const uint64_t pos = (uint64_t)hist[bk] + (br[m] & ((1u << kRankBits) - 1));
                 if (pos < (uint64_t)s.cap) {
-                    const int64_t at = (int64_t)bk * s.cap + (int64_t)pos;
+                    const int64_t at = ((int64_t)bk * kStageParts + part) * s.cap + (int64_t)pos;
                     if (PAIR)
                         reinterpret_cast<uint4*>(s.stage)[at] =
                             make_uint4(pk_[m], __float_as_uint(w0[m]), __float_as_uint(w1[m]), 0u);
@@ -646,18 +650,31 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     const int32_t b = blockIdx.x;
     const int t = threadIdx.x;
     for (int x = t; x < s.bsize * C; x += blockDim.x) slice[x] = 0.0f;
-    const uint32_t cnt = (uint32_t)min((uint64_t)s.cursor[(size_t)b * kCursorStride], (uint64_t)s.cap);
+    // the bucket's partitions back to back: pre[p] = events in partitions < p
+    uint32_t pre[kStageParts + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int p = 0; p < kStageParts; ++p)
+        pre[p + 1] = pre[p] + (uint32_t)min((uint64_t)s.cursor[((size_t)b * kStageParts + p) * kCursorStride],
+                                            (uint64_t)s.cap);
+    const uint32_t cnt = pre[kStageParts];
     const bool ovf = s.ovf[(size_t)b * kCursorStride] != 0u;
     __syncthreads();
     constexpr int U = 8; // staged events in flight per lane
-    const int64_t sb = (int64_t)b * s.cap;
+    const int64_t sb = (int64_t)b * kStageParts * s.cap;
+    auto at = [&](uint32_t e) -> int64_t {
+        int p = 0;
+#pragma unroll
+        for (int q = 1; q < kStageParts; ++q) p += e >= pre[q] ? 1 : 0;
+        return sb + (int64_t)p * s.cap + (e - pre[p]);
+    };
     for (uint32_t base = t; base < cnt; base += blockDim.x * U) {
         if (PAIR) {
             uint4 v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t e = base + (uint32_t)u * blockDim.x;
-                v[u] = e < cnt ? reinterpret_cast<const uint4*>(s.stage)[sb + e] : make_uint4(0u, 0u, 0u, 0u);
+                v[u] = e < cnt ? reinterpret_cast<const uint4*>(s.stage)[at(e)] : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -671,7 +688,7 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t e = base + (uint32_t)u * blockDim.x;
-                v[u] = e < cnt ? reinterpret_cast<const uint2*>(s.stage)[sb + e] : make_uint2(0u, 0u);
+                v[u] = e < cnt ? reinterpret_cast<const uint2*>(s.stage)[at(e)] : make_uint2(0u, 0u);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -709,10 +726,8 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
         }
     }
     __syncthreads();
-    if (t == 0) {
-        s.cursor[(size_t)b * kCursorStride] = 0u;
-        s.ovf[(size_t)b * kCursorStride] = 0u;
-    }
+    if (t < kStageParts) s.cursor[((size_t)b * kStageParts + t) * kCursorStride] = 0u;
+    if (t == 0) s.ovf[(size_t)b * kCursorStride] = 0u;
 }
 
 // Batch prologue: times of the K steps, step counters, record clearing.
@@ -1247,14 +1262,15 @@ static int ensure_bucketed(Block* b)
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     // room for 1% of all synapses per step (a mean rate of 100 Hz at 0.1 ms),
-    // at least 8192 events per bucket; fuller buckets fall back to atomics
-    b->stage_cap = std::max<int64_t>(8192, (b->nnz / 100 + b->nbk - 1) / b->nbk);
+    // at least 1024 events per bucket partition; fuller partitions fall back to atomics
+    b->stage_cap = std::max<int64_t>(1024, (b->nnz / 100 + (int64_t)b->nbk * kStageParts - 1) /
+                                               ((int64_t)b->nbk * kStageParts));
     const size_t esz = b->pair ? 16 : 8;
-    DTB_TRY(alloc_n(b, &b->cursor, (int64_t)b->nbk * kCursorStride));
+    DTB_TRY(alloc_n(b, &b->cursor, (int64_t)b->nbk * kStageParts * kCursorStride));
     DTB_TRY(alloc_n(b, &b->ovf, (int64_t)b->nbk * kCursorStride));
-    DTB_TRY(block_alloc(b, &b->stage, esz * (size_t)b->nbk * (size_t)b->stage_cap));
+    DTB_TRY(block_alloc(b, &b->stage, esz * (size_t)b->nbk * kStageParts * (size_t)b->stage_cap));
     DTB_TRY(block_alloc(b, &b->acc2, (b->pair ? 16 : 8) * (size_t)b->n));
-    DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)b->nbk * kCursorStride, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)b->nbk * kStageParts * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)b->nbk * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->acc2, 0, (b->pair ? 16 : 8) * (size_t)b->n, b->stream));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
