@@ -480,13 +480,83 @@ struct StageArgs {
 };
 
 
+// Stage-kernel pieces.  Event e of a chunk lives in segment sg at offset
+// e - segpre[sg]; loc packs (sg << 24) | offset, or ~0 past the chunk's end.
+template <int EPT>
+__device__ inline void stage_locate(const uint32_t* segpre, uint32_t r0, uint32_t E, int t, uint32_t* loc)
+{
+    int sg = 0;
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const uint32_t e = r0 + (uint32_t)m * 256 + t;
+        loc[m] = 0xffffffffu;
+        if (e < E) {
+            while (segpre[sg + 1] <= e) ++sg; // e grows with m
+            loc[m] = ((uint32_t)sg << 24) | (e - segpre[sg]);
+        }
+    }
+}
+
+// all of a lane's loads in flight together
+template <bool PAIR, int EPT>
+__device__ inline void stage_load(const DeliverArgs& a, const int64_t* segbeg, const uint32_t* loc, uint32_t* cw,
+                                  float* w0, float* w1)
+{
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        cw[m] = 0u;
+        w0[m] = 0.0f;
+        w1[m] = 0.0f;
+        if (loc[m] != 0xffffffffu) {
+            const int64_t idx = segbeg[loc[m] >> 24] + (loc[m] & 0xffffffu);
+            cw[m] = a.col[idx];
+            if (PAIR) {
+                const float2 wv = a.w2[idx];
+                w0[m] = wv.x;
+                w1[m] = wv.y;
+            } else {
+                w0[m] = a.w[idx];
+            }
+        }
+    }
+}
+
+// staged event at position pos of bucket bk's partition, or through the
+// overflow accumulator when the partition is full
+template <bool PAIR>
+__device__ inline void stage_put(const StageArgs& s, int32_t part, uint32_t bk, uint64_t pos, uint32_t pk, float w0,
+                                 float w1)
+{
+    if (pos < (uint64_t)s.cap) {
+        const int64_t at = ((int64_t)bk * kStageParts + part) * s.cap + (int64_t)pos;
+        if (PAIR)
+            reinterpret_cast<uint4*>(s.stage)[at] = make_uint4(pk, __float_as_uint(w0), __float_as_uint(w1), 0u);
+        else
+            reinterpret_cast<uint2*>(s.stage)[at] = make_uint2(pk, __float_as_uint(w0));
+    } else {
+        const int64_t post = (int64_t)bk * s.bsize + (pk & 0x7fffffffu);
+        const uint32_t kind = pk >> 31;
+        if (PAIR) {
+            float* dst = reinterpret_cast<float*>(s.acc2) + (post << 2) + (kind << 1);
+            unsafeAtomicAdd(dst, w0);
+            unsafeAtomicAdd(dst + 1, w1);
+        } else {
+            unsafeAtomicAdd(reinterpret_cast<float*>(s.acc2) + (post << 1) + kind, w0);
+        }
+        s.ovf[(size_t)bk * kCursorStride] = 1u;
+    }
+}
+
 // SEG (64, 128 or 256) segments per chunk: fewer give more, smaller chunks
 // (parallelism when few neurons spike), more amortise the chunk prologue.
 // EPT events per lane per round (16 or 32): fewer use fewer registers (more
 // resident waves), more take fewer staging reservations.  The bucket
 // histogram is dynamic LDS of nbk words, so small bucket counts leave room for
-// more resident workgroups.
-template <bool PAIR, int SEG, int EPT>
+// more resident workgroups.  TWO: each chunk takes one staging reservation
+// per bucket (a histogram pass over the chunk's targets, then a second pass
+// that writes) instead of one per round, at the price of reading the targets
+// twice.
+template <bool PAIR, int SEG, int EPT, bool TWO>
 __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t k)
 {
     static_assert(SEG >= 64 && SEG <= 256, "segment index is packed in 8 bits");
@@ -536,81 +606,81 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
         __syncthreads();
         const uint32_t E = segpre[SEG];
         ev_wg += E;
-        for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
-            uint32_t br[EPT], pk_[EPT]; // br = bucket << kRankBits | rank
-            float w0[EPT], w1[EPT];
-            // (a) where each event lives: segment (8 bits) and offset in it (24 bits), from LDS only
-            uint32_t loc[EPT];
-            int sg = 0;
+        if constexpr (!TWO) {
+            for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
+                uint32_t br[EPT], pk_[EPT]; // br = bucket << kRankBits | rank
+                float w0[EPT], w1[EPT];
+                uint32_t loc[EPT];
+                stage_locate<EPT>(segpre, r0, E, t, loc);
+                uint32_t cw[EPT];
+                stage_load<PAIR, EPT>(a, segbeg, loc, cw, w0, w1);
+                // bucket and rank inside the bucket within this round
 #pragma unroll
-            for (int m = 0; m < EPT; ++m) {
-                const uint32_t e = r0 + (uint32_t)m * 256 + t;
-                loc[m] = 0xffffffffu;
-                if (e < E) {
-                    while (segpre[sg + 1] <= e) ++sg; // e grows with m
-                    loc[m] = ((uint32_t)sg << 24) | (e - segpre[sg]);
-                }
-            }
-            // (b) all loads in flight together
-            uint32_t cw[EPT];
-#pragma unroll
-            for (int m = 0; m < EPT; ++m) {
-                cw[m] = 0u;
-                w0[m] = 0.0f;
-                w1[m] = 0.0f;
-                if (loc[m] != 0xffffffffu) {
-                    const int64_t idx = segbeg[loc[m] >> 24] + (loc[m] & 0xffffffu);
-                    cw[m] = a.col[idx];
-                    if (PAIR) {
-                        const float2 wv = a.w2[idx];
-                        w0[m] = wv.x;
-                        w1[m] = wv.y;
-                    } else {
-                        w0[m] = a.w[idx];
+                for (int m = 0; m < EPT; ++m) {
+                    br[m] = 0xffffffffu;
+                    if (loc[m] != 0xffffffffu) {
+                        const uint32_t post = cw[m] & a.pk.idx_mask;
+                        const uint32_t bk = (uint32_t)(((uint64_t)post * s.bmagic) >> s.bshift);
+                        pk_[m] = (post - bk * (uint32_t)s.bsize) | (cw[m] & 0x80000000u);
+                        br[m] = (bk << kRankBits) | atomicAdd(&hist[bk], 1u);
                     }
                 }
-            }
-            // (c) bucket and rank inside the bucket
+                __syncthreads();
+                // reserve space in each touched bucket; hist[b] becomes the base
+                for (int b = t; b < s.nbk; b += 256) {
+                    const uint32_t h = hist[b];
+                    hist[b] = h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
+                }
+                __syncthreads();
 #pragma unroll
-            for (int m = 0; m < EPT; ++m) {
-                br[m] = 0xffffffffu;
-                if (loc[m] != 0xffffffffu) {
+                for (int m = 0; m < EPT; ++m) {
+                    if (br[m] == 0xffffffffu) continue;
+                    const uint32_t bk = br[m] >> kRankBits;
+                    stage_put<PAIR>(s, part, bk, (uint64_t)hist[bk] + (br[m] & ((1u << kRankBits) - 1)), pk_[m],
+                                    w0[m], w1[m]);
+                }
+                __syncthreads();
+                for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
+                __syncthreads();
+            }
+        } else {
+            // pass 1: the chunk's bucket histogram (targets only)
+            for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
+                uint32_t loc[EPT];
+                stage_locate<EPT>(segpre, r0, E, t, loc);
+                uint32_t cw[EPT];
+#pragma unroll
+                for (int m = 0; m < EPT; ++m)
+                    cw[m] = loc[m] != 0xffffffffu ? a.col[segbeg[loc[m] >> 24] + (loc[m] & 0xffffffu)] : 0u;
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    if (loc[m] == 0xffffffffu) continue;
                     const uint32_t post = cw[m] & a.pk.idx_mask;
-                    const uint32_t bk = (uint32_t)(((uint64_t)post * s.bmagic) >> s.bshift);
-                    pk_[m] = (post - bk * (uint32_t)s.bsize) | (cw[m] & 0x80000000u);
-                    br[m] = (bk << kRankBits) | atomicAdd(&hist[bk], 1u);
+                    atomicAdd(&hist[(uint32_t)(((uint64_t)post * s.bmagic) >> s.bshift)], 1u);
                 }
             }
             __syncthreads();
-            // reserve space in each touched bucket; hist[b] becomes the base
+            // one reservation per touched bucket for the whole chunk; hist[b] becomes the base
             for (int b = t; b < s.nbk; b += 256) {
                 const uint32_t h = hist[b];
                 hist[b] = h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
             }
             __syncthreads();
+            // pass 2: reload with the weights, take positions from the bases, write
+            for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
+                float w0[EPT], w1[EPT];
+                uint32_t loc[EPT];
+                stage_locate<EPT>(segpre, r0, E, t, loc);
+                uint32_t cw[EPT];
+                stage_load<PAIR, EPT>(a, segbeg, loc, cw, w0, w1);
 #pragma unroll
-            for (int m = 0; m < EPT; ++m) {
-                if (br[m] == 0xffffffffu) continue;
-                const uint32_t bk = br[m] >> kRankBits;
-                const uint64_t pos = (uint64_t)hist[bk] + (br[m] & ((1u << kRankBits) - 1));
-                if (pos < (uint64_t)s.cap) {
-                    const int64_t at = ((int64_t)bk * kStageParts + part) * s.cap + (int64_t)pos;
-                    if (PAIR)
-                        reinterpret_cast<uint4*>(s.stage)[at] =
-                            make_uint4(pk_[m], __float_as_uint(w0[m]), __float_as_uint(w1[m]), 0u);
-                    else
-                        reinterpret_cast<uint2*>(s.stage)[at] = make_uint2(pk_[m], __float_as_uint(w0[m]));
-                } else {
-                    const int64_t post = (int64_t)bk * s.bsize + (pk_[m] & 0x7fffffffu);
-                    const uint32_t kind = pk_[m] >> 31;
-                    if (PAIR) {
-                        float* dst = reinterpret_cast<float*>(s.acc2) + (post << 2) + (kind << 1);
-                        unsafeAtomicAdd(dst, w0[m]);
-                        unsafeAtomicAdd(dst + 1, w1[m]);
-                    } else {
-                        unsafeAtomicAdd(reinterpret_cast<float*>(s.acc2) + (post << 1) + kind, w0[m]);
-                    }
-                    s.ovf[(size_t)bk * kCursorStride] = 1u;
+                for (int m = 0; m < EPT; ++m) {
+                    if (loc[m] == 0xffffffffu) continue;
+                    const uint32_t post = cw[m] & a.pk.idx_mask;
+                    const uint32_t bk = (uint32_t)(((uint64_t)post * s.bmagic) >> s.bshift);
+                    const uint32_t pos = atomicAdd(&hist[bk], 1u);
+                    stage_put<PAIR>(s, part, bk, pos, (post - bk * (uint32_t)s.bsize) | (cw[m] & 0x80000000u), w0[m],
+                                    w1[m]);
                 }
             }
             __syncthreads();
@@ -621,23 +691,28 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     if (t == 0 && ev_wg && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev_wg);
 }
 
-template <bool PAIR, int EPT>
+template <bool PAIR, int EPT, bool TWO>
 static void launch_stage_ept(const Block* b, const StageArgs& sa, int32_t k)
 {
     const dim3 g(b->deliver_grid), t(256);
     const size_t lds = sizeof(uint32_t) * (size_t)b->nbk;
     switch (b->stage_segs) {
-    case 64: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 64, EPT>), g, t, lds, b->stream, sa, k); break;
-    case 128: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 128, EPT>), g, t, lds, b->stream, sa, k); break;
-    default: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 256, EPT>), g, t, lds, b->stream, sa, k); break;
+    case 64: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 64, EPT, TWO>), g, t, lds, b->stream, sa, k); break;
+    case 128: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 128, EPT, TWO>), g, t, lds, b->stream, sa, k); break;
+    default: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 256, EPT, TWO>), g, t, lds, b->stream, sa, k); break;
     }
 }
 
 template <bool PAIR>
 static void launch_stage(const Block* b, const StageArgs& sa, int32_t k)
 {
-    if (b->stage_ept == 16) launch_stage_ept<PAIR, 16>(b, sa, k);
-    else launch_stage_ept<PAIR, 32>(b, sa, k);
+    if (b->stage_passes == 2) {
+        if (b->stage_ept == 16) launch_stage_ept<PAIR, 16, true>(b, sa, k);
+        else launch_stage_ept<PAIR, 32, true>(b, sa, k);
+    } else {
+        if (b->stage_ept == 16) launch_stage_ept<PAIR, 16, false>(b, sa, k);
+        else launch_stage_ept<PAIR, 32, false>(b, sa, k);
+    }
 }
 
 template <bool PAIR>
@@ -2257,6 +2332,9 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
     } else if (what == DTB_LAUNCH_STAGE_EPT) {
         DTB_REQUIRE(value == 16 || value == 32, "events per lane per round must be 16 or 32");
         b->stage_ept = value;
+    } else if (what == DTB_LAUNCH_STAGE_PASSES) {
+        DTB_REQUIRE(value == 1 || value == 2, "staging passes must be 1 or 2");
+        b->stage_passes = value;
     } else if (what == DTB_LAUNCH_DELIVER_MODE) {
         DTB_REQUIRE(value == 0 || value == 1, "delivery mode must be 0 (atomics) or 1 (bucketed)");
         if (value == 1) DTB_TRY(ensure_bucketed(b));
@@ -2279,6 +2357,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     else if (what == DTB_LAUNCH_NEURON_PSETS) *value = b->npsets;
     else if (what == DTB_LAUNCH_STAGE_SEGS) *value = b->stage_segs;
     else if (what == DTB_LAUNCH_STAGE_EPT) *value = b->stage_ept;
+    else if (what == DTB_LAUNCH_STAGE_PASSES) *value = b->stage_passes;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

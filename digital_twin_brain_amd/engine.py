@@ -205,7 +205,8 @@ class NativeBlock:
         return thr
 
     # -------------------------------------------------------- launch shape
-    LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2, neuron_psets=3, stage_segs=4, stage_ept=5)
+    LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2, neuron_psets=3, stage_segs=4, stage_ept=5,
+                  stage_passes=6)
 
     def set_launch(self, **kw):
         for k, v in kw.items():
