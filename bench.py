@@ -104,8 +104,8 @@ def main():
     ap.add_argument("--da-windows", type=int, default=20)
     ap.add_argument("--train-windows", type=int, default=16, help="surrogate training windows (x64 samples)")
     ap.add_argument("--no-graph", action="store_true", help="launch step batches directly (for counter runs)")
-    ap.add_argument("--launch", default="", metavar="MODE,GRID,SEGS[,EPT[,PASSES]]",
-                    help="fixed delivery launch shape instead of the autotuner (e.g. 1,1024,128,32,1)")
+    ap.add_argument("--launch", default="", metavar="MODE,GRID,SEGS[,EPT[,VARIANT]]",
+                    help="fixed delivery launch shape instead of the autotuner (e.g. 1,1024,128,32,0)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -158,9 +158,9 @@ def main():
         v = [int(x) for x in a.launch.split(",")]
         mode, grid, segs = v[:3]
         ept = v[3] if len(v) > 3 else 32
-        passes = v[4] if len(v) > 4 else 1
-        nb.set_launch(deliver_mode=mode, deliver_grid=grid, stage_segs=segs, stage_ept=ept, stage_passes=passes)
-        log(f"fixed delivery launch: mode={mode} grid={grid} segs={segs} ept={ept} passes={passes}")
+        var = v[4] if len(v) > 4 else 0
+        nb.set_launch(deliver_mode=mode, deliver_grid=grid, stage_segs=segs, stage_ept=ept, stage_variant=var)
+        log(f"fixed delivery launch: mode={mode} grid={grid} segs={segs} ept={ept} variant={var}")
     elif not a.no_tune:
         from digital_twin_brain_amd.autotune import tune_delivery
 

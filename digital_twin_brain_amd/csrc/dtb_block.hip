@@ -552,11 +552,14 @@ __device__ inline void stage_put(const StageArgs& s, int32_t part, uint32_t bk, 
 // EPT events per lane per round (16 or 32): fewer use fewer registers (more
 // resident waves), more take fewer staging reservations.  The bucket
 // histogram is dynamic LDS of nbk words, so small bucket counts leave room for
-// more resident workgroups.  TWO: each chunk takes one staging reservation
-// per bucket (a histogram pass over the chunk's targets, then a second pass
-// that writes) instead of one per round, at the price of reading the targets
-// twice.
-template <bool PAIR, int SEG, int EPT, bool TWO>
+// more resident workgroups.  VAR selects how events reach the staging arrays:
+//   0  one reservation per (round, bucket); each lane stores its own events;
+//   1  one reservation per (chunk, bucket): a histogram pass over the chunk's
+//      targets, then a second pass that stores (targets read twice);
+//   2  one reservation per (round, bucket), the round's events first sorted
+//      by bucket in LDS so that consecutive lanes store each bucket's run
+//      contiguously (whole cache lines instead of scattered 8-byte stores).
+template <bool PAIR, int SEG, int EPT, int VAR>
 __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t k)
 {
     static_assert(SEG >= 64 && SEG <= 256, "segment index is packed in 8 bits");
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
         __syncthreads();
         const uint32_t E = segpre[SEG];
         ev_wg += E;
-        if constexpr (!TWO) {
+        if constexpr (VAR == 0) {
             for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
                 uint32_t br[EPT], pk_[EPT]; // br = bucket << kRankBits | rank
                 float w0[EPT], w1[EPT];
@@ -638,6 +641,71 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                     const uint32_t bk = br[m] >> kRankBits;
                     stage_put<PAIR>(s, part, bk, (uint64_t)hist[bk] + (br[m] & ((1u << kRankBits) - 1)), pk_[m],
                                     w0[m], w1[m]);
+                }
+                __syncthreads();
+                for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
+                __syncthreads();
+            }
+        } else if constexpr (VAR == 2) {
+            uint32_t* pre = hist + s.nbk;      // [nbk] first LDS slot of each bucket in the round
+            uint32_t* base = pre + s.nbk;      // [nbk] reserved staging position of each bucket
+            uint32_t* bpk = base + s.nbk;      // [256 * EPT] sorted round: packed target
+            uint32_t* bw0 = bpk + 256 * EPT;   //                           weight 0
+            uint32_t* bw1 = bw0 + 256 * EPT;   //                           weight 1 (pair)
+            const int per = (s.nbk + 255) / 256;
+            for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
+                uint32_t br[EPT], pk_[EPT];
+                float w0[EPT], w1[EPT];
+                uint32_t loc[EPT];
+                stage_locate<EPT>(segpre, r0, E, t, loc);
+                uint32_t cw[EPT];
+                stage_load<PAIR, EPT>(a, segbeg, loc, cw, w0, w1);
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    br[m] = 0xffffffffu;
+                    if (loc[m] != 0xffffffffu) {
+                        const uint32_t post = cw[m] & a.pk.idx_mask;
+                        const uint32_t bk = (uint32_t)(((uint64_t)post * s.bmagic) >> s.bshift);
+                        pk_[m] = (post - bk * (uint32_t)s.bsize) | (cw[m] & 0x80000000u);
+                        br[m] = (bk << kRankBits) | atomicAdd(&hist[bk], 1u);
+                    }
+                }
+                __syncthreads();
+                // bucket offsets inside the round (exclusive prefix) and staging reservations
+                const int b0 = t * per;
+                uint32_t sum = 0;
+                for (int x = 0; x < per; ++x)
+                    if (b0 + x < s.nbk) sum += hist[b0 + x];
+                uint32_t run = block_inclusive_scan_256(sum, wtot) - sum;
+                for (int x = 0; x < per; ++x) {
+                    const int b = b0 + x;
+                    if (b >= s.nbk) break;
+                    const uint32_t h = hist[b];
+                    pre[b] = run;
+                    run += h;
+                    base[b] = h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    if (br[m] == 0xffffffffu) continue;
+                    const uint32_t j = pre[br[m] >> kRankBits] + (br[m] & ((1u << kRankBits) - 1));
+                    bpk[j] = pk_[m];
+                    bw0[j] = __float_as_uint(w0[m]);
+                    if (PAIR) bw1[j] = __float_as_uint(w1[m]);
+                }
+                __syncthreads();
+                // consecutive lanes store consecutive events of a bucket
+                const uint32_t cnt = min(E - r0, (uint32_t)(256 * EPT));
+                for (uint32_t j = t; j < cnt; j += 256) {
+                    int32_t lo = 0, hi = s.nbk - 1; // last bucket with pre[b] <= j
+                    while (lo < hi) {
+                        const int32_t mid = (lo + hi + 1) >> 1;
+                        if (pre[mid] <= j) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    stage_put<PAIR>(s, part, (uint32_t)lo, (uint64_t)base[lo] + (j - pre[lo]), bpk[j],
+                                    __uint_as_float(bw0[j]), PAIR ? __uint_as_float(bw1[j]) : 0.0f);
                 }
                 __syncthreads();
                 for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
@@ -691,28 +759,52 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     if (t == 0 && ev_wg && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev_wg);
 }
 
-template <bool PAIR, int EPT, bool TWO>
+// dynamic LDS of the stage kernel: the bucket histogram, plus for the sorted
+// variant the round's bucket offsets, reservations and sorted events
+static size_t stage_lds(const Block* b, int ept, int var)
+{
+    size_t words = (size_t)b->nbk;
+    if (var == 2) words += 2 * (size_t)b->nbk + (size_t)256 * ept * (b->pair ? 3 : 2);
+    return sizeof(uint32_t) * words;
+}
+
+template <bool PAIR, int EPT, int VAR>
 static void launch_stage_ept(const Block* b, const StageArgs& sa, int32_t k)
 {
     const dim3 g(b->deliver_grid), t(256);
-    const size_t lds = sizeof(uint32_t) * (size_t)b->nbk;
+    const size_t lds = stage_lds(b, EPT, VAR);
     switch (b->stage_segs) {
-    case 64: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 64, EPT, TWO>), g, t, lds, b->stream, sa, k); break;
-    case 128: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 128, EPT, TWO>), g, t, lds, b->stream, sa, k); break;
-    default: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 256, EPT, TWO>), g, t, lds, b->stream, sa, k); break;
+    case 64: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 64, EPT, VAR>), g, t, lds, b->stream, sa, k); break;
+    case 128: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 128, EPT, VAR>), g, t, lds, b->stream, sa, k); break;
+    default: hipLaunchKernelGGL((deliver_stage_kernel<PAIR, 256, EPT, VAR>), g, t, lds, b->stream, sa, k); break;
     }
 }
 
 template <bool PAIR>
 static void launch_stage(const Block* b, const StageArgs& sa, int32_t k)
 {
-    if (b->stage_passes == 2) {
-        if (b->stage_ept == 16) launch_stage_ept<PAIR, 16, true>(b, sa, k);
-        else launch_stage_ept<PAIR, 32, true>(b, sa, k);
-    } else {
-        if (b->stage_ept == 16) launch_stage_ept<PAIR, 16, false>(b, sa, k);
-        else launch_stage_ept<PAIR, 32, false>(b, sa, k);
+    switch (b->stage_variant) {
+    case 2: launch_stage_ept<PAIR, 16, 2>(b, sa, k); break; // sorted rounds use 16 events per lane
+    case 1:
+        if (b->stage_ept == 16) launch_stage_ept<PAIR, 16, 1>(b, sa, k);
+        else launch_stage_ept<PAIR, 32, 1>(b, sa, k);
+        break;
+    default:
+        if (b->stage_ept == 16) launch_stage_ept<PAIR, 16, 0>(b, sa, k);
+        else launch_stage_ept<PAIR, 32, 0>(b, sa, k);
     }
+}
+
+template <bool PAIR>
+static int allow_sorted_stage_lds(int bytes)
+{
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_stage_kernel<PAIR, 64, 16, 2>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_stage_kernel<PAIR, 128, 16, 2>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_stage_kernel<PAIR, 256, 16, 2>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return DTB_OK;
 }
 
 template <bool PAIR>
@@ -1336,6 +1428,10 @@ static int ensure_bucketed(Block* b)
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    // the sorted staging variant needs its round in LDS next to ~12 KB of static arrays
+    b->sorted_ok = stage_lds(b, 16, 2) + 16384 <= (size_t)lds;
+    DTB_TRY(allow_sorted_stage_lds<true>(lds - 16384));
+    DTB_TRY(allow_sorted_stage_lds<false>(lds - 16384));
     // room for 1% of all synapses per step (a mean rate of 100 Hz at 0.1 ms),
     // at least 1024 events per bucket partition; fuller partitions fall back to atomics
     b->stage_cap = std::max<int64_t>(1024, (b->nnz / 100 + (int64_t)b->nbk * kStageParts - 1) /
@@ -2332,9 +2428,13 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
     } else if (what == DTB_LAUNCH_STAGE_EPT) {
         DTB_REQUIRE(value == 16 || value == 32, "events per lane per round must be 16 or 32");
         b->stage_ept = value;
-    } else if (what == DTB_LAUNCH_STAGE_PASSES) {
-        DTB_REQUIRE(value == 1 || value == 2, "staging passes must be 1 or 2");
-        b->stage_passes = value;
+    } else if (what == DTB_LAUNCH_STAGE_VARIANT) {
+        DTB_REQUIRE(value >= 0 && value <= 2, "staging variant must be 0, 1 or 2");
+        if (value == 2) {
+            DTB_TRY(ensure_bucketed(b));
+            DTB_REQUIRE(b->sorted_ok, "too many delivery buckets for the sorted staging variant");
+        }
+        b->stage_variant = value;
     } else if (what == DTB_LAUNCH_DELIVER_MODE) {
         DTB_REQUIRE(value == 0 || value == 1, "delivery mode must be 0 (atomics) or 1 (bucketed)");
         if (value == 1) DTB_TRY(ensure_bucketed(b));
@@ -2357,7 +2457,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     else if (what == DTB_LAUNCH_NEURON_PSETS) *value = b->npsets;
     else if (what == DTB_LAUNCH_STAGE_SEGS) *value = b->stage_segs;
     else if (what == DTB_LAUNCH_STAGE_EPT) *value = b->stage_ept;
-    else if (what == DTB_LAUNCH_STAGE_PASSES) *value = b->stage_passes;
+    else if (what == DTB_LAUNCH_STAGE_VARIANT) *value = b->stage_variant;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

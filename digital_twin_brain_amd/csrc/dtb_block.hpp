@@ -101,7 +101,8 @@ struct Block {
     int32_t deliver_mode = 0;   // 0: global atomics, 1: bucketed (staged + LDS accumulation)
     int32_t stage_segs = 256;   // bucketed: (row, delay) segments per staging chunk
     int32_t stage_ept = 32;     // bucketed: events per lane per staging round
-    int32_t stage_passes = 1;   // bucketed: 2 = one staging reservation per chunk (two passes)
+    int32_t stage_variant = 0;  // bucketed: 0 scattered stores, 1 two passes per chunk, 2 LDS-sorted rounds
+    bool sorted_ok = false;     // the sorted variant's LDS fits
     ExtDrive ext{};             // external Poisson drive (n = 0: off)
     int32_t bsize = 0;          // bucketed: neurons per destination bucket
     uint64_t bmagic = 0;        // bucket = (post * bmagic) >> bshift

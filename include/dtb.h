@@ -243,7 +243,9 @@ int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* t
                                       distinct; get: number of sets in use (0 = per-neuron) */
 #define DTB_LAUNCH_STAGE_SEGS 4    /* bucketed delivery: 64, 128 or 256 (row, delay) segments per chunk */
 #define DTB_LAUNCH_STAGE_EPT 5     /* bucketed delivery: 16 or 32 events per lane per staging round */
-#define DTB_LAUNCH_STAGE_PASSES 6  /* bucketed delivery: 1 (a staging reservation per round) or 2 (per chunk) */
+#define DTB_LAUNCH_STAGE_VARIANT 6 /* bucketed delivery staging: 0 a reservation per round, lanes store their own
+                                      events; 1 a reservation per chunk (two passes); 2 rounds sorted by
+                                      bucket in LDS, stored as contiguous runs (16 events per lane) */
 int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
 int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 
