@@ -52,7 +52,7 @@ def _save(c):
 def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
     """Pick the delivery launch shape for `nb` under `run_kwargs` (noise mode/rate).
 
-    Returns (best, table) where table maps 'mode/lanes/grid/segs/ept/variant' to mean delivery us.
+    Returns (best, table) where table maps 'mode/lanes/grid/segs/ept/variant/occ' to mean delivery us.
     """
     step_q = getattr(nb, "bold_sps", 1) or 1
     steps = max(step_q, steps - steps % step_q)
@@ -65,39 +65,41 @@ def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
         best = cache[ckey]
         nb.set_launch(deliver_mode=best["deliver_mode"], deliver_lanes=best["deliver_lanes"],
                       deliver_grid=best["deliver_grid"], stage_segs=best.get("stage_segs", 256),
-                      stage_ept=best.get("stage_ept", 32), stage_variant=best.get("stage_variant", 0))
+                      stage_ept=best.get("stage_ept", 32), stage_variant=best.get("stage_variant", 0),
+                      accum_occ=best.get("accum_occ", 1))
         return best, {"cached": ckey}
     segs = max(64.0, float(spk.mean()) * nb.max_delay)
     cands = []
     for lanes in (64, 32, 16):  # direct atomics: one lane group per segment
         base = max(4, int(math.ceil(segs / (4 * (64 // lanes)))))
         for mult in (0.25, 0.5, 1.0, 2.0):
-            cands.append((0, lanes, int(min(65536, max(4, base * mult))), 256, 32, 0))
+            cands.append((0, lanes, int(min(65536, max(4, base * mult))), 256, 32, 0, 1))
     if nb.n <= (1 << 26):  # bucketed: workgroups each expanding `sg` segments at a time
         for sg in (64, 128, 256):
             chunks = max(1, int(math.ceil(segs / sg)))
-            for grid in sorted({chunks, 1024, 2048, 4096}):
-                for ept, var in ((16, 0), (32, 0), (16, 1), (32, 1), (16, 2)):
-                    cands.append((1, 64, grid, sg, ept, var))
+            for grid in sorted({chunks, 1024, 2048}):
+                for ept, var in ((16, 0), (32, 0), (16, 1), (16, 2)):
+                    for occ in (1, 2):
+                        cands.append((1, 64, grid, sg, ept, var, occ))
     table = {}
-    for mode, lanes, grid, sg, ept, var in cands:
+    for mode, lanes, grid, sg, ept, var, occ in cands:
         try:
             nb.set_launch(deliver_mode=mode, deliver_lanes=lanes, deliver_grid=grid, stage_segs=sg, stage_ept=ept,
-                          stage_variant=var)
+                          accum_occ=occ, stage_variant=var)
         except nat_error():
             continue
         nb.run(steps, graph=False, kernel_ms=kms, **run_kwargs)
-        key = f"{mode}/{lanes}/{grid}/{sg}/{ept}/{var}"
+        key = f"{mode}/{lanes}/{grid}/{sg}/{ept}/{var}/{occ}"
         table[key] = float(kms[:, 1].mean()) * 1e3
         if log:
-            log(f"tune mode={mode} lanes={lanes} grid={grid} segs={sg} ept={ept} variant={var}: "
+            log(f"tune mode={mode} lanes={lanes} grid={grid} segs={sg} ept={ept} variant={var} occ={occ}: "
                 f"deliver {table[key]:.1f} us")
     bkey = min(table, key=table.get)
-    mode, lanes, grid, sg, ept, var = (int(x) for x in bkey.split("/"))
+    mode, lanes, grid, sg, ept, var, occ = (int(x) for x in bkey.split("/"))
     best = {"deliver_mode": mode, "deliver_lanes": lanes, "deliver_grid": grid, "stage_segs": sg, "stage_ept": ept,
-            "stage_variant": var, "us": table[bkey]}
+            "stage_variant": var, "accum_occ": occ, "us": table[bkey]}
     nb.set_launch(deliver_mode=mode, deliver_lanes=lanes, deliver_grid=grid, stage_segs=sg, stage_ept=ept,
-                  stage_variant=var)
+                  accum_occ=occ, stage_variant=var)
     if use_cache:
         cache[ckey] = best
         _save(cache)

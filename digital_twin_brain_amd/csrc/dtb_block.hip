@@ -689,8 +689,9 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     if (br[m] == 0xffffffffu) continue;
-                    const uint32_t j = pre[br[m] >> kRankBits] + (br[m] & ((1u << kRankBits) - 1));
-                    bpk[j] = pk_[m];
+                    const uint32_t bk = br[m] >> kRankBits;
+                    const uint32_t j = pre[bk] + (br[m] & ((1u << kRankBits) - 1));
+                    bpk[j] = pk_[m] | (bk << kMaxBucketBits); // the bucket rides in bits 15..30
                     bw0[j] = __float_as_uint(w0[m]);
                     if (PAIR) bw1[j] = __float_as_uint(w1[m]);
                 }
@@ -698,13 +699,9 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                 // consecutive lanes store consecutive events of a bucket
                 const uint32_t cnt = min(E - r0, (uint32_t)(256 * EPT));
                 for (uint32_t j = t; j < cnt; j += 256) {
-                    int32_t lo = 0, hi = s.nbk - 1; // last bucket with pre[b] <= j
-                    while (lo < hi) {
-                        const int32_t mid = (lo + hi + 1) >> 1;
-                        if (pre[mid] <= j) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    stage_put<PAIR>(s, part, (uint32_t)lo, (uint64_t)base[lo] + (j - pre[lo]), bpk[j],
+                    const uint32_t v = bpk[j];
+                    const uint32_t bk = (v >> kMaxBucketBits) & 0xffffu;
+                    stage_put<PAIR>(s, part, bk, (uint64_t)base[bk] + (j - pre[bk]), v & 0x80007fffu,
                                     __uint_as_float(bw0[j]), PAIR ? __uint_as_float(bw1[j]) : 0.0f);
                 }
                 __syncthreads();
@@ -1401,48 +1398,84 @@ static int alloc_ring_and_records(Block* b)
 }
 
 // Staging buffers of the bucketed delivery, allocated on first use.
+// Bucket geometry for `occ` accumulation workgroups per CU: the largest LDS
+// slice the device allows (at most 160 KB) divided by occ, then as few full
+// waves of occ buckets per CU as that permits.  The bucket of a target is
+// (post * bmagic) >> bshift, an exact division for post < 2^26.
+struct BucketGeom {
+    int32_t bsize, bshift, nbk;
+    uint64_t bmagic;
+    int64_t cap;
+};
+
+static BucketGeom bucket_geom(const Block* b, int occ)
+{
+    const int64_t ncu = std::max(b->ncu, 1);
+    const int64_t bpn = b->pair ? 16 : 8; // LDS bytes per neuron
+    const int64_t max_bs = std::min<int64_t>((1 << kMaxBucketBits) - 64, b->lds_max / occ / bpn / 64 * 64);
+    const int64_t per_wave = ncu * occ * max_bs;
+    const int64_t waves = (b->n + per_wave - 1) / per_wave;
+    const int64_t per = (b->n + waves * ncu * occ - 1) / (waves * ncu * occ);
+    BucketGeom g;
+    g.bsize = (int32_t)std::min<int64_t>(max_bs, std::max<int64_t>(64, (per + 63) / 64 * 64));
+    g.bshift = 26;
+    while ((1ll << (g.bshift - 26)) < g.bsize) ++g.bshift;
+    g.bmagic = ((1ull << g.bshift) + (uint64_t)g.bsize - 1) / (uint64_t)g.bsize;
+    g.nbk = (int32_t)((b->n + g.bsize - 1) / g.bsize);
+    // room for 1% of all synapses per step (a mean rate of 100 Hz at 0.1 ms),
+    // at least 1024 events per bucket partition; fuller partitions fall back to atomics
+    g.cap = std::max<int64_t>(1024, (b->nnz / 100 + (int64_t)g.nbk * kStageParts - 1) / ((int64_t)g.nbk * kStageParts));
+    return g;
+}
+
+static int use_bucket_geom(Block* b, int occ)
+{
+    const BucketGeom g = bucket_geom(b, occ);
+    DTB_REQUIRE(g.nbk <= kMaxBucketsLds, "too many delivery buckets");
+    b->acc_occ = occ;
+    b->bsize = g.bsize;
+    b->bshift = g.bshift;
+    b->bmagic = g.bmagic;
+    b->nbk = g.nbk;
+    b->stage_cap = g.cap;
+    // the sorted staging variant needs its round in LDS next to ~12 KB of static arrays
+    b->sorted_ok = stage_lds(b, 16, 2) + 16384 <= (size_t)b->lds_max;
+    return DTB_OK;
+}
+
+// Staging buffers of the bucketed delivery, allocated on first use for the
+// largest geometry (1 or 2 accumulation workgroups per CU) so that the
+// launch option can switch between them.
 static int ensure_bucketed(Block* b)
 {
     if (b->stage) return DTB_OK;
     DTB_REQUIRE(b->n <= (1ll << 26), "bucketed delivery supports n <= 2^26");
     DTB_REQUIRE(b->max_row < (1ll << 24), "bucketed delivery supports rows of fewer than 2^24 synapses");
-    // bucket size: the largest LDS slice the device allows (at most 160 KB),
-    // then as few full waves of one bucket per CU as that permits
-    int ncu = 0, lds = 0;
-    DTB_TRY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+    int lds = 0;
+    DTB_TRY_HIP(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device));
     DTB_TRY_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
-    lds = std::min(std::max(lds, 131072), 163840);
-    const int64_t bpn = b->pair ? 16 : 8; // LDS bytes per neuron
-    const int64_t max_bs = std::min<int64_t>((1 << kMaxBucketBits) - 64, lds / bpn / 64 * 64);
-    const int64_t per_wave = (int64_t)std::max(ncu, 1) * max_bs;
-    const int64_t waves = (b->n + per_wave - 1) / per_wave;
-    const int64_t per = (b->n + waves * std::max(ncu, 1) - 1) / (waves * std::max(ncu, 1));
-    b->bsize = (int32_t)std::min<int64_t>(max_bs, std::max<int64_t>(64, (per + 63) / 64 * 64));
-    b->bshift = 26;
-    while ((1ll << (b->bshift - 26)) < b->bsize) ++b->bshift;
-    b->bmagic = ((1ull << b->bshift) + (uint64_t)b->bsize - 1) / (uint64_t)b->bsize;
-    b->nbk = (int32_t)((b->n + b->bsize - 1) / b->bsize);
-    DTB_REQUIRE(b->nbk <= kMaxBucketsLds, "too many delivery buckets");
+    b->lds_max = std::min(std::max(lds, 131072), 163840);
     // the attribute is per kernel and process-wide: allow the largest slice any block may use
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-    // the sorted staging variant needs its round in LDS next to ~12 KB of static arrays
-    b->sorted_ok = stage_lds(b, 16, 2) + 16384 <= (size_t)lds;
-    DTB_TRY(allow_sorted_stage_lds<true>(lds - 16384));
-    DTB_TRY(allow_sorted_stage_lds<false>(lds - 16384));
-    // room for 1% of all synapses per step (a mean rate of 100 Hz at 0.1 ms),
-    // at least 1024 events per bucket partition; fuller partitions fall back to atomics
-    b->stage_cap = std::max<int64_t>(1024, (b->nnz / 100 + (int64_t)b->nbk * kStageParts - 1) /
-                                               ((int64_t)b->nbk * kStageParts));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+    DTB_TRY(allow_sorted_stage_lds<true>(b->lds_max - 16384));
+    DTB_TRY(allow_sorted_stage_lds<false>(b->lds_max - 16384));
+    int64_t nbk_max = 0, events_max = 0;
+    for (int occ = 1; occ <= 2; ++occ) {
+        const BucketGeom g = bucket_geom(b, occ);
+        nbk_max = std::max<int64_t>(nbk_max, g.nbk);
+        events_max = std::max<int64_t>(events_max, (int64_t)g.nbk * kStageParts * g.cap);
+    }
+    DTB_TRY(use_bucket_geom(b, 1));
     const size_t esz = b->pair ? 16 : 8;
-    DTB_TRY(alloc_n(b, &b->cursor, (int64_t)b->nbk * kStageParts * kCursorStride));
-    DTB_TRY(alloc_n(b, &b->ovf, (int64_t)b->nbk * kCursorStride));
-    DTB_TRY(block_alloc(b, &b->stage, esz * (size_t)b->nbk * kStageParts * (size_t)b->stage_cap));
+    DTB_TRY(alloc_n(b, &b->cursor, nbk_max * kStageParts * kCursorStride));
+    DTB_TRY(alloc_n(b, &b->ovf, nbk_max * kCursorStride));
+    DTB_TRY(block_alloc(b, &b->stage, esz * (size_t)events_max));
     DTB_TRY(block_alloc(b, &b->acc2, (b->pair ? 16 : 8) * (size_t)b->n));
-    DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)b->nbk * kStageParts * kCursorStride, b->stream));
-    DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)b->nbk * kCursorStride, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)nbk_max * kStageParts * kCursorStride, b->stream));
+    DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)nbk_max * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->acc2, 0, (b->pair ? 16 : 8) * (size_t)b->n, b->stream));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     return DTB_OK;
@@ -2428,6 +2461,12 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
     } else if (what == DTB_LAUNCH_STAGE_EPT) {
         DTB_REQUIRE(value == 16 || value == 32, "events per lane per round must be 16 or 32");
         b->stage_ept = value;
+    } else if (what == DTB_LAUNCH_ACCUM_OCC) {
+        DTB_REQUIRE(value == 1 || value == 2, "accumulation workgroups per CU must be 1 or 2");
+        DTB_TRY(ensure_bucketed(b));
+        DTB_TRY_HIP(hipStreamSynchronize(b->stream)); // cursors are all zero between steps
+        DTB_TRY(use_bucket_geom(b, value));
+        if (b->stage_variant == 2 && !b->sorted_ok) b->stage_variant = 0;
     } else if (what == DTB_LAUNCH_STAGE_VARIANT) {
         DTB_REQUIRE(value >= 0 && value <= 2, "staging variant must be 0, 1 or 2");
         if (value == 2) {
@@ -2458,6 +2497,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     else if (what == DTB_LAUNCH_STAGE_SEGS) *value = b->stage_segs;
     else if (what == DTB_LAUNCH_STAGE_EPT) *value = b->stage_ept;
     else if (what == DTB_LAUNCH_STAGE_VARIANT) *value = b->stage_variant;
+    else if (what == DTB_LAUNCH_ACCUM_OCC) *value = b->acc_occ;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

@@ -103,6 +103,9 @@ struct Block {
     int32_t stage_ept = 32;     // bucketed: events per lane per staging round
     int32_t stage_variant = 0;  // bucketed: 0 scattered stores, 1 two passes per chunk, 2 LDS-sorted rounds
     bool sorted_ok = false;     // the sorted variant's LDS fits
+    int32_t acc_occ = 1;        // bucketed: accumulation workgroups per CU (bucket size follows)
+    int32_t ncu = 0;            // compute units of the device
+    int32_t lds_max = 0;        // LDS bytes a workgroup may use
     ExtDrive ext{};             // external Poisson drive (n = 0: off)
     int32_t bsize = 0;          // bucketed: neurons per destination bucket
     uint64_t bmagic = 0;        // bucket = (post * bmagic) >> bshift

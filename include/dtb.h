@@ -246,6 +246,8 @@ int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* t
 #define DTB_LAUNCH_STAGE_VARIANT 6 /* bucketed delivery staging: 0 a reservation per round, lanes store their own
                                       events; 1 a reservation per chunk (two passes); 2 rounds sorted by
                                       bucket in LDS, stored as contiguous runs (16 events per lane) */
+#define DTB_LAUNCH_ACCUM_OCC 7     /* bucketed delivery: 1 or 2 accumulation workgroups per CU (the
+                                      destination bucket is the LDS slice divided by this) */
 int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
 int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 

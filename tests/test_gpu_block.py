@@ -392,17 +392,19 @@ def test_results_independent_of_delivery_launch_shape():
     ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, False, seed=8)
     ref["w0"] = (np.floor(ref["w0"] * 256) / 256).astype(np.float32)
     outs = []
-    for mode, lanes, grid, sg, ept, ps in ((0, 64, 4, 256, 32, 0), (0, 32, 37, 256, 32, 0), (0, 16, 300, 256, 32, 0),
-                                           (0, 64, 2048, 256, 32, 0), (1, 64, 1, 256, 32, 0), (1, 64, 300, 256, 16, 0),
-                                           (1, 64, 7, 64, 16, 1), (1, 64, 1000, 128, 32, 1), (1, 64, 3, 256, 32, 1),
-                                           (1, 64, 5, 128, 16, 2), (1, 64, 900, 64, 16, 2)):
+    for mode, lanes, grid, sg, ept, ps, occ in ((0, 64, 4, 256, 32, 0, 1), (0, 32, 37, 256, 32, 0, 1),
+                                                (0, 16, 300, 256, 32, 0, 1), (0, 64, 2048, 256, 32, 0, 1),
+                                                (1, 64, 1, 256, 32, 0, 1), (1, 64, 300, 256, 16, 0, 2),
+                                                (1, 64, 7, 64, 16, 1, 1), (1, 64, 1000, 128, 32, 1, 2),
+                                                (1, 64, 3, 256, 32, 1, 1), (1, 64, 5, 128, 16, 2, 2),
+                                                (1, 64, 900, 64, 16, 2, 1)):
         nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"],
                                   dt=0.1, noise_seed=2)
         nb.set_launch(deliver_lanes=lanes, deliver_grid=grid, deliver_mode=mode, neuron_psets=mode, stage_segs=sg,
-                      stage_ept=ept, stage_variant=ps)
+                      stage_ept=ept, accum_occ=occ, stage_variant=ps)
         assert nb.get_launch() == {"deliver_grid": grid, "deliver_lanes": lanes, "deliver_mode": mode,
                                    "neuron_psets": mode, "stage_segs": sg, "stage_ept": ept,
-                                   "stage_variant": ps}  # one shared parameter set when enabled
+                                   "stage_variant": ps, "accum_occ": occ}  # one shared parameter set when enabled
         ev = np.zeros(300, np.uint64)
         nb.run(300, noise_rate=0.02, graph=True, events=ev)
         outs.append((nb.get("V"), nb.get("J"), ev))
@@ -427,10 +429,10 @@ def test_bucketed_delivery_matches_atomics_including_overflow(pair):
         if ref.get(key) is not None:
             ref[key] = (np.floor(ref[key] * 64) / 64).astype(np.float32)
     outs = []
-    for mode, var in ((0, 0), (1, 0), (1, 1), (1, 2)):
+    for mode, var, occ in ((0, 0, 1), (1, 0, 1), (1, 1, 2), (1, 2, 1), (1, 2, 2)):
         nb = NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], ref["w1"] if pair else None,
                                   ref["delay"], dt=0.1, noise_seed=4)
-        nb.set_launch(deliver_mode=mode, deliver_grid=512, stage_variant=var)
+        nb.set_launch(deliver_mode=mode, deliver_grid=512, accum_occ=occ, stage_variant=var)
         ev = np.zeros(40, np.uint64)
         # 20 steps of heavy noise (up to ~10^7 events per step: buckets overflow), then 20 light ones
         nb.run(20, noise_rate=0.25, graph=True, events=ev[:20])
