@@ -104,10 +104,13 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
     t_build = time.perf_counter() - t0
     steps = int(round(window_ms / dt))
     truth_vox = slice(E * voxels, (E + 1) * voxels)
+    say = log or (lambda *a: None)
     da.evolve(steps, bold_detail=True)
+    say(f"surrogate DA: block of {nb.n} neurons built and warmed in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     X, Y = collect_windows(da, train_windows, steps, rng=rng)
     t_collect = time.perf_counter() - t0
+    say(f"surrogate DA: {train_windows} training windows ({X.shape[0]} samples) in {t_collect:.1f} s")
     ws = WindowSurrogate(voxels, hidden=hidden, max_batch=min(256, max(32, E)), seed=seed)
     ws.add(X, Y)
     torch.cuda.synchronize()
@@ -115,6 +118,7 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
     hist = ws.fit(epochs=epochs)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
+    say(f"surrogate DA: fitted in {t_fit:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
 
     def reset_hp():
         hp = da.hp_random_initialize(low, high, rng=np.random.default_rng(seed + 1))
@@ -131,9 +135,12 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
 
     w_sim, _, t_sim = da.da_rest_run(observe, "/tmp", observation_times=windows, steps=steps, save=False)
     obs = np.stack(obs)
+    say(f"surrogate DA: {windows} simulator-only windows, {t_sim.mean():.3f} s each")
     reset_hp()
     sda = SurrogateDA(da, ws, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs)
     res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
+    say(f"surrogate DA: {windows} surrogate-assisted windows, {res['times'].mean():.3f} s each, "
+        f"{int(res['simulated'].sum())} simulated")
     rmse = lambda w: np.sqrt(((w[:, :, :, -1].mean(1) - obs) ** 2).mean(1))
     out = dict(members=E, voxels=voxels, windows=windows, steps_per_window=steps, build_s=t_build,
                neurons=int(nb.n), synapses=int(nb.nnz), surrogate="MLP " + "x".join(str(h) for h in hidden),
