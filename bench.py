@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--da-windows", type=int, default=20)
-    ap.add_argument("--train-windows", type=int, default=16, help="surrogate training windows (x64 samples)")
+    ap.add_argument("--train-windows", type=int, default=24, help="surrogate training windows (x64 samples)")
     ap.add_argument("--no-graph", action="store_true", help="launch step batches directly (for counter runs)")
     ap.add_argument("--launch", default="", metavar="MODE,GRID,SEGS[,EPT[,VARIANT]]",
                     help="fixed delivery launch shape instead of the autotuner (e.g. 1,1024,128,32,0)")

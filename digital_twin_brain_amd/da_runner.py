@@ -82,7 +82,7 @@ def run_ensemble_da(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=
 
 
 def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
-           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=200, finetune_epochs=20, log=None, tune=True,
+           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=400, finetune_epochs=20, log=None, tune=True,
            **member_kw):
     """Surrogate-assisted DA against simulator-only DA on the same ensemble.
 
@@ -108,7 +108,15 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
     da.evolve(steps, bold_detail=True)
     say(f"surrogate DA: block of {nb.n} neurons built and warmed in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    X, Y = collect_windows(da, train_windows, steps, rng=rng)
+    # training windows: random walks of the logits from the assimilation's own starting spread
+    # (twice its per-window step), restarted every 8 windows
+    Xs, Ys = [], []
+    for w0 in range(0, train_windows, 8):
+        da.hp_random_initialize(low, high, rng=np.random.default_rng(seed + 100 + w0))
+        Xw, Yw = collect_windows(da, min(8, train_windows - w0), steps, rng=rng, walk_sigma=2 * 0.25 ** 0.5)
+        Xs.append(Xw)
+        Ys.append(Yw)
+    X, Y = torch.cat(Xs), torch.cat(Ys)
     t_collect = time.perf_counter() - t0
     say(f"surrogate DA: {train_windows} training windows ({X.shape[0]} samples) in {t_collect:.1f} s")
     ws = WindowSurrogate(voxels, hidden=hidden, max_batch=min(256, max(32, E)), seed=seed)

@@ -39,6 +39,20 @@ def window_inputs(da):
     return torch.cat((hp, st), dim=2).reshape(E, -1)
 
 
+# default_params.py:11-12 (the BOLD output constants)
+_V0, _E0 = 0.02, 0.8
+
+
+def window_base(X, V):
+    """What a window would end at if nothing changed: per voxel (act 0, the
+    start s, q, v, f_in, and the BOLD signal of the start q, v as
+    models/bold_model_pytorch.py:76-77 computes it).  X: [E, V*8]."""
+    x = torch.as_tensor(X, dtype=torch.float64).reshape(X.shape[0], V, 8)
+    s, q, v, f = x[..., 4], x[..., 5], x[..., 6], x[..., 7]
+    bold = _V0 * (7 * _E0 * (1 - q) + 2 * (1 - q / v) + (2 * _E0 - 0.2) * (1 - v))
+    return torch.stack((torch.zeros_like(s), s, q, v, f, bold), dim=-1).reshape(X.shape[0], V * 6)
+
+
 def nrmse(sur, sim):
     sur = np.asarray(sur, np.float64)
     sim = np.asarray(sim, np.float64)
@@ -47,7 +61,9 @@ def nrmse(sur, sim):
 
 
 class WindowSurrogate:
-    """MLP from window_inputs -> hidden-state tail [E, V*6]."""
+    """MLP from window_inputs -> hidden-state tail [E, V*6], trained on the
+    change over the window (tail - window_base) so that the slow hemodynamic
+    state needs no absolute precision from the network."""
 
     def __init__(self, voxels, hp_per_voxel=4, hidden=(512, 512, 512), max_batch=256, lr=1e-3, seed=0, device=0):
         self.V = int(voxels)
@@ -59,8 +75,10 @@ class WindowSurrogate:
         self.Y = []
 
     def add(self, X, Y):
-        self.X.append(torch.as_tensor(X, dtype=torch.float64).cpu())
-        self.Y.append(torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], -1).cpu())
+        X = torch.as_tensor(X, dtype=torch.float64).cpu()
+        Y = torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], -1).cpu()
+        self.X.append(X)
+        self.Y.append(Y - window_base(X, self.V))
 
     def fit(self, epochs=200, batch=None, renormalize=True):
         X = torch.cat(self.X)
@@ -70,7 +88,9 @@ class WindowSurrogate:
         return self.mlp.fit(X, Y, epochs=epochs, batch=batch)
 
     def predict(self, X):
-        return self.mlp.predict(torch.as_tensor(X)).double().reshape(X.shape[0], self.V, 6)
+        X = torch.as_tensor(X, dtype=torch.float64)
+        d = self.mlp.predict(X).double().cpu()
+        return (d + window_base(X, self.V)).reshape(X.shape[0], self.V, 6).to(self.mlp.dev)
 
 
 class SurrogateDA:
@@ -121,16 +141,21 @@ class SurrogateDA:
         return dict(forecasts=np.stack(forecasts), simulated=np.array(flags), nrmse=errs, times=np.array(times))
 
 
-def collect_windows(da, windows, steps, logit_range=10.0, rng=None):
-    """Simulator windows without analysis, each with fresh hyper-parameter
-    logits drawn uniformly from [-logit_range, logit_range] (independent
-    coverage of the range the assimilation explores); returns
-    (X [windows*E, 8V], Y [windows*E, 6V]) for training."""
+def collect_windows(da, windows, steps, logit_range=10.0, rng=None, walk_sigma=None):
+    """Simulator windows without analysis; returns (X [windows*E, 8V],
+    Y [windows*E, 6V]) for training.  Hyper-parameter logits are either
+    fresh uniform draws from [-logit_range, logit_range] each window
+    (walk_sigma None), or a Gaussian random walk of step walk_sigma from the
+    current logits, the way the assimilation moves them."""
     rng = np.random.default_rng(0) if rng is None else rng
     Xs, Ys = [], []
     E = da.ensemble_number
     for _ in range(windows):
-        da._hp_log = torch.from_numpy(rng.uniform(-logit_range, logit_range, size=tuple(da._hp_log.shape))).float()
+        if walk_sigma is None:
+            hl = rng.uniform(-logit_range, logit_range, size=tuple(da._hp_log.shape))
+        else:
+            hl = da._hp_log.numpy() + rng.normal(0.0, walk_sigma, size=tuple(da._hp_log.shape))
+        da._hp_log = torch.from_numpy(hl).float()
         da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
         X = window_inputs(da)
         da.mul_property_by_subblk(da._hp_index_updating, da._hp.float().reshape(-1))
