@@ -82,7 +82,7 @@ def run_ensemble_da(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=
 
 
 def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
-           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=400, finetune_epochs=20, log=None, tune=True,
+           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=800, finetune_epochs=60, log=None, tune=True,
            **member_kw):
     """Surrogate-assisted DA against simulator-only DA on the same ensemble.
 

@@ -26,9 +26,13 @@ import torch
 from .surrogate import MLPSurrogate
 
 
+NF = 9  # input features per voxel
+
+
 def window_inputs(da):
-    """[E, 8V]: per voxel the 4 hyper-parameter logits of its hidden state and
-    the BOLD state (s, q, v, f_in) at the window start."""
+    """[E, 9V]: per voxel the 4 hyper-parameter logits of its hidden state,
+    the BOLD state (s, q, v, f_in) at the window start and the firing rate
+    (act) of the previous window (network activity carries over)."""
     E = da.ensemble_number
     Vb = da._num_voxel_pblk
     idx = da._index_da_voxel_pblk.numpy()
@@ -36,7 +40,11 @@ def window_inputs(da):
     nv = da._da_voxels
     st = np.stack([da.block_model.nb.bold_state(w)[:nv] for w in ("s", "q", "v", "f_in")], axis=1)
     st = torch.from_numpy(st.reshape(E, Vb, 4)[:, idx])
-    return torch.cat((hp, st), dim=2).reshape(E, -1)
+    if da._hidden_state is not None:
+        act = da._hidden_state[:, :, -6:-5].double().cpu()
+    else:
+        act = torch.zeros((E, len(idx), 1), dtype=torch.float64)
+    return torch.cat((hp, st, act), dim=2).reshape(E, -1)
 
 
 # default_params.py:11-12 (the BOLD output constants)
@@ -44,13 +52,13 @@ _V0, _E0 = 0.02, 0.8
 
 
 def window_base(X, V):
-    """What a window would end at if nothing changed: per voxel (act 0, the
-    start s, q, v, f_in, and the BOLD signal of the start q, v as
-    models/bold_model_pytorch.py:76-77 computes it).  X: [E, V*8]."""
-    x = torch.as_tensor(X, dtype=torch.float64).reshape(X.shape[0], V, 8)
-    s, q, v, f = x[..., 4], x[..., 5], x[..., 6], x[..., 7]
+    """What a window would end at if nothing changed: per voxel (the previous
+    act, the start s, q, v, f_in, and the BOLD signal of the start q, v as
+    models/bold_model_pytorch.py:76-77 computes it).  X: [E, V*NF]."""
+    x = torch.as_tensor(X, dtype=torch.float64).reshape(X.shape[0], V, NF)
+    s, q, v, f, act = x[..., 4], x[..., 5], x[..., 6], x[..., 7], x[..., 8]
     bold = _V0 * (7 * _E0 * (1 - q) + 2 * (1 - q / v) + (2 * _E0 - 0.2) * (1 - v))
-    return torch.stack((torch.zeros_like(s), s, q, v, f, bold), dim=-1).reshape(X.shape[0], V * 6)
+    return torch.stack((act, s, q, v, f, bold), dim=-1).reshape(X.shape[0], V * 6)
 
 
 def nrmse(sur, sim):
@@ -67,7 +75,7 @@ class WindowSurrogate:
 
     def __init__(self, voxels, hp_per_voxel=4, hidden=(512, 512, 512), max_batch=256, lr=1e-3, seed=0, device=0):
         self.V = int(voxels)
-        self.n_in = self.V * (hp_per_voxel + 4)
+        self.n_in = self.V * (hp_per_voxel + 5)
         self.n_out = self.V * 6
         self.mlp = MLPSurrogate(self.n_in, self.n_out, hidden=hidden, max_batch=max_batch, lr=lr, seed=seed,
                                 device=device)
@@ -94,7 +102,7 @@ class WindowSurrogate:
 
 
 class SurrogateDA:
-    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=20):
+    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=60):
         self.da = da
         self.sur = surrogate
         self.every = int(every)
