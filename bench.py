@@ -52,13 +52,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(kernel_prefix, workload):
-    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+def pmc_traffic(kernel_prefix, workload, events_per_launch):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary (a fixed
+    part plus bytes per synaptic event, scaled to this run's events), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel_prefix}_{workload}_pmc.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
-    return float(m.group(1)) if m else None
+    txt = open(path).read()
+    fixed = re.search(r"traffic_fixed_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    per = re.search(r"traffic_bytes_per_event:\s*([0-9.eE+]+)", txt)
+    if not (fixed and per):
+        return None
+    return float(fixed.group(1)) + float(per.group(1)) * events_per_launch
 
 
 def cpu_baseline(n_sample, in_degree, steps_warm, steps_timed):
@@ -234,7 +239,7 @@ def main():
         log(f"cpu baseline {cpu['value']:.3e} events/s ({time.perf_counter() - t0:.1f} s)")
 
     value = tot_ev / wall
-    traffic = pmc_traffic(prof, a.workload)
+    traffic = pmc_traffic(prof, a.workload, float(ev_w.mean())) if prof == "delivery" else None
     out = {
         "metric": "synaptic events/s", "value": value, "unit": "events/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
