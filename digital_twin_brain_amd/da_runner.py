@@ -82,7 +82,8 @@ def run_ensemble_da(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=
 
 
 def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
-           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=800, finetune_epochs=60, log=None, tune=True,
+           every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=None, finetune_epochs=None, per_voxel=True,
+           log=None, tune=True,
            **member_kw):
     """Surrogate-assisted DA against simulator-only DA on the same ensemble.
 
@@ -119,7 +120,7 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
     X, Y = torch.cat(Xs), torch.cat(Ys)
     t_collect = time.perf_counter() - t0
     say(f"surrogate DA: {train_windows} training windows ({X.shape[0]} samples) in {t_collect:.1f} s")
-    ws = WindowSurrogate(voxels, hidden=hidden, max_batch=min(256, max(32, E)), seed=seed)
+    ws = WindowSurrogate(voxels, hidden=hidden, seed=seed, per_voxel=per_voxel)
     ws.add(X, Y)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -151,7 +152,8 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
         f"{int(res['simulated'].sum())} simulated")
     rmse = lambda w: np.sqrt(((w[:, :, :, -1].mean(1) - obs) ** 2).mean(1))
     out = dict(members=E, voxels=voxels, windows=windows, steps_per_window=steps, build_s=t_build,
-               neurons=int(nb.n), synapses=int(nb.nnz), surrogate="MLP " + "x".join(str(h) for h in hidden),
+               neurons=int(nb.n), synapses=int(nb.nnz),
+               surrogate=("per-voxel " if per_voxel else "per-member ") + "MLP " + "x".join(str(h) for h in hidden),
                train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect, fit_s=t_fit,
                fit_loss_first=hist[0], fit_loss_last=hist[-1], sim_window_s_mean=float(t_sim.mean()),
                sur_window_s_mean=float(res["times"].mean()),

@@ -69,44 +69,63 @@ def nrmse(sur, sim):
 
 
 class WindowSurrogate:
-    """MLP from window_inputs -> hidden-state tail [E, V*6], trained on the
-    change over the window (tail - window_base) so that the slow hemodynamic
-    state needs no absolute precision from the network."""
+    """Window map window_inputs -> hidden-state tail [E, V, 6], learned as the
+    change over the window (tail - window_base), so that the slow
+    hemodynamic state needs no absolute precision from the network.
 
-    def __init__(self, voxels, hp_per_voxel=4, hidden=(512, 512, 512), max_batch=256, lr=1e-3, seed=0, device=0):
+    per_voxel=True: one network shared by all voxels; a row is a voxel's own
+    NF features and its member's mean of them (voxels of a member are
+    exchangeable: every neuron of a member draws its inputs from the whole
+    member), and each window gives E*V rows.  per_voxel=False: one row per
+    member, all V voxels at once (V*NF inputs, V*6 outputs)."""
+
+    def __init__(self, voxels, hidden=(512, 512, 512), max_batch=None, lr=1e-3, seed=0, device=0, per_voxel=True):
         self.V = int(voxels)
-        self.n_in = self.V * (hp_per_voxel + 5)
-        self.n_out = self.V * 6
-        self.mlp = MLPSurrogate(self.n_in, self.n_out, hidden=hidden, max_batch=max_batch, lr=lr, seed=seed,
-                                device=device)
+        self.per_voxel = bool(per_voxel)
+        n_in, n_out = (2 * NF, 6) if self.per_voxel else (self.V * NF, self.V * 6)
+        mb = max_batch or (1024 if self.per_voxel else 256)
+        self.mlp = MLPSurrogate(n_in, n_out, hidden=hidden, max_batch=mb, lr=lr, seed=seed, device=device)
         self.X = []
         self.Y = []
+
+    def _rows(self, X):
+        X = torch.as_tensor(X, dtype=torch.float64).cpu()
+        if not self.per_voxel:
+            return X
+        x = X.reshape(X.shape[0], self.V, NF)
+        m = x.mean(dim=1, keepdim=True).expand(-1, self.V, -1)
+        return torch.cat((x, m), dim=2).reshape(-1, 2 * NF)
 
     def add(self, X, Y):
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
         Y = torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], -1).cpu()
-        self.X.append(X)
-        self.Y.append(Y - window_base(X, self.V))
+        d = Y - window_base(X, self.V)
+        self.X.append(self._rows(X))
+        self.Y.append(d.reshape(-1, 6) if self.per_voxel else d)
 
-    def fit(self, epochs=200, batch=None, renormalize=True):
+    def fit(self, epochs=None, batch=None, renormalize=True):
         X = torch.cat(self.X)
         Y = torch.cat(self.Y)
         if renormalize:
             self.mlp.set_normalization(X, Y)
+        if epochs is None:
+            epochs = 20 if self.per_voxel else 400
         return self.mlp.fit(X, Y, epochs=epochs, batch=batch)
 
     def predict(self, X):
-        X = torch.as_tensor(X, dtype=torch.float64)
-        d = self.mlp.predict(X).double().cpu()
+        X = torch.as_tensor(X, dtype=torch.float64).cpu()
+        d = self.mlp.predict(self._rows(X)).double().cpu().reshape(X.shape[0], -1)
         return (d + window_base(X, self.V)).reshape(X.shape[0], self.V, 6).to(self.mlp.dev)
 
 
 class SurrogateDA:
-    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=60):
+    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=None):
         self.da = da
         self.sur = surrogate
         self.every = int(every)
         self.nrmse_max = float(nrmse_max)
+        if finetune_epochs is None:
+            finetune_epochs = 2 if getattr(surrogate, "per_voxel", False) else 40
         self.finetune_epochs = int(finetune_epochs)
 
     def _advance_hp(self, hp_sigma):
