@@ -477,6 +477,8 @@ struct StageArgs {
     void* stage;      // [nbk][kStageParts][cap] uint2 (compact) / uint4 (pair)
     void* acc2;       // overflow accumulator [n]
     uint32_t* ovf;    // [nbk] overflow flags
+    float fscale;     // fixed-point accumulation: weight * 2^S as int32
+    double inv_scale; // 2^-S
 };
 
 
@@ -792,6 +794,104 @@ static void launch_stage(const Block* b, const StageArgs& sa, int32_t k)
     }
 }
 
+template <bool PAIR, int U, bool FIXED>
+__global__ void deliver_accum_kernel(StageArgs s);
+
+template <bool PAIR, bool FIXED>
+static void launch_accum_f(const Block* b, const StageArgs& sa)
+{
+    const size_t lds = (size_t)b->bsize * (PAIR ? 16 : 8);
+    if (b->accum_unroll == 16)
+        hipLaunchKernelGGL((deliver_accum_kernel<PAIR, 16, FIXED>), dim3(b->nbk), dim3(1024), lds, b->stream, sa);
+    else
+        hipLaunchKernelGGL((deliver_accum_kernel<PAIR, 8, FIXED>), dim3(b->nbk), dim3(1024), lds, b->stream, sa);
+}
+
+template <bool PAIR>
+static void launch_accum(const Block* b, const StageArgs& sa)
+{
+    if (b->accum_fixed) launch_accum_f<PAIR, true>(b, sa);
+    else launch_accum_f<PAIR, false>(b, sa);
+}
+
+// Per-step sum bound: for every (post, channel), the sum of |w| over its
+// incoming synapses (each synapse delivers at most once per step).
+template <bool PAIR>
+__global__ void insum_kernel(int64_t nnz, const uint32_t* __restrict__ col, const float* __restrict__ w,
+                             const float2* __restrict__ w2, uint32_t idx_mask, float* __restrict__ sums)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) {
+        const uint32_t c = col[e];
+        const int64_t post = c & idx_mask;
+        const uint32_t kind = c >> 31;
+        if (PAIR) {
+            const float2 v = w2[e];
+            unsafeAtomicAdd(&sums[post * 4 + kind * 2], fabsf(v.x));
+            unsafeAtomicAdd(&sums[post * 4 + kind * 2 + 1], fabsf(v.y));
+        } else {
+            unsafeAtomicAdd(&sums[post * 2 + kind], fabsf(w[e]));
+        }
+    }
+}
+
+__global__ void max_nonneg_kernel(int64_t n, const float* __restrict__ v, uint32_t* __restrict__ out)
+{
+    uint32_t m = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        m = max(m, __float_as_uint(v[i])); // non-negative floats order like their bits
+    m = max(m, (uint32_t)__shfl_xor(m, 1));
+    m = max(m, (uint32_t)__shfl_xor(m, 2));
+    m = max(m, (uint32_t)__shfl_xor(m, 4));
+    m = max(m, (uint32_t)__shfl_xor(m, 8));
+    m = max(m, (uint32_t)__shfl_xor(m, 16));
+    m = max(m, (uint32_t)__shfl_xor(m, 32));
+    if (__lane_id() == 0) atomicMax(out, m);
+}
+
+// Fixed-point accumulation scale: S such that the largest possible per-step
+// sum times 2^S stays below 2^30.  Usable when S >= 16.
+static int fixed_scale(Block* b)
+{
+    const int C = b->pair ? 4 : 2;
+    float* sums = nullptr;
+    uint32_t* mx = nullptr;
+    float bound = 0.0f;
+    if (b->nnz > 0) {
+        DTB_TRY_HIP(hipMalloc(&sums, sizeof(float) * (size_t)b->n * C));
+        hipError_t e = hipMalloc(&mx, sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemsetAsync(sums, 0, sizeof(float) * (size_t)b->n * C, b->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(mx, 0, sizeof(uint32_t), b->stream);
+        if (e == hipSuccess) {
+            const unsigned g = (unsigned)std::min<int64_t>(65536, (b->nnz + 255) / 256);
+            if (b->pair)
+                hipLaunchKernelGGL(insum_kernel<true>, dim3(g), dim3(256), 0, b->stream, b->nnz, b->col, b->w, b->w2,
+                                   b->pk.idx_mask, sums);
+            else
+                hipLaunchKernelGGL(insum_kernel<false>, dim3(g), dim3(256), 0, b->stream, b->nnz, b->col, b->w, b->w2,
+                                   b->pk.idx_mask, sums);
+            const unsigned g2 = (unsigned)std::min<int64_t>(4096, (b->n * C + 255) / 256);
+            hipLaunchKernelGGL(max_nonneg_kernel, dim3(g2), dim3(256), 0, b->stream, b->n * C, sums, mx);
+            uint32_t bits = 0;
+            e = hipMemcpyAsync(&bits, mx, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+            memcpy(&bound, &bits, sizeof(float));
+        }
+        hipFree(sums);
+        if (mx) hipFree(mx);
+        DTB_TRY_HIP(e);
+    }
+    // margin for the float rounding of the bound itself
+    const double lim = (double)(1 << 30) / std::max(1e-30, (double)bound * 1.01 + 1e-30);
+    int S = (int)std::floor(std::log2(lim));
+    S = std::min(S, 60);
+    b->acc_shift = S;
+    b->fixed_ok = std::isfinite(bound) && S >= 16;
+    b->accum_fixed = b->fixed_ok;
+    return DTB_OK;
+}
+
 template <bool PAIR>
 static int allow_sorted_stage_lds(int bytes)
 {
@@ -804,7 +904,11 @@ static int allow_sorted_stage_lds(int bytes)
     return DTB_OK;
 }
 
-template <bool PAIR>
+// U staged events in flight per lane (8 or 16).  FIXED: the slice holds int32
+// sums of round(w * 2^S) (integer LDS atomics run ~10x faster than float ones
+// on gfx950; S is chosen per block so that no neuron's sum can overflow, and
+// integer sums do not depend on the order of the events); otherwise fp32.
+template <bool PAIR, int U, bool FIXED>
 __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 {
     using Cfg = BucketCfg<PAIR>;
@@ -824,7 +928,6 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     const uint32_t cnt = pre[kStageParts];
     const bool ovf = s.ovf[(size_t)b * kCursorStride] != 0u;
     __syncthreads();
-    constexpr int U = 8; // staged events in flight per lane
     const int64_t sb = (int64_t)b * kStageParts * s.cap;
     auto at = [&](uint32_t e) -> int64_t {
         int p = 0;
@@ -844,8 +947,14 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
             for (int u = 0; u < U; ++u) {
                 if (base + (uint32_t)u * blockDim.x >= cnt) break;
                 const uint32_t o = (v[u].x & 0x7fffffffu) * 4 + ((v[u].x >> 31) << 1);
-                atomicAdd(&slice[o], __uint_as_float(v[u].y));
-                atomicAdd(&slice[o + 1], __uint_as_float(v[u].z));
+                if (FIXED) {
+                    int* si = reinterpret_cast<int*>(slice);
+                    atomicAdd(&si[o], __float2int_rn(__uint_as_float(v[u].y) * s.fscale));
+                    atomicAdd(&si[o + 1], __float2int_rn(__uint_as_float(v[u].z) * s.fscale));
+                } else {
+                    atomicAdd(&slice[o], __uint_as_float(v[u].y));
+                    atomicAdd(&slice[o + 1], __uint_as_float(v[u].z));
+                }
             }
         } else {
             uint2 v[U];
@@ -857,7 +966,11 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (base + (uint32_t)u * blockDim.x >= cnt) break;
-                atomicAdd(&slice[(v[u].x & 0x7fffffffu) * 2 + (v[u].x >> 31)], __uint_as_float(v[u].y));
+                const uint32_t o = (v[u].x & 0x7fffffffu) * 2 + (v[u].x >> 31);
+                if (FIXED)
+                    atomicAdd(&reinterpret_cast<int*>(slice)[o], __float2int_rn(__uint_as_float(v[u].y) * s.fscale));
+                else
+                    atomicAdd(&slice[o], __uint_as_float(v[u].y));
             }
         }
     }
@@ -868,7 +981,14 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
         float4* acc = reinterpret_cast<float4*>(a.acc) + i0;
         float4* acc2 = reinterpret_cast<float4*>(s.acc2) + i0;
         for (int x = t; x < nloc; x += blockDim.x) {
-            float4 v = make_float4(slice[4 * x], slice[4 * x + 1], slice[4 * x + 2], slice[4 * x + 3]);
+            float4 v;
+            if (FIXED) {
+                const int* si = reinterpret_cast<const int*>(slice) + 4 * x;
+                v = make_float4((float)((double)si[0] * s.inv_scale), (float)((double)si[1] * s.inv_scale),
+                                (float)((double)si[2] * s.inv_scale), (float)((double)si[3] * s.inv_scale));
+            } else {
+                v = make_float4(slice[4 * x], slice[4 * x + 1], slice[4 * x + 2], slice[4 * x + 3]);
+            }
             if (ovf) {
                 const float4 o = acc2[x];
                 v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
@@ -880,7 +1000,13 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
         float2* acc = reinterpret_cast<float2*>(a.acc) + i0;
         float2* acc2 = reinterpret_cast<float2*>(s.acc2) + i0;
         for (int x = t; x < nloc; x += blockDim.x) {
-            float2 v = make_float2(slice[2 * x], slice[2 * x + 1]);
+            float2 v;
+            if (FIXED) {
+                const int* si = reinterpret_cast<const int*>(slice) + 2 * x;
+                v = make_float2((float)((double)si[0] * s.inv_scale), (float)((double)si[1] * s.inv_scale));
+            } else {
+                v = make_float2(slice[2 * x], slice[2 * x + 1]);
+            }
             if (ovf) {
                 const float2 o = acc2[x];
                 v.x += o.x; v.y += o.y;
@@ -1456,9 +1582,21 @@ static int ensure_bucketed(Block* b)
     DTB_TRY_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
     b->lds_max = std::min(std::max(lds, 131072), 163840);
     // the attribute is per kernel and process-wide: allow the largest slice any block may use
-    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true>,
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 8, true>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
-    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false>,
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 8, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 16, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 16, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 8, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 8, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 16, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+    DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 16, false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
     DTB_TRY(allow_sorted_stage_lds<true>(b->lds_max - 16384));
     DTB_TRY(allow_sorted_stage_lds<false>(b->lds_max - 16384));
@@ -1477,6 +1615,7 @@ static int ensure_bucketed(Block* b)
     DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)nbk_max * kStageParts * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)nbk_max * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->acc2, 0, (b->pair ? 16 : 8) * (size_t)b->n, b->stream));
+    DTB_TRY(fixed_scale(b));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     return DTB_OK;
 }
@@ -1747,15 +1886,14 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         }
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
         if (b->deliver_mode == 1) {
-            StageArgs sa{da, b->nbk, b->bsize, b->bmagic, b->bshift, b->stage_cap, b->cursor, b->stage, b->acc2, b->ovf};
+            StageArgs sa{da,     b->nbk,  b->bsize, b->bmagic, b->bshift, b->stage_cap, b->cursor,
+                         b->stage, b->acc2, b->ovf, std::ldexp(1.0f, b->acc_shift), std::ldexp(1.0, -b->acc_shift)};
             if (b->pair) {
                 launch_stage<true>(b, sa, k);
-                hipLaunchKernelGGL(deliver_accum_kernel<true>, dim3(b->nbk), dim3(1024), (size_t)b->bsize * 16,
-                                   b->stream, sa);
+                launch_accum<true>(b, sa);
             } else {
                 launch_stage<false>(b, sa, k);
-                hipLaunchKernelGGL(deliver_accum_kernel<false>, dim3(b->nbk), dim3(1024), (size_t)b->bsize * 8,
-                                   b->stream, sa);
+                launch_accum<false>(b, sa);
             }
         } else if (b->pair) {
             launch_deliver<true>(b, da, k);
@@ -2461,6 +2599,15 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
     } else if (what == DTB_LAUNCH_STAGE_EPT) {
         DTB_REQUIRE(value == 16 || value == 32, "events per lane per round must be 16 or 32");
         b->stage_ept = value;
+    } else if (what == DTB_LAUNCH_ACCUM_FIXED) {
+        DTB_REQUIRE(value == 0 || value == 1, "fixed-point accumulation must be 0 or 1");
+        DTB_TRY(ensure_bucketed(b));
+        DTB_REQUIRE(value == 0 || b->fixed_ok,
+                    "fixed-point accumulation needs 2^S resolution with S >= 16 (largest per-step input sum < 16384)");
+        b->accum_fixed = value != 0;
+    } else if (what == DTB_LAUNCH_ACCUM_UNROLL) {
+        DTB_REQUIRE(value == 8 || value == 16, "accumulation unroll must be 8 or 16");
+        b->accum_unroll = value;
     } else if (what == DTB_LAUNCH_ACCUM_OCC) {
         DTB_REQUIRE(value == 1 || value == 2, "accumulation workgroups per CU must be 1 or 2");
         DTB_TRY(ensure_bucketed(b));
@@ -2498,6 +2645,8 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     else if (what == DTB_LAUNCH_STAGE_EPT) *value = b->stage_ept;
     else if (what == DTB_LAUNCH_STAGE_VARIANT) *value = b->stage_variant;
     else if (what == DTB_LAUNCH_ACCUM_OCC) *value = b->acc_occ;
+    else if (what == DTB_LAUNCH_ACCUM_UNROLL) *value = b->accum_unroll;
+    else if (what == DTB_LAUNCH_ACCUM_FIXED) *value = b->accum_fixed ? b->acc_shift : 0;
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

@@ -104,6 +104,10 @@ struct Block {
     int32_t stage_variant = 0;  // bucketed: 0 scattered stores, 1 two passes per chunk, 2 LDS-sorted rounds
     bool sorted_ok = false;     // the sorted variant's LDS fits
     int32_t acc_occ = 1;        // bucketed: accumulation workgroups per CU (bucket size follows)
+    int32_t accum_unroll = 8;   // bucketed: staged events in flight per accumulation lane
+    bool accum_fixed = false;   // bucketed: int32 fixed-point accumulation (scale 2^acc_shift)
+    bool fixed_ok = false;      // acc_shift >= 16
+    int32_t acc_shift = 0;
     int32_t ncu = 0;            // compute units of the device
     int32_t lds_max = 0;        // LDS bytes a workgroup may use
     ExtDrive ext{};             // external Poisson drive (n = 0: off)

@@ -248,6 +248,11 @@ int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* t
                                       bucket in LDS, stored as contiguous runs (16 events per lane) */
 #define DTB_LAUNCH_ACCUM_OCC 7     /* bucketed delivery: 1 or 2 accumulation workgroups per CU (the
                                       destination bucket is the LDS slice divided by this) */
+#define DTB_LAUNCH_ACCUM_UNROLL 8  /* bucketed delivery: 8 or 16 staged events in flight per accumulation lane */
+#define DTB_LAUNCH_ACCUM_FIXED 9   /* bucketed delivery: set 1 to sum arrivals as int32 round(w * 2^S), 0 for fp32
+                                      atomics; S is the largest shift keeping every neuron's largest possible
+                                      per-step sum (sum of |w| over its inputs) below 2^30, allowed when S >= 16
+                                      (the default then); get returns S when on, else 0 */
 int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
 int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 
