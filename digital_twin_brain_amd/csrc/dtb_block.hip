@@ -484,18 +484,22 @@ struct StageArgs {
 
 // Stage-kernel pieces.  Event e of a chunk lives in segment sg at offset
 // e - segpre[sg]; loc packs (sg << 24) | offset, or ~0 past the chunk's end.
-template <int EPT>
+// sg is the last segment with segpre[sg] <= e (empty segments share their
+// start with the next one), found by a fixed-depth bisection: the EPT
+// searches of a lane are independent, so their LDS reads overlap instead of
+// forming one dependent chain.
+template <int SEG, int EPT>
 __device__ inline void stage_locate(const uint32_t* segpre, uint32_t r0, uint32_t E, int t, uint32_t* loc)
 {
-    int sg = 0;
+    static_assert((SEG & (SEG - 1)) == 0, "SEG is a power of two");
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
         const uint32_t e = r0 + (uint32_t)m * 256 + t;
-        loc[m] = 0xffffffffu;
-        if (e < E) {
-            while (segpre[sg + 1] <= e) ++sg; // e grows with m
-            loc[m] = ((uint32_t)sg << 24) | (e - segpre[sg]);
-        }
+        int sg = 0;
+#pragma unroll
+        for (int step = SEG / 2; step >= 1; step >>= 1)
+            if (segpre[sg + step] <= e) sg += step;
+        loc[m] = e < E ? (((uint32_t)sg << 24) | (e - segpre[sg])) : 0xffffffffu;
     }
 }
 
@@ -616,7 +620,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                 uint32_t br[EPT], pk_[EPT]; // br = bucket << kRankBits | rank
                 float w0[EPT], w1[EPT];
                 uint32_t loc[EPT];
-                stage_locate<EPT>(segpre, r0, E, t, loc);
+                stage_locate<SEG, EPT>(segpre, r0, E, t, loc);
                 uint32_t cw[EPT];
                 stage_load<PAIR, EPT>(a, segbeg, loc, cw, w0, w1);
                 // bucket and rank inside the bucket within this round
@@ -659,7 +663,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                 uint32_t br[EPT], pk_[EPT];
                 float w0[EPT], w1[EPT];
                 uint32_t loc[EPT];
-                stage_locate<EPT>(segpre, r0, E, t, loc);
+                stage_locate<SEG, EPT>(segpre, r0, E, t, loc);
                 uint32_t cw[EPT];
                 stage_load<PAIR, EPT>(a, segbeg, loc, cw, w0, w1);
 #pragma unroll
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
             // pass 1: the chunk's bucket histogram (targets only)
             for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
                 uint32_t loc[EPT];
-                stage_locate<EPT>(segpre, r0, E, t, loc);
+                stage_locate<SEG, EPT>(segpre, r0, E, t, loc);
                 uint32_t cw[EPT];
 #pragma unroll
                 for (int m = 0; m < EPT; ++m)
@@ -737,7 +741,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
             for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
                 float w0[EPT], w1[EPT];
                 uint32_t loc[EPT];
-                stage_locate<EPT>(segpre, r0, E, t, loc);
+                stage_locate<SEG, EPT>(segpre, r0, E, t, loc);
                 uint32_t cw[EPT];
                 stage_load<PAIR, EPT>(a, segbeg, loc, cw, w0, w1);
 #pragma unroll
