@@ -147,6 +147,38 @@ __device__ inline int32_t refractory_end(int64_t step, double t_spike, float tre
     return (int32_t)(e > 0x7fffffff ? 0x7fffffff : e);
 }
 
+// Streaming loads.  Neuron state, synapse targets and weights and staged
+// events are each read once per step from arrays far larger than L2, so they
+// are loaded non-temporally: they then do not evict the lines that are
+// reused (staging partitions being filled, spike lists, row pointers).  On
+// the 5e6-neuron, 5e9-synapse block this shortens the step by ~10%.
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t nt_u2 __attribute__((ext_vector_type(2)));
+typedef uint32_t nt_u4 __attribute__((ext_vector_type(4)));
+__device__ inline float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ inline uint32_t ld_stream(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ inline float2 ld_stream(const float2* p)
+{
+    const nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2*>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ inline float4 ld_stream(const float4* p)
+{
+    const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline uint2 ld_stream(const uint2* p)
+{
+    const nt_u2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u2*>(p));
+    return make_uint2(v.x, v.y);
+}
+__device__ inline uint4 ld_stream(const uint4* p)
+{
+    const nt_u4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // PSET: the constants shared by groups of neurons (reversal potentials,
 // synaptic decays, leak, threshold, reset, I_ext) come from a small table in
 // LDS indexed by a 1-byte set id instead of 56 B of per-neuron arrays.
@@ -172,20 +204,20 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     float vout = 0.0f;
     int32_t p = -1;
     if (i < a.n) {
-        const float V = a.V[i];
-        float4 J = a.Jp[i];
+        const float V = ld_stream(a.V + i);
+        float4 J = ld_stream(a.Jp + i);
         if (PAIR) {
             float4* r = reinterpret_cast<float4*>(a.acc) + i;
-            const float4 d = *r;
+            const float4 d = ld_stream(r);
             if (a.clear_acc) *r = make_float4(0.f, 0.f, 0.f, 0.f);
             J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
         } else {
             float2* r = reinterpret_cast<float2*>(a.acc) + i;
-            const float2 d = *r;
+            const float2 d = ld_stream(r);
             if (a.clear_acc) *r = make_float2(0.f, 0.f);
             J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
         }
-        const float4 g = a.g[i];
+        const float4 g = ld_stream(a.g + i);
         float4 vu, dc;
         float gL, vL, dtC, vth, vrst, iext;
         if (PSET) {
@@ -515,13 +547,13 @@ __device__ inline void stage_load(const DeliverArgs& a, const int64_t* segbeg, c
         w1[m] = 0.0f;
         if (loc[m] != 0xffffffffu) {
             const int64_t idx = segbeg[loc[m] >> 24] + (loc[m] & 0xffffffu);
-            cw[m] = a.col[idx];
+            cw[m] = ld_stream(a.col + idx);
             if (PAIR) {
-                const float2 wv = a.w2[idx];
+                const float2 wv = ld_stream(a.w2 + idx);
                 w0[m] = wv.x;
                 w1[m] = wv.y;
             } else {
-                w0[m] = a.w[idx];
+                w0[m] = ld_stream(a.w + idx);
             }
         }
     }
@@ -945,7 +977,7 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t e = base + (uint32_t)u * blockDim.x;
-                v[u] = e < cnt ? reinterpret_cast<const uint4*>(s.stage)[at(e)] : make_uint4(0u, 0u, 0u, 0u);
+                v[u] = e < cnt ? ld_stream(reinterpret_cast<const uint4*>(s.stage) + at(e)) : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -965,7 +997,7 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t e = base + (uint32_t)u * blockDim.x;
-                v[u] = e < cnt ? reinterpret_cast<const uint2*>(s.stage)[at(e)] : make_uint2(0u, 0u);
+                v[u] = e < cnt ? ld_stream(reinterpret_cast<const uint2*>(s.stage) + at(e)) : make_uint2(0u, 0u);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
