@@ -342,19 +342,31 @@ __device__ inline uint32_t block_inclusive_scan_256(uint32_t v, uint32_t* wtot)
     return v + add;
 }
 
-// where sub-list entry e = (d - 1) * kSubLists + q of step `step` starts in spk
-__device__ inline int64_t entry_base(const DeliverArgs& a, int64_t step, int32_t e)
+// Spike-list slot of the step that spiked d - 1 steps before the step whose
+// slot is s0 (d <= D < L, so one wrap at most).
+__device__ inline int32_t delay_slot(int32_t s0, int32_t d, int32_t L)
+{
+    const int32_t s = s0 - (d - 1);
+    return s < 0 ? s + L : s;
+}
+
+// where sub-list entry e = (d - 1) * kSubLists + q starts in spk, for the
+// step whose slot is s0
+__device__ inline int64_t entry_base(const DeliverArgs& a, int32_t s0, int32_t e)
 {
     const int32_t d = e / kSubLists + 1, q = e % kSubLists;
-    return ((int64_t)((step - d + 1 + a.L) % a.L) * kSubLists + q) * a.spk_cap;
+    return ((int64_t)delay_slot(s0, d, a.L) * kSubLists + q) * a.spk_cap;
 }
 
 // Segments to deliver at batch step k: entry e = (d - 1) * kSubLists + q holds
 // sub-list q of the step that spiked d - 1 steps ago; cum[e + 1] = segments in
 // entries <= e.  Workgroups of 256 threads; returns the number of entries.
+// Every counter load is issued before the first is waited for (entries past
+// E read entry E - 1's counter and are then dropped).
 __device__ inline int32_t delivery_entries(const DeliverArgs& a, int32_t k, uint32_t* cum, uint32_t* wtot)
 {
     const int64_t step = a.ctl->nbase + k + 1;
+    const int32_t s0 = (int32_t)(step % a.L);
     const int32_t E = a.D * kSubLists;
     constexpr int kPer = kMaxEntries / 256;
     uint32_t c[kPer];
@@ -362,13 +374,13 @@ __device__ inline int32_t delivery_entries(const DeliverArgs& a, int32_t k, uint
     const int32_t e0 = threadIdx.x * kPer;
 #pragma unroll
     for (int x = 0; x < kPer; ++x) {
-        const int32_t e = e0 + x;
-        c[x] = 0;
-        if (e < E) {
-            const int32_t d = e / kSubLists + 1, q = e % kSubLists;
-            const int64_t sl = (int64_t)((step - d + 1 + a.L) % a.L) * kSubLists + q;
-            c[x] = min(a.spk_cnt[sl * kCntStride], (uint32_t)a.spk_cap);
-        }
+        const int32_t e = min(e0 + x, E - 1);
+        const int32_t d = e / kSubLists + 1, q = e % kSubLists;
+        c[x] = a.spk_cnt[((int64_t)delay_slot(s0, d, a.L) * kSubLists + q) * kCntStride];
+    }
+#pragma unroll
+    for (int x = 0; x < kPer; ++x) {
+        c[x] = e0 + x < E ? min(c[x], (uint32_t)a.spk_cap) : 0u;
         sum += c[x];
     }
     uint32_t run = block_inclusive_scan_256(sum, wtot) - sum;
@@ -401,7 +413,7 @@ __global__ __launch_bounds__(256) void deliver_kernel(DeliverArgs a, int32_t k)
 {
     __shared__ uint32_t cum[kMaxEntries + 1];
     __shared__ uint32_t wtot[4];
-    const int64_t step = a.ctl->nbase + k + 1;
+    const int32_t s0 = (int32_t)((a.ctl->nbase + k + 1) % a.L);
     const int32_t E = delivery_entries(a, k, cum, wtot);
     const uint32_t total = cum[E];
     const uint32_t lane = __lane_id();
@@ -415,7 +427,7 @@ __global__ __launch_bounds__(256) void deliver_kernel(DeliverArgs a, int32_t k)
     for (uint32_t r = gid; r < total; r += ngroups) {
         while (r >= cum[e + 1]) ++e; // r only grows, so e only moves forward
         const int32_t d = e / kSubLists + 1;
-        const int32_t j = a.spk[entry_base(a, step, e) + (r - cum[e])];
+        const int32_t j = a.spk[entry_base(a, s0, e) + (r - cum[e])];
         const int64_t row = a.rowptr[j];
         int64_t beg, end;
         if (a.D == 1) {
@@ -423,8 +435,9 @@ __global__ __launch_bounds__(256) void deliver_kernel(DeliverArgs a, int32_t k)
             end = a.rowptr[j + 1];
         } else {
             const uint16_t* de = a.dend + (int64_t)j * a.D;
-            beg = row + (d > 1 ? de[d - 2] : 0);
-            end = row + de[d - 1];
+            const uint32_t lo = de[d > 1 ? d - 2 : 0], hi = de[d - 1]; // both loads in flight
+            beg = row + (d > 1 ? lo : 0u);
+            end = row + hi;
         }
         ev += (uint64_t)(end - beg);
         for (int64_t e = beg + sub; e < end; e += 2 * G) {
@@ -608,7 +621,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     extern __shared__ uint32_t hist[]; // [nbk]
     __shared__ uint32_t wtot[4];
     const DeliverArgs& a = s.d;
-    const int64_t step = a.ctl->nbase + k + 1;
+    const int32_t s0 = (int32_t)((a.ctl->nbase + k + 1) % a.L);
     const int t = threadIdx.x;
     const int32_t part = (int32_t)(blockIdx.x % kStageParts);
     for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
@@ -625,7 +638,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
         if (t < SEG && g < total) {
             const int32_t e = entry_of(cum, NE, g);
             const int32_t d = e / kSubLists + 1;
-            const int32_t j = a.spk[entry_base(a, step, e) + (g - cum[e])];
+            const int32_t j = a.spk[entry_base(a, s0, e) + (g - cum[e])];
             const int64_t row = a.rowptr[j];
             int64_t end;
             if (a.D == 1) {
@@ -633,8 +646,9 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                 end = a.rowptr[j + 1];
             } else {
                 const uint16_t* de = a.dend + (int64_t)j * a.D;
-                beg = row + (d > 1 ? de[d - 2] : 0);
-                end = row + de[d - 1];
+                const uint32_t lo = de[d > 1 ? d - 2 : 0], hi = de[d - 1]; // both loads in flight
+                beg = row + (d > 1 ? lo : 0u);
+                end = row + hi;
             }
             len = (uint32_t)(end - beg);
         }
