@@ -530,21 +530,32 @@ struct StageArgs {
 // Stage-kernel pieces.  Event e of a chunk lives in segment sg at offset
 // e - segpre[sg]; loc packs (sg << 24) | offset, or ~0 past the chunk's end.
 // sg is the last segment with segpre[sg] <= e (empty segments share their
-// start with the next one), found by a fixed-depth bisection: the EPT
-// searches of a lane are independent, so their LDS reads overlap instead of
-// forming one dependent chain.
+// start with the next one), found by a fixed-depth bisection.  The step loop
+// is outermost: each step issues the EPT independent LDS reads of a lane
+// back to back and waits once, so a round costs log2(SEG) + 1 LDS latencies
+// rather than EPT times that.
 template <int SEG, int EPT>
 __device__ inline void stage_locate(const uint32_t* segpre, uint32_t r0, uint32_t E, int t, uint32_t* loc)
 {
     static_assert((SEG & (SEG - 1)) == 0, "SEG is a power of two");
+    uint32_t sg[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) sg[m] = 0;
+#pragma unroll
+    for (int step = SEG / 2; step >= 1; step >>= 1) {
+        uint32_t v[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) v[m] = segpre[sg[m] + step];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) sg[m] += v[m] <= r0 + (uint32_t)m * 256 + t ? step : 0;
+    }
+    uint32_t b[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) b[m] = segpre[sg[m]];
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
         const uint32_t e = r0 + (uint32_t)m * 256 + t;
-        int sg = 0;
-#pragma unroll
-        for (int step = SEG / 2; step >= 1; step >>= 1)
-            if (segpre[sg + step] <= e) sg += step;
-        loc[m] = e < E ? (((uint32_t)sg << 24) | (e - segpre[sg])) : 0xffffffffu;
+        loc[m] = e < E ? ((sg[m] << 24) | (e - b[m])) : 0xffffffffu;
     }
 }
 
@@ -687,12 +698,15 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                     hist[b] = h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
                 }
                 __syncthreads();
+                uint32_t hb[EPT]; // all reads of the reserved bases before the stores
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) hb[m] = hist[br[m] == 0xffffffffu ? 0u : br[m] >> kRankBits];
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     if (br[m] == 0xffffffffu) continue;
                     const uint32_t bk = br[m] >> kRankBits;
-                    stage_put<PAIR>(s, part, bk, (uint64_t)hist[bk] + (br[m] & ((1u << kRankBits) - 1)), pk_[m],
-                                    w0[m], w1[m]);
+                    stage_put<PAIR>(s, part, bk, (uint64_t)hb[m] + (br[m] & ((1u << kRankBits) - 1)), pk_[m], w0[m],
+                                    w1[m]);
                 }
                 __syncthreads();
                 for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
@@ -701,9 +715,9 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
         } else if constexpr (VAR == 2) {
             uint32_t* pre = hist + s.nbk;      // [nbk] first LDS slot of each bucket in the round
             uint32_t* base = pre + s.nbk;      // [nbk] reserved staging position of each bucket
-            uint32_t* bpk = base + s.nbk;      // [256 * EPT] sorted round: packed target
-            uint32_t* bw0 = bpk + 256 * EPT;   //                           weight 0
-            uint32_t* bw1 = bw0 + 256 * EPT;   //                           weight 1 (pair)
+            uint32_t* bpk = base + s.nbk;        // [256 * EPT + 1] sorted round: packed target
+            uint32_t* bw0 = bpk + 256 * EPT + 1; //                               weight 0
+            uint32_t* bw1 = bw0 + 256 * EPT + 1; //                               weight 1 (pair)
             const int per = (s.nbk + 255) / 256;
             for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
                 uint32_t br[EPT], pk_[EPT];
@@ -738,23 +752,45 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
                     base[b] = h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
                 }
                 __syncthreads();
+                // scatter into the sorted round: all reads of pre[] first, then
+                // the writes (a lane without an event writes the spare slot)
+                uint32_t jw[EPT];
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) jw[m] = pre[br[m] == 0xffffffffu ? 0u : br[m] >> kRankBits];
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
-                    if (br[m] == 0xffffffffu) continue;
+                    const bool ok = br[m] != 0xffffffffu;
                     const uint32_t bk = br[m] >> kRankBits;
-                    const uint32_t j = pre[bk] + (br[m] & ((1u << kRankBits) - 1));
+                    const uint32_t j = ok ? jw[m] + (br[m] & ((1u << kRankBits) - 1)) : 256u * EPT;
                     bpk[j] = pk_[m] | (bk << kMaxBucketBits); // the bucket rides in bits 15..30
                     bw0[j] = __float_as_uint(w0[m]);
                     if (PAIR) bw1[j] = __float_as_uint(w1[m]);
                 }
                 __syncthreads();
-                // consecutive lanes store consecutive events of a bucket
+                // consecutive lanes store consecutive events of a bucket; the
+                // LDS reads of EPT events are issued before their stores
                 const uint32_t cnt = min(E - r0, (uint32_t)(256 * EPT));
-                for (uint32_t j = t; j < cnt; j += 256) {
-                    const uint32_t v = bpk[j];
-                    const uint32_t bk = (v >> kMaxBucketBits) & 0xffffu;
-                    stage_put<PAIR>(s, part, bk, (uint64_t)base[bk] + (j - pre[bk]), v & 0x80007fffu,
-                                    __uint_as_float(bw0[j]), PAIR ? __uint_as_float(bw1[j]) : 0.0f);
+                uint32_t v[EPT], x0[EPT], x1[EPT], at[EPT];
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const uint32_t j = (uint32_t)m * 256 + t;
+                    v[m] = bpk[j];
+                    x0[m] = bw0[j];
+                    x1[m] = PAIR ? bw1[j] : 0u;
+                }
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const uint32_t j = (uint32_t)m * 256 + t;
+                    const uint32_t bk = j < cnt ? (v[m] >> kMaxBucketBits) & 0xffffu : 0u;
+                    at[m] = base[bk] - pre[bk];
+                }
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const uint32_t j = (uint32_t)m * 256 + t;
+                    if (j >= cnt) break;
+                    const uint32_t bk = (v[m] >> kMaxBucketBits) & 0xffffu;
+                    stage_put<PAIR>(s, part, bk, (uint64_t)(at[m] + j), v[m] & 0x80007fffu, __uint_as_float(x0[m]),
+                                    __uint_as_float(x1[m]));
                 }
                 __syncthreads();
                 for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
@@ -813,7 +849,7 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
 static size_t stage_lds(const Block* b, int ept, int var)
 {
     size_t words = (size_t)b->nbk;
-    if (var == 2) words += 2 * (size_t)b->nbk + (size_t)256 * ept * (b->pair ? 3 : 2);
+    if (var == 2) words += 2 * (size_t)b->nbk + ((size_t)256 * ept + 1) * (b->pair ? 3 : 2); // + a spare slot
     return sizeof(uint32_t) * words;
 }
 
