@@ -510,6 +510,7 @@ constexpr int kStageParts = 8;       // staging partitions per bucket: workgroup
                                      // same-address reservation atomics
 constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
 constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
+constexpr int kStageStaticLds = 4096; // static LDS of the stage kernel (segment table), bytes
 
 struct StageArgs {
     DeliverArgs d;
@@ -621,15 +622,21 @@ __device__ inline void stage_put(const StageArgs& s, int32_t part, uint32_t bk, 
 //   2  one reservation per (round, bucket), the round's events first sorted
 //      by bucket in LDS so that consecutive lanes store each bucket's run
 //      contiguously (whole cache lines instead of scattered 8-byte stores).
+// Four workgroups per CU at 16 events per lane: registers are capped at 128
+// per lane and the dynamic LDS holds only what the block needs (entries of
+// its delays, its buckets, the sorted round), so a launch of up to 4 x CUs
+// workgroups is resident at once instead of leaving a tail wave.  (At 32
+// events per lane that cap would spill; two per CU there.)
 template <bool PAIR, int SEG, int EPT, int VAR>
-__global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t k)
+__global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(StageArgs s, int32_t k)
 {
     static_assert(SEG >= 64 && SEG <= 256, "segment index is packed in 8 bits");
     static_assert(256 * EPT <= (1 << 13), "rank inside a round must fit kRankBits");
-    __shared__ uint32_t cum[kMaxEntries + 1];
     __shared__ uint32_t segpre[SEG + 1];
     __shared__ int64_t segbeg[SEG];
-    extern __shared__ uint32_t hist[]; // [nbk]
+    extern __shared__ uint32_t dyn[];
+    uint32_t* cum = dyn;                                // [D * kSubLists + 1]
+    uint32_t* hist = dyn + s.d.D * kSubLists + 1;       // [nbk]
     __shared__ uint32_t wtot[4];
     const DeliverArgs& a = s.d;
     const int32_t s0 = (int32_t)((a.ctl->nbase + k + 1) % a.L);
@@ -844,11 +851,12 @@ __global__ __launch_bounds__(256) void deliver_stage_kernel(StageArgs s, int32_t
     if (t == 0 && ev_wg && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev_wg);
 }
 
-// dynamic LDS of the stage kernel: the bucket histogram, plus for the sorted
-// variant the round's bucket offsets, reservations and sorted events
+// dynamic LDS of the stage kernel: the delivery entries and the bucket
+// histogram, plus for the sorted variant the round's bucket offsets,
+// reservations and sorted events
 static size_t stage_lds(const Block* b, int ept, int var)
 {
-    size_t words = (size_t)b->nbk;
+    size_t words = (size_t)b->D * kSubLists + 1 + (size_t)b->nbk;
     if (var == 2) words += 2 * (size_t)b->nbk + ((size_t)256 * ept + 1) * (b->pair ? 3 : 2); // + a spare slot
     return sizeof(uint32_t) * words;
 }
@@ -1650,8 +1658,8 @@ static int use_bucket_geom(Block* b, int occ)
     b->bmagic = g.bmagic;
     b->nbk = g.nbk;
     b->stage_cap = g.cap;
-    // the sorted staging variant needs its round in LDS next to ~12 KB of static arrays
-    b->sorted_ok = stage_lds(b, 16, 2) + 16384 <= (size_t)b->lds_max;
+    // the sorted staging variant needs its round in LDS next to the static segment table
+    b->sorted_ok = stage_lds(b, 16, 2) + kStageStaticLds <= (size_t)b->lds_max;
     return DTB_OK;
 }
 
@@ -1684,8 +1692,8 @@ static int ensure_bucketed(Block* b)
                                     hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 16, false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
-    DTB_TRY(allow_sorted_stage_lds<true>(b->lds_max - 16384));
-    DTB_TRY(allow_sorted_stage_lds<false>(b->lds_max - 16384));
+    DTB_TRY(allow_sorted_stage_lds<true>(b->lds_max - kStageStaticLds));
+    DTB_TRY(allow_sorted_stage_lds<false>(b->lds_max - kStageStaticLds));
     int64_t nbk_max = 0, events_max = 0;
     for (int occ = 1; occ <= 2; ++occ) {
         const BucketGeom g = bucket_geom(b, occ);
