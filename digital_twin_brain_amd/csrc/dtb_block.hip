@@ -511,6 +511,7 @@ constexpr int kStageParts = 8;       // staging partitions per bucket: workgroup
 constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
 constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
 constexpr int kStageStaticLds = 4096; // static LDS of the stage kernel (segment table), bytes
+constexpr int kAccumStaticLds = 256;  // static LDS of the accumulation kernel (partition counts), bytes
 
 struct StageArgs {
     DeliverArgs d;
@@ -1011,17 +1012,22 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     const DeliverArgs& a = s.d;
     const int32_t b = blockIdx.x;
     const int t = threadIdx.x;
+    // the partition counts and the overflow flag, one lane each (loads in
+    // parallel, overlapped with clearing the slice)
+    __shared__ uint32_t s_part[kStageParts + 1];
+    if (t < kStageParts)
+        s_part[t] = (uint32_t)min((uint64_t)s.cursor[((size_t)b * kStageParts + t) * kCursorStride], (uint64_t)s.cap);
+    else if (t == kStageParts)
+        s_part[t] = s.ovf[(size_t)b * kCursorStride];
     for (int x = t; x < s.bsize * C; x += blockDim.x) slice[x] = 0.0f;
+    __syncthreads();
     // the bucket's partitions back to back: pre[p] = events in partitions < p
     uint32_t pre[kStageParts + 1];
     pre[0] = 0;
 #pragma unroll
-    for (int p = 0; p < kStageParts; ++p)
-        pre[p + 1] = pre[p] + (uint32_t)min((uint64_t)s.cursor[((size_t)b * kStageParts + p) * kCursorStride],
-                                            (uint64_t)s.cap);
+    for (int p = 0; p < kStageParts; ++p) pre[p + 1] = pre[p] + s_part[p];
     const uint32_t cnt = pre[kStageParts];
-    const bool ovf = s.ovf[(size_t)b * kCursorStride] != 0u;
-    __syncthreads();
+    const bool ovf = s_part[kStageParts] != 0u;
     const int64_t sb = (int64_t)b * kStageParts * s.cap;
     auto at = [&](uint32_t e) -> int64_t {
         int p = 0;
@@ -1632,7 +1638,8 @@ static BucketGeom bucket_geom(const Block* b, int occ)
 {
     const int64_t ncu = std::max(b->ncu, 1);
     const int64_t bpn = b->pair ? 16 : 8; // LDS bytes per neuron
-    const int64_t max_bs = std::min<int64_t>((1 << kMaxBucketBits) - 64, b->lds_max / occ / bpn / 64 * 64);
+    const int64_t max_bs =
+        std::min<int64_t>((1 << kMaxBucketBits) - 64, (b->lds_max - kAccumStaticLds) / occ / bpn / 64 * 64);
     const int64_t per_wave = ncu * occ * max_bs;
     const int64_t waves = (b->n + per_wave - 1) / per_wave;
     const int64_t per = (b->n + waves * ncu * occ - 1) / (waves * ncu * occ);
@@ -1677,21 +1684,21 @@ static int ensure_bucketed(Block* b)
     b->lds_max = std::min(std::max(lds, 131072), 163840);
     // the attribute is per kernel and process-wide: allow the largest slice any block may use
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 8, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 8, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 16, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<true, 16, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 8, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 8, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 16, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY_HIP(hipFuncSetAttribute((const void*)deliver_accum_kernel<false, 16, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, b->lds_max - kAccumStaticLds));
     DTB_TRY(allow_sorted_stage_lds<true>(b->lds_max - kStageStaticLds));
     DTB_TRY(allow_sorted_stage_lds<false>(b->lds_max - kStageStaticLds));
     int64_t nbk_max = 0, events_max = 0;
