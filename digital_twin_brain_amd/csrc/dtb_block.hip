@@ -647,14 +647,22 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
     const int32_t NE = delivery_entries(a, k, cum, wtot);
     const uint32_t total = cum[NE];
     if (blockIdx.x == 0 && t == 0 && a.rec_nspk) a.rec_nspk[k] = cum[kSubLists];
-    const uint32_t nchunks = (total + SEG - 1) / SEG;
+    // The step's segments in kper * gridDim chunks of equal size (at most
+    // SEG), dealt round-robin: every workgroup gets kper chunks, and each
+    // gets chunks from all delays (segment lengths vary with the delay).
+    const uint64_t G = gridDim.x;
+    const uint64_t kper = ((uint64_t)total + G * SEG - 1) / (G * SEG);
+    const uint64_t nch = G * kper;
+    const uint64_t cs = nch ? ((uint64_t)total + nch - 1) / nch : 0;
     uint64_t ev_wg = 0;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    for (uint64_t c = blockIdx.x; c < nch && c * cs < total; c += G) {
         // segment lengths of this chunk -> LDS prefix
-        const uint32_t g = c * SEG + t;
+        const uint32_t c0 = (uint32_t)(c * cs);
+        const uint32_t g_end = (uint32_t)min((uint64_t)total, c * cs + cs);
+        const uint32_t g = c0 + t;
         uint32_t len = 0;
         int64_t beg = 0;
-        if (t < SEG && g < total) {
+        if (t < SEG && g < g_end) {
             const int32_t e = entry_of(cum, NE, g);
             const int32_t d = e / kSubLists + 1;
             const int32_t j = a.spk[entry_base(a, s0, e) + (g - cum[e])];
