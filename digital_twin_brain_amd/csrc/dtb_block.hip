@@ -204,20 +204,23 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     float vout = 0.0f;
     int32_t p = -1;
     if (i < a.n) {
+        // every load of the neuron before its first store: the compiler makes a
+        // load that follows a store to another array wait for that store
         const float V = ld_stream(a.V + i);
         float4 J = ld_stream(a.Jp + i);
+        float4 d; // this step's arrivals per receptor (compact: AMPA, NMDA share one sum, GABAs the other)
         if (PAIR) {
-            float4* r = reinterpret_cast<float4*>(a.acc) + i;
-            const float4 d = ld_stream(r);
-            if (a.clear_acc) *r = make_float4(0.f, 0.f, 0.f, 0.f);
-            J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
+            d = ld_stream(reinterpret_cast<const float4*>(a.acc) + i);
         } else {
-            float2* r = reinterpret_cast<float2*>(a.acc) + i;
-            const float2 d = ld_stream(r);
-            if (a.clear_acc) *r = make_float2(0.f, 0.f);
-            J.x += d.x; J.y += d.x; J.z += d.y; J.w += d.y;
+            const float2 d2 = ld_stream(reinterpret_cast<const float2*>(a.acc) + i);
+            d = make_float4(d2.x, d2.x, d2.y, d2.y);
         }
         const float4 g = ld_stream(a.g + i);
+        const int32_t rend = a.ref_end[i];
+        const int32_t pp = a.pop[i];
+        const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
+        const float un = a.noise_mode == DTB_NOISE_UNIFORMS ? a.uniforms[(int64_t)k * a.n + i] : 0.0f;
+        J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
         float4 vu, dc;
         float gL, vL, dtC, vth, vrst, iext;
         if (PSET) {
@@ -236,22 +239,21 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
         isyn += g.w * (vu.w - V) * J.w;
         const float main_part = -gL * (V - vL);
         const float vn = V + dtC * (main_part + isyn + iext);
-        const bool not_ref = step >= (int64_t)a.ref_end[i];
+        const bool not_ref = step >= (int64_t)rend;
         const float vi = not_ref ? vn : vrst;
         act = vi >= vth;
         vout = vi > vth ? vth : vi;
+        if (a.clear_acc) {
+            if (PAIR) reinterpret_cast<float4*>(a.acc)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            else reinterpret_cast<float2*>(a.acc)[i] = make_float2(0.f, 0.f);
+        }
         a.V[i] = vout;
         if (act) {
             a.ref_end[i] = refractory_end(step, t, a.tref[i], a.dt);
             a.last_step[i] = (int32_t)step;
             a.t_last[i] = t;
         }
-        float u;
-        if (a.noise_mode == DTB_NOISE_UNIFORMS)
-            u = a.uniforms[(int64_t)k * a.n + i];
-        else
-            u = noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
-        const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
+        const float u = a.noise_mode == DTB_NOISE_UNIFORMS ? un : noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
         spike = (u < pr) || (act && !a.isolated);
         float4 jn = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
         if (a.ext.n) {
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
         }
         a.Jp[i] = jn;
         a.flags[i] = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
-        p = a.pop[i];
+        p = pp;
     }
 
     // spike compaction: ballot + per-wave prefix, one global atomic per workgroup
