@@ -241,7 +241,8 @@ int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* t
 #define DTB_LAUNCH_DELIVER_MODE 2  /* 0: global atomics, 1: bucketed LDS accumulation */
 #define DTB_LAUNCH_NEURON_PSETS 3  /* set: 0 per-neuron constants, 1 shared parameter sets when <= 64
                                       distinct; get: number of sets in use (0 = per-neuron) */
-#define DTB_LAUNCH_STAGE_SEGS 4    /* bucketed delivery: 64, 128 or 256 (row, delay) segments per chunk */
+#define DTB_LAUNCH_STAGE_SEGS 4    /* bucketed delivery: at most 64, 128 or 256 (row, delay) segments per chunk (the
+                                      step's segments are split into equal chunks, the same count per workgroup) */
 #define DTB_LAUNCH_STAGE_EPT 5     /* bucketed delivery: 16 or 32 events per lane per staging round */
 #define DTB_LAUNCH_STAGE_VARIANT 6 /* bucketed delivery staging: 0 a reservation per round, lanes store their own
                                       events; 1 a reservation per chunk (two passes); 2 rounds sorted by
