@@ -637,7 +637,7 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
     static_assert(256 * EPT <= (1 << 13), "rank inside a round must fit kRankBits");
     __shared__ uint32_t segpre[SEG + 1];
     __shared__ int64_t segbeg[SEG];
-    extern __shared__ uint32_t dyn[];
+    extern __shared__ __align__(16) uint32_t dyn[];
     uint32_t* cum = dyn;                                // [D * kSubLists + 1]
     uint32_t* hist = dyn + s.d.D * kSubLists + 1;       // [nbk]
     __shared__ uint32_t wtot[4];
@@ -731,11 +731,11 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
                 __syncthreads();
             }
         } else if constexpr (VAR == 2) {
-            uint32_t* pre = hist + s.nbk;      // [nbk] first LDS slot of each bucket in the round
-            uint32_t* base = pre + s.nbk;      // [nbk] reserved staging position of each bucket
-            uint32_t* bpk = base + s.nbk;        // [256 * EPT + 1] sorted round: packed target
-            uint32_t* bw0 = bpk + 256 * EPT + 1; //                               weight 0
-            uint32_t* bw1 = bw0 + 256 * EPT + 1; //                               weight 1 (pair)
+            uint32_t* pre = hist + s.nbk; // [nbk] first LDS slot of each bucket in the round
+            uint32_t* off = pre + s.nbk;  // [nbk] staging position of the round's slot 0 for that bucket
+            // sorted round (+ a spare slot): (packed target, weight 0) pairs, 8-byte aligned, then weight 1
+            uint2* bev = reinterpret_cast<uint2*>(off + s.nbk + ((reinterpret_cast<uintptr_t>(off + s.nbk) >> 2) & 1));
+            uint32_t* bw1 = reinterpret_cast<uint32_t*>(bev + 256 * EPT + 1);
             const int per = (s.nbk + 255) / 256;
             for (uint32_t r0 = 0; r0 < E; r0 += 256 * EPT) {
                 uint32_t br[EPT], pk_[EPT];
@@ -766,8 +766,10 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
                     if (b >= s.nbk) break;
                     const uint32_t h = hist[b];
                     pre[b] = run;
+                    const uint32_t base =
+                        h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
+                    off[b] = base - run; // modulo 2^32: off + slot is the event's staging position
                     run += h;
-                    base[b] = h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
                 }
                 __syncthreads();
                 // scatter into the sorted round: all reads of pre[] first, then
@@ -780,8 +782,8 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
                     const bool ok = br[m] != 0xffffffffu;
                     const uint32_t bk = br[m] >> kRankBits;
                     const uint32_t j = ok ? jw[m] + (br[m] & ((1u << kRankBits) - 1)) : 256u * EPT;
-                    bpk[j] = pk_[m] | (bk << kMaxBucketBits); // the bucket rides in bits 15..30
-                    bw0[j] = __float_as_uint(w0[m]);
+                    // the bucket rides in bits 15..30 of the packed target
+                    bev[j] = make_uint2(pk_[m] | (bk << kMaxBucketBits), __float_as_uint(w0[m]));
                     if (PAIR) bw1[j] = __float_as_uint(w1[m]);
                 }
                 __syncthreads();
@@ -792,15 +794,15 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const uint32_t j = (uint32_t)m * 256 + t;
-                    v[m] = bpk[j];
-                    x0[m] = bw0[j];
+                    const uint2 e = bev[j];
+                    v[m] = e.x;
+                    x0[m] = e.y;
                     x1[m] = PAIR ? bw1[j] : 0u;
                 }
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const uint32_t j = (uint32_t)m * 256 + t;
-                    const uint32_t bk = j < cnt ? (v[m] >> kMaxBucketBits) & 0xffffu : 0u;
-                    at[m] = base[bk] - pre[bk];
+                    at[m] = off[j < cnt ? (v[m] >> kMaxBucketBits) & 0xffffu : 0u];
                 }
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
@@ -868,7 +870,8 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
 static size_t stage_lds(const Block* b, int ept, int var)
 {
     size_t words = (size_t)b->D * kSubLists + 1 + (size_t)b->nbk;
-    if (var == 2) words += 2 * (size_t)b->nbk + ((size_t)256 * ept + 1) * (b->pair ? 3 : 2); // + a spare slot
+    // + pre, off, one word of alignment, the round with a spare slot
+    if (var == 2) words += 2 * (size_t)b->nbk + 1 + ((size_t)256 * ept + 1) * (b->pair ? 3 : 2);
     return sizeof(uint32_t) * words;
 }
 
