@@ -112,6 +112,20 @@ class WindowSurrogate:
             epochs = 20 if self.per_voxel else 400
         return self.mlp.fit(X, Y, epochs=epochs, batch=batch)
 
+    def finetune(self, epochs, replay=1.0, seed=0):
+        """Train on the last added window plus `replay` times as many rows drawn
+        at random from the earlier ones (so the fit does not drift towards the
+        latest window, at a small fraction of a pass over everything)."""
+        Xn, Yn = self.X[-1], self.Y[-1]
+        if len(self.X) > 1 and replay > 0:
+            Xo = torch.cat(self.X[:-1])
+            Yo = torch.cat(self.Y[:-1])
+            k = min(Xo.shape[0], int(round(replay * Xn.shape[0])))
+            idx = torch.randperm(Xo.shape[0], generator=torch.Generator().manual_seed(seed))[:k]
+            Xn = torch.cat((Xn, Xo.index_select(0, idx)))
+            Yn = torch.cat((Yn, Yo.index_select(0, idx)))
+        return self.mlp.fit(Xn, Yn, epochs=epochs, seed=seed)
+
     def predict(self, X):
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
         d = self.mlp.predict(self._rows(X)).double().cpu().reshape(X.shape[0], -1)
@@ -119,14 +133,18 @@ class WindowSurrogate:
 
 
 class SurrogateDA:
-    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=None):
+    """Every simulated window fine-tunes the surrogate on that window plus an
+    equal-sized replay of earlier rows (`finetune_epochs` passes)."""
+
+    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=None, replay=1.0):
         self.da = da
         self.sur = surrogate
         self.every = int(every)
         self.nrmse_max = float(nrmse_max)
         if finetune_epochs is None:
-            finetune_epochs = 2 if getattr(surrogate, "per_voxel", False) else 40
+            finetune_epochs = 4 if getattr(surrogate, "per_voxel", False) else 40
         self.finetune_epochs = int(finetune_epochs)
+        self.replay = float(replay)
 
     def _advance_hp(self, hp_sigma):
         da = self.da
@@ -155,7 +173,7 @@ class SurrogateDA:
                 force = e > self.nrmse_max
                 self.sur.add(X, tail.reshape(E, -1))
                 if self.finetune_epochs:
-                    self.sur.fit(epochs=self.finetune_epochs, renormalize=False)
+                    self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t)
             else:
                 hpl = da._hp_log.reshape(E, -1)[:, da.for_hidden_state].reshape(E, pred.shape[1], -1)
                 da._hidden_state = torch.cat((hpl.double().to(pred.device), pred), dim=2)
