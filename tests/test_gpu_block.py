@@ -445,6 +445,39 @@ def test_bucketed_delivery_matches_atomics_including_overflow(pair):
     assert outs[0][2][:20].max() > 3 * 8192 * 3  # some bucket saw more events than its staging room
 
 
+def test_autotuner_contract_and_cache(tmp_path, monkeypatch):
+    """The tuner re-measures its finalists, keeps the best of them, leaves
+    the block on that shape, and a second call is served from the cache
+    (which restores the same shape on a fresh block)."""
+    from digital_twin_brain_amd import NativeBlock, autotune
+
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tune.json"))
+    n, ne, K, D = 20000, 16000, 100, 4
+    rng = np.random.default_rng(21)
+    props = _plain_like_props(n, ne, rng, g_scale=0.3)
+    ref = orc.gen_fixed_indegree(n, ne, K, D, 0.0, 1.0, False, seed=22)
+    kw = dict(noise_mode=0, noise_rate=0.01, isolated=True)  # stationary spike counts: a stable cache key
+    keys = ("deliver_mode", "deliver_lanes", "deliver_grid", "stage_segs", "stage_ept", "stage_variant", "accum_occ")
+
+    def block():
+        return NativeBlock.from_csr(props, ref["rowptr"], ref["post"], ref["kind"], ref["w0"], None, ref["delay"],
+                                    dt=0.1, noise_seed=2)
+
+    nb = block()
+    best, table = autotune.tune_delivery(nb, kw, steps=10, use_cache=True)
+    finalists = {k[:-1]: v for k, v in table.items() if k.endswith("*")}
+    assert 1 <= len(finalists) <= 6
+    key = "/".join(str(best[k]) for k in keys)
+    assert finalists[key] == min(finalists.values()) == best["us_per_1e6_events"]
+    got = nb.get_launch()
+    assert all(got[k] == best[k] for k in keys)
+    nb2 = block()
+    best2, table2 = autotune.tune_delivery(nb2, kw, steps=10, use_cache=True)
+    assert "cached" in table2 and best2 == best
+    got2 = nb2.get_launch()
+    assert all(got2[k] == best[k] for k in keys)
+
+
 def test_gpu_members_generator_csr_bit_exact():
     """Uneven packed members (the batched ensemble layout) against the oracle."""
     from digital_twin_brain_amd import NativeBlock
