@@ -512,6 +512,7 @@ constexpr int kStageParts = 8;       // staging partitions per bucket: workgroup
                                      // same-address reservation atomics
 constexpr int kMaxBucketsLds = 8192; // n <= 2^26 neurons
 constexpr int kRankBits = 13;        // rank of an event inside its bucket within a round (< 8192)
+constexpr uint32_t kPartial = 0xffffffffu; // a bucket whose round reservation does not fit entirely
 constexpr int kStageStaticLds = 4096; // static LDS of the stage kernel (segment table), bytes
 constexpr int kAccumStaticLds = 256;  // static LDS of the accumulation kernel (partition counts), bytes
 
@@ -732,7 +733,9 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
             }
         } else if constexpr (VAR == 2) {
             uint32_t* pre = hist + s.nbk; // [nbk] first LDS slot of each bucket in the round
-            uint32_t* off = pre + s.nbk;  // [nbk] staging position of the round's slot 0 for that bucket
+            // [nbk] staging index of the round's slot 0 for that bucket when the
+            // bucket's whole reservation fits its partition, else kPartial
+            uint32_t* off = pre + s.nbk;
             // sorted round (+ a spare slot): (packed target, weight 0) pairs, 8-byte aligned, then weight 1
             uint2* bev = reinterpret_cast<uint2*>(off + s.nbk + ((reinterpret_cast<uintptr_t>(off + s.nbk) >> 2) & 1));
             uint32_t* bw1 = reinterpret_cast<uint32_t*>(bev + 256 * EPT + 1);
@@ -768,7 +771,13 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
                     pre[b] = run;
                     const uint32_t base =
                         h ? atomicAdd(&s.cursor[((size_t)b * kStageParts + part) * kCursorStride], h) : 0u;
-                    off[b] = base - run; // modulo 2^32: off + slot is the event's staging position
+                    // modulo 2^32, off + slot is the event's staging index (the
+                    // staging array holds fewer than 2^32 events); a partly full
+                    // reservation keeps the per-event check of stage_put
+                    off[b] = (uint64_t)base + h <= (uint64_t)s.cap
+                                 ? (uint32_t)(((uint64_t)b * kStageParts + part) * s.cap) + base - run
+                                 : kPartial;
+                    hist[b] = base - run; // position in the partition of the round's slot 0, for kPartial
                     run += h;
                 }
                 __syncthreads();
@@ -808,9 +817,18 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
                 for (int m = 0; m < EPT; ++m) {
                     const uint32_t j = (uint32_t)m * 256 + t;
                     if (j >= cnt) break;
-                    const uint32_t bk = (v[m] >> kMaxBucketBits) & 0xffffu;
-                    stage_put<PAIR>(s, part, bk, (uint64_t)(at[m] + j), v[m] & 0x80007fffu, __uint_as_float(x0[m]),
-                                    __uint_as_float(x1[m]));
+                    const uint32_t pk = v[m] & 0x80007fffu;
+                    if (at[m] != kPartial) {
+                        const uint32_t i = at[m] + j;
+                        if (PAIR)
+                            reinterpret_cast<uint4*>(s.stage)[i] = make_uint4(pk, x0[m], x1[m], 0u);
+                        else
+                            reinterpret_cast<uint2*>(s.stage)[i] = make_uint2(pk, x0[m]);
+                    } else {
+                        const uint32_t bk = (v[m] >> kMaxBucketBits) & 0xffffu;
+                        stage_put<PAIR>(s, part, bk, (uint64_t)(hist[bk] + j), pk, __uint_as_float(x0[m]),
+                                        __uint_as_float(x1[m]));
+                    }
                 }
                 __syncthreads();
                 for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
@@ -1721,6 +1739,9 @@ static int ensure_bucketed(Block* b)
         events_max = std::max<int64_t>(events_max, (int64_t)g.nbk * kStageParts * g.cap);
     }
     DTB_TRY(use_bucket_geom(b, 1));
+    // the sorted staging variant indexes the staging array in 32 bits
+    DTB_REQUIRE(events_max < (int64_t)kPartial, "staging array of %lld events exceeds 2^32 - 1",
+                (long long)events_max);
     const size_t esz = b->pair ? 16 : 8;
     DTB_TRY(alloc_n(b, &b->cursor, nbk_max * kStageParts * kCursorStride));
     DTB_TRY(alloc_n(b, &b->ovf, nbk_max * kCursorStride));
