@@ -179,10 +179,12 @@ def test_compaction_and_delivery_bit_exact_on_fixed_spike_vector():
     np.testing.assert_allclose(Jg, Jo, rtol=1e-6, atol=1e-6 * np.abs(Jo).max())
 
 
-@pytest.mark.parametrize("pair,D,graph", [(False, 1, True), (True, 1, False), (False, 6, True), (True, 4, True)])
-def test_network_run_matches_oracle(pair, D, graph):
-    """Plain-block-like recurrent network (fixed in-degree, delays 1..D) against the
-    FP64 push oracle on the same CSR and the same Philox noise."""
+@pytest.mark.parametrize("pair,D,graph,mode", [(False, 1, True, None), (True, 1, False, None), (False, 6, True, None),
+                                              (True, 4, True, None), (False, 63, True, 1), (True, 63, False, 0)])
+def test_network_run_matches_oracle(pair, D, graph, mode):
+    """Plain-block-like recurrent network (fixed in-degree, delays 1..D, up to
+    63, the generator's limit) against the FP64 push oracle on the same CSR and
+    the same Philox noise; mode forces direct (0) or bucketed (1) delivery."""
     from digital_twin_brain_amd import NativeBlock
 
     n, ne, K, steps = 4000, 3200, 100, 600
@@ -190,6 +192,8 @@ def test_network_run_matches_oracle(pair, D, graph):
     props = _plain_like_props(n, ne, rng)
     props[:, 3] = np.arange(n) // 500  # 8 sub-populations
     nb = NativeBlock.fixed_indegree(props, ne, K, max_delay=D, pair_weights=pair, conn_seed=3, noise_seed=9)
+    if mode is not None:
+        nb.set_launch(deliver_mode=mode, deliver_grid=64)
     csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
                w0=nb.get("w0"), w1=nb.get("w1") if pair else None)
     ob = orc.OracleBlock(props, csr, 0.1)
