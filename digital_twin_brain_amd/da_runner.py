@@ -83,7 +83,7 @@ def run_ensemble_da(E=64, voxels=90, windows=20, window_ms=800.0, dt=0.1, hp_lo=
 
 def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=800.0, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7,
            every=5, nrmse_max=0.02, hidden=(512, 512, 512), epochs=None, finetune_epochs=None, per_voxel=True,
-           log=None, tune=True,
+           log=None, tune=True, adapt=0.0,
            **member_kw):
     """Surrogate-assisted DA against simulator-only DA on the same ensemble.
 
@@ -146,7 +146,7 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
     obs = np.stack(obs)
     say(f"surrogate DA: {windows} simulator-only windows, {t_sim.mean():.3f} s each")
     reset_hp()
-    sda = SurrogateDA(da, ws, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs)
+    sda = SurrogateDA(da, ws, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs, adapt=adapt)
     res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
     say(f"surrogate DA: {windows} surrogate-assisted windows, {res['times'].mean():.3f} s each, "
         f"{int(res['simulated'].sum())} simulated")

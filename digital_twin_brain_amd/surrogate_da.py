@@ -134,13 +134,20 @@ class WindowSurrogate:
 
 class SurrogateDA:
     """Every simulated window fine-tunes the surrogate on that window plus an
-    equal-sized replay of earlier rows (`finetune_epochs` passes)."""
+    equal-sized replay of earlier rows (`finetune_epochs` passes).
 
-    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=None, replay=1.0):
+    Fallback: the window after a simulated one is simulated too when the
+    surrogate's NRMSE on it exceeds nrmse_max and, with adapt > 0, also exceeds
+    adapt times the median NRMSE of the last 5 simulated windows (the error
+    floor that a single noisy realisation sets), so that fallbacks follow
+    drift of the surrogate rather than the noise of the simulator."""
+
+    def __init__(self, da, surrogate, every=5, nrmse_max=0.02, finetune_epochs=None, replay=1.0, adapt=0.0):
         self.da = da
         self.sur = surrogate
         self.every = int(every)
         self.nrmse_max = float(nrmse_max)
+        self.adapt = float(adapt)
         if finetune_epochs is None:
             finetune_epochs = 4 if getattr(surrogate, "per_voxel", False) else 40
         self.finetune_epochs = int(finetune_epochs)
@@ -169,8 +176,9 @@ class SurrogateDA:
                 da.get_hidden_state(steps)
                 tail = da._hidden_state[:, :, -6:].double().cpu()
                 e = nrmse(pred[:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
+                floor = float(np.median([x for _, x in errs[-5:]])) if (self.adapt > 0 and errs) else 0.0
                 errs.append((t, e))
-                force = e > self.nrmse_max
+                force = e > max(self.nrmse_max, self.adapt * floor)
                 self.sur.add(X, tail.reshape(E, -1))
                 if self.finetune_epochs:
                     self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t)

@@ -162,20 +162,25 @@ def test_ensemble_da_runner_small():
     assert np.all(np.isfinite(rec["obs"]))
 
 
-def test_surrogate_da_small():
-    """Surrogate-assisted DA: fallback every 5th window (and after any window
-    over the NRMSE limit), finite forecasts, surrogate windows faster than
+@pytest.mark.parametrize("adapt", [0.0, 1.5])
+def test_surrogate_da_small(adapt):
+    """Surrogate-assisted DA: fallback every 5th window, and after any window
+    over the NRMSE limit (with adapt > 0, over adapt x the median of the last 5
+    simulated windows too); finite forecasts, surrogate windows faster than
     simulated ones."""
     from digital_twin_brain_amd.da_runner import run_surrogate_da
 
     out, rec = run_surrogate_da(E=8, voxels=4, train_windows=12, windows=7, window_ms=50.0, hidden=(128, 128), epochs=60,
-                      finetune_epochs=5, n_lo=500, n_hi=3000, k_lo=40, k_hi=80, truth_n=1000, truth_k=60)
+                      finetune_epochs=5, n_lo=500, n_hi=3000, k_lo=40, k_hi=80, truth_n=1000, truth_k=60, adapt=adapt)
     sim = rec["sur"]["simulated"]
     assert sim[0] and sim[5]
     assert out["simulated_windows"] == int(sim.sum()) and len(out["nrmse"]) == out["simulated_windows"]
-    for (t, e), nxt in zip(rec["sur"]["nrmse"], range(7)):
-        if e > 0.02 and t + 1 < 7:
-            assert sim[t + 1]
+    errs = rec["sur"]["nrmse"]
+    for i, (t, e) in enumerate(errs):
+        floor = float(np.median([x for _, x in errs[max(0, i - 5):i]])) if (adapt > 0 and i) else 0.0
+        force = e > max(0.02, adapt * floor)
+        if t + 1 < 7:
+            assert bool(sim[t + 1]) == (force or (t + 1) % 5 == 0)
     assert np.all(np.isfinite(rec["sur"]["forecasts"]))
     times = rec["sur"]["times"]
     if (~sim).any():
