@@ -12,7 +12,10 @@ python bench.py [--gpus 1] [--steps 10000] [--warmup 200]
 Prints one JSON line.  The CPU baseline times the oracle's FP32 restatement
 of the reference algorithm (a full pass over every synapse per step, as
 torch.sparse.mm in models/block.py:76-81) with OpenMP on a bounded sample of
-the same workload.
+the same workload.  Beside the main line's fields: `da` (surrogate-assisted
+against simulator-only DA windows on a 64-member ensemble) and `regions` (the
+90-region, 2e7-neuron share of a whole-brain model, timed the same way);
+--da-windows 0 and --region-steps 0 skip them.
 """
 from __future__ import annotations
 
@@ -92,6 +95,49 @@ def cpu_baseline(n_sample, in_degree, steps_warm, steps_timed):
                         f"{dt:.1f} s, {ev} events"))
 
 
+def build_region_share():
+    """90 regions, one GPU's share of a whole-brain model: 2e7 neurons, 1e10 synapses."""
+    from digital_twin_brain_amd import networks
+
+    nb, props, spec, _ = networks.build_regions()
+    vop = (np.arange(nb.S) // 2).astype(np.int32)  # one BOLD voxel per region
+    nb.attach_bold(steps_per_sample=10, delta_t=1e-3, voxel_of_pop=vop,
+                   voxel_size=np.array([s[0] for s in spec], np.float64), nvox=len(spec))
+    config = {"workload": "90 regions, one GPU's share of a whole-brain model", "neurons": int(nb.n),
+              "synapses": int(nb.nnz), "in_degree": "400 intra + 100 inter (Exp(1) connectome)",
+              "region_sizes": "lognormal sigma 0.8", "delays": "1-40 steps", "dt_ms": 0.1,
+              "external": "Poisson, 20% of input", "bold": "per region, 1 ms", "weights": "compact fp32"}
+    return nb, config
+
+
+def region_share_line(steps, warmup, log):
+    """The region workload timed like the main one (graph-replayed batches,
+    autotuned delivery), reported beside it."""
+    import torch
+    from digital_twin_brain_amd.autotune import tune_delivery
+
+    t0 = time.perf_counter()
+    nb, config = build_region_share()
+    nb.run(warmup, noise_mode=1, graph=True)
+    tune_delivery(nb, dict(noise_mode=1), steps=20, use_cache=False)
+    nb.run(100, noise_mode=1, graph=True)
+    ev = np.zeros(steps, np.uint64)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nb.run(steps, noise_mode=1, graph=True, events=ev)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    out = {"metric": "synaptic events/s", "value": float(ev.sum()) / wall, "unit": "events/s", "steps": steps,
+           "warmup": warmup, "ms_per_step": wall / steps * 1e3, "events_per_step": float(ev.mean()),
+           "config": config, "launch": nb.get_launch()}
+    log(f"region share: {out['value']:.3e} events/s, {out['ms_per_step']:.3f} ms/step "
+        f"({time.perf_counter() - t0:.1f} s with build and tuning)")
+    nb.close()
+    del nb
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +151,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--region-steps", type=int, default=10000,
+                    help="steps of the region workload reported beside the main one (0: skip)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--da-windows", type=int, default=20)
     ap.add_argument("--train-windows", type=int, default=24, help="surrogate training windows (x64 samples)")
@@ -137,14 +185,7 @@ def main():
     graph = not a.no_graph
     t0 = time.perf_counter()
     if a.workload == "regions":
-        nb, props, spec, _ = networks.build_regions()
-        vop = (np.arange(nb.S) // 2).astype(np.int32)  # one BOLD voxel per region
-        nb.attach_bold(steps_per_sample=10, delta_t=1e-3, voxel_of_pop=vop,
-                       voxel_size=np.array([s[0] for s in spec], np.float64), nvox=len(spec))
-        config = {"workload": "90 regions, one GPU's share of a whole-brain model", "neurons": int(nb.n),
-                  "synapses": int(nb.nnz), "in_degree": "400 intra + 100 inter (Exp(1) connectome)",
-                  "region_sizes": "lognormal sigma 0.8", "delays": "1-40 steps", "dt_ms": 0.1,
-                  "external": "Poisson, 20% of input", "bold": "per region, 1 ms", "weights": "compact fp32"}
+        nb, config = build_region_share()
     else:
         nb, props = networks.build_large(n=a.n, in_degree=a.in_degree)
         nb.attach_bold(steps_per_sample=10, delta_t=1e-3)
@@ -232,6 +273,10 @@ def main():
               "detail": c5, "total_s": time.perf_counter() - t0}
         del rec
 
+    regions = None
+    if a.region_steps > 0 and a.workload == "large":
+        regions = region_share_line(a.region_steps - a.region_steps % 10, 200, log)
+
     cpu = None
     if not a.no_cpu and a.workload == "large":
         t0 = time.perf_counter()
@@ -250,6 +295,7 @@ def main():
         "kernels": kernels,
         "launch": launch,
         "da": da,
+        "regions": regions,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
