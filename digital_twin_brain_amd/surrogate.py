@@ -170,7 +170,8 @@ class MLPSurrogate:
                                                         None, 0, self.dH.data_ptr(), i, None, 0, s))
                 nat.check(self.lib.dtb_mlp_relu_backward(B * i, self.dH.data_ptr(), self.H[l].data_ptr(),
                                                          self.G[l - 1].data_ptr(), self.Gb[l - 1].data_ptr(), s))
-        return self.lsum
+        # the partial sums this batch wrote (the buffer is sized for the largest batch)
+        return self.lsum[: (B * no + 255) // 256]
 
     def step(self):
         """One Adam step on every parameter from the last `gradients` call."""
@@ -385,7 +386,8 @@ class GRUSurrogate:
         self._gemm(3 * H, H, TB, self.dGT, TB, self.HbT, TB, None, self.grad["W_hh"], H)
         nat.check(self.lib.dtb_mlp_bias_grad(TB, 3 * H, self.dGi.data_ptr(), self.grad["b_ih"].data_ptr(), s))
         nat.check(self.lib.dtb_mlp_bias_grad(TB, 3 * H, self.dGh.data_ptr(), self.grad["b_hh"].data_ptr(), s))
-        return self.lsum
+        # the partial sums this batch wrote (the buffer is sized for the longest, largest batch)
+        return self.lsum[: (TB * O + 255) // 256]
 
     def step(self):
         s = self._stream()

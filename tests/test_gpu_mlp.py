@@ -197,6 +197,32 @@ def test_surrogate_predict_matches_oracle_and_training_fits():
     assert nrmse < 0.05, nrmse
 
 
+def test_loss_partial_sums_cover_only_the_current_batch():
+    """After a full-size batch fills the loss buffer, a smaller batch's
+    returned partial sums add up to that batch's own squared error (the
+    buffer's tail belongs to the earlier batch)."""
+    from digital_twin_brain_amd.surrogate import GRUSurrogate, MLPSurrogate
+
+    rng = np.random.default_rng(5)
+    sur = MLPSurrogate(20, 10, hidden=(64, 64), max_batch=256, seed=1)
+    sur.gradients(torch.randn(256, 20), torch.randn(256, 10) * 100.0)
+    X, Y = rng.normal(size=(40, 20)), rng.normal(size=(40, 10)).astype(np.float32)
+    ls = sur.gradients(torch.from_numpy(X), torch.from_numpy(Y))
+    out = sur.H[-1][:40, :10].cpu().double().numpy()
+    want = ((out - Y) ** 2).sum()
+    assert abs(float(ls.sum()) - want) <= 1e-4 * want
+
+    gru = GRUSurrogate(10, 6, hidden=32, max_batch=64, max_steps=8, seed=1)
+    gru.gradients(torch.randn(8, 64, 10), torch.randn(8, 64, 6) * 100.0)
+    T, r = 3, 20
+    X, Y = rng.normal(size=(T, r, 10)), rng.normal(size=(T, r, 6)).astype(np.float32)
+    ls = gru.gradients(torch.from_numpy(X), torch.from_numpy(Y))
+    B = (r + 31) // 32 * 32  # rows padded to a multiple of 32
+    out = gru.Y[:T * B].cpu().double().numpy().reshape(T, B, -1)[:, :r, :6]
+    want = ((out - Y) ** 2).sum()
+    assert abs(float(ls.sum()) - want) <= 1e-4 * want
+
+
 def test_gru_gradients_match_fp64_oracle():
     """BPTT of the GRU surrogate on the GPU against the FP64 oracle (itself
     pinned to torch.nn.GRU) with bf16 rounding at the same points."""
