@@ -449,6 +449,40 @@ def test_bucketed_delivery_matches_atomics_including_overflow(pair):
     assert outs[0][2][:20].max() > 3 * 8192 * 3  # some bucket saw more events than its staging room
 
 
+def test_large_block_full_size_event_identity():
+    """The benchmark's single large block at full size (5e6 neurons, 5e9
+    synapses, delays 1-10): with noise-only spikes during 4 steps and then
+    10 silent steps, every spike's whole row is delivered inside the window
+    and nothing else is, so the delivered events equal the spiking rows'
+    out-degrees exactly; the state stays finite."""
+    from digital_twin_brain_amd import networks
+
+    nb, props = networks.build_large()
+    try:
+        D = nb.max_delay
+        outdeg = np.diff(nb.get("rowptr"))
+        # the launch made at creation, then the benchmark's tuned shape (LDS-sorted staging)
+        for launch in ({}, dict(deliver_mode=1, deliver_grid=1024, stage_segs=256, stage_ept=16, stage_variant=2)):
+            if launch:
+                nb.set_launch(**launch)
+            spiked = 0
+            delivered = 0
+            for k in range(4 + D):
+                ev = np.zeros(1, np.uint64)
+                nb.run(1, noise_mode=0, noise_rate=0.002 if k < 4 else 0.0, isolated=True, graph=False, events=ev)
+                delivered += int(ev[0])
+                if k < 4:
+                    sl = nb.get("spike_list")
+                    assert sl.size > 5000
+                    spiked += int(outdeg[sl].sum())
+                else:
+                    assert nb.get("spike_list").size == 0
+            assert delivered == spiked > 0, launch
+        assert np.isfinite(nb.get("V")).all() and np.isfinite(nb.get("J")).all()
+    finally:
+        nb.close()
+
+
 def test_autotuner_contract_and_cache(tmp_path, monkeypatch):
     """The tuner re-measures its finalists, keeps the best of them, leaves
     the block on that shape, and a second call is served from the cache
