@@ -483,6 +483,32 @@ def test_large_block_full_size_event_identity():
         nb.close()
 
 
+def test_region_share_full_size_event_identity():
+    """The 90-region share at full size (2e7 neurons, 1e10 synapses, delays
+    1-40, external drive): the same closed-window identity as for the large
+    block, for direct scattered and LDS-sorted staging."""
+    from digital_twin_brain_amd import networks
+
+    nb, props, spec, _ = networks.build_regions()
+    try:
+        D = nb.max_delay
+        outdeg = np.diff(nb.get("rowptr"))
+        for var in (0, 2):
+            nb.set_launch(deliver_mode=1, deliver_grid=1024, stage_segs=128, stage_ept=16, stage_variant=var)
+            spiked = 0
+            delivered = 0
+            for k in range(3 + D):
+                ev = np.zeros(1, np.uint64)
+                nb.run(1, noise_mode=0, noise_rate=0.001 if k < 3 else 0.0, isolated=True, graph=False, events=ev)
+                delivered += int(ev[0])
+                if k < 3:
+                    spiked += int(outdeg[nb.get("spike_list")].sum())
+            assert delivered == spiked > 0, var
+        assert np.isfinite(nb.get("V")).all()
+    finally:
+        nb.close()
+
+
 def test_autotuner_contract_and_cache(tmp_path, monkeypatch):
     """The tuner re-measures its finalists, keeps the best of them, leaves
     the block on that shape, and a second call is served from the cache
