@@ -1331,33 +1331,33 @@ __global__ void init_state_kernel(int64_t n, const float* __restrict__ props, fl
 __global__ void pack_kernel(int64_t nnz, int64_t n, const int32_t* __restrict__ post, const uint8_t* __restrict__ kind,
                             const uint8_t* __restrict__ delay, SynPack pk, uint32_t* col, uint32_t* err)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nnz) return;
-    const int32_t q = post[e];
-    const uint32_t d = delay ? (uint32_t)delay[e] : 1u;
-    const uint32_t kd = kind[e];
-    if (q < 0 || q >= n || d < 1 || d > (pk.dmask + 1u) || kd > 1u) {
-        atomicAdd(err, 1u);
-        col[e] = 0;
-        return;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t q = post[e];
+        const uint32_t d = delay ? (uint32_t)delay[e] : 1u;
+        const uint32_t kd = kind[e];
+        if (q < 0 || q >= n || d < 1 || d > (pk.dmask + 1u) || kd > 1u) {
+            atomicAdd(err, 1u);
+            col[e] = 0;
+            continue;
+        }
+        col[e] = (uint32_t)q | ((d - 1u) << pk.idx_bits) | (kd << 31);
     }
-    col[e] = (uint32_t)q | ((d - 1u) << pk.idx_bits) | (kd << 31);
 }
 
 __global__ void interleave_kernel(int64_t nnz, const float* w0, const float* w1, float2* w2)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < nnz) w2[e] = make_float2(w0[e], w1[e]);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+        w2[e] = make_float2(w0[e], w1[e]);
 }
 
 __global__ void unpack_kernel(int64_t nnz, const uint32_t* __restrict__ col, SynPack pk, int32_t field, void* out)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nnz) return;
-    const uint32_t c = col[e];
-    if (field == DTB_FIELD_POST) reinterpret_cast<int32_t*>(out)[e] = (int32_t)(c & pk.idx_mask);
-    if (field == DTB_FIELD_KIND) reinterpret_cast<uint8_t*>(out)[e] = (uint8_t)(c >> 31);
-    if (field == DTB_FIELD_DELAY) reinterpret_cast<uint8_t*>(out)[e] = (uint8_t)(((c >> pk.idx_bits) & pk.dmask) + 1);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = col[e];
+        if (field == DTB_FIELD_POST) reinterpret_cast<int32_t*>(out)[e] = (int32_t)(c & pk.idx_mask);
+        if (field == DTB_FIELD_KIND) reinterpret_cast<uint8_t*>(out)[e] = (uint8_t)(c >> 31);
+        if (field == DTB_FIELD_DELAY) reinterpret_cast<uint8_t*>(out)[e] = (uint8_t)(((c >> pk.idx_bits) & pk.dmask) + 1);
+    }
 }
 
 template <bool PAIR>
@@ -1491,6 +1491,9 @@ __global__ void set_props_kernel(int64_t m, const uint32_t* __restrict__ neuron,
 
 // ------------------------------------------------------------- host side
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+// Per-synapse kernels loop over their range: a dispatch holds at most 2^32-1
+// work-items, fewer than the synapses of a large block.
+static inline unsigned grid_syn(int64_t nnz) { return (unsigned)std::min<int64_t>(grid_for(nnz, 256), 1 << 16); }
 
 static int upload(Block* b, void* dst, const void* src, size_t bytes)
 {
@@ -1912,7 +1915,7 @@ static int create_from_csr(Block* b, const dtb_block_desc* d)
             if ((st = upload(b, dkind, d->kind, (size_t)nnz))) break;
             if (d->delay && (st = upload(b, ddelay, d->delay, (size_t)nnz))) break;
             if (hipMemsetAsync(derr, 0, sizeof(uint32_t), b->stream) != hipSuccess) { st = DTB_RUNTIME_ERROR; break; }
-            hipLaunchKernelGGL(pack_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, b->stream, nnz, n,
+            hipLaunchKernelGGL(pack_kernel, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, n,
                                (const int32_t*)dpost, (const uint8_t*)dkind, (const uint8_t*)ddelay, b->pk, b->col,
                                (uint32_t*)derr);
             if (hipMemcpyAsync(&nerr, derr, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
@@ -1935,7 +1938,7 @@ static int create_from_csr(Block* b, const dtb_block_desc* d)
             int s2 = upload(b, d0, d->w0, sizeof(float) * (size_t)nnz);
             if (!s2) s2 = upload(b, d1, d->w1, sizeof(float) * (size_t)nnz);
             if (!s2) {
-                hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, b->stream, nnz, d0, d1,
+                hipLaunchKernelGGL(interleave_kernel, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, d0, d1,
                                    b->w2);
                 if (hipStreamSynchronize(b->stream) != hipSuccess) s2 = DTB_RUNTIME_ERROR;
             }
@@ -2295,8 +2298,13 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
         DTB_TRY(need(sz));
         if (nnz == 0) break;
         DTB_TRY_HIP(hipMalloc(&tmp, (size_t)sz));
-        hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, b->stream, nnz, b->col, b->pk, field,
+        hipLaunchKernelGGL(unpack_kernel, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col, b->pk, field,
                            tmp);
+        if (hipError_t le = hipGetLastError(); le != hipSuccess) {
+            set_error("unpack launch failed: %s", hipGetErrorString(le));
+            st = DTB_RUNTIME_ERROR;
+            break;
+        }
         st = copy_out(b, dst, tmp, (size_t)sz);
         break;
     }

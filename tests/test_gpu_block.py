@@ -483,6 +483,50 @@ def test_large_block_full_size_event_identity():
         nb.close()
 
 
+def test_large_block_full_size_weight_identity():
+    """Full-size linearity: from rest (J = 0), one step's arrivals summed over
+    all 5e6 neurons equal, per receptor group, the weights of that step's
+    spiking rows' delay-1 segments, summed on the GPU from the block's own CSR
+    (int32 fixed-point sums: within half a quantum per event)."""
+    import torch
+
+    from digital_twin_brain_amd import networks
+
+    nb, props = networks.build_large()
+    try:
+        S = nb.get_launch()["accum_fixed"]
+        assert S >= 16
+        ev = np.zeros(1, np.uint64)
+        nb.run(1, noise_mode=0, noise_rate=0.002, isolated=True, graph=False, events=ev)
+        J = nb.get("J").astype(np.float64)
+        dev = torch.device("cuda")
+        spk = torch.from_numpy(nb.get("spike_list").astype(np.int64)).to(dev)
+        w = torch.empty(nb.nnz, dtype=torch.float32, device=dev)
+        dl = torch.empty(nb.nnz, dtype=torch.uint8, device=dev)
+        kd = torch.empty(nb.nnz, dtype=torch.uint8, device=dev)
+        nb.get("w0", out=w)
+        nb.get("delay", out=dl)
+        nb.get("kind", out=kd)
+        rp = torch.from_numpy(nb.get("rowptr")).to(dev)
+        starts = rp[spk]
+        lens = rp[spk + 1] - starts
+        total = int(lens.sum())
+        offs = torch.cumsum(lens, 0) - lens
+        idx = starts.repeat_interleave(lens) + (torch.arange(total, device=dev) - offs.repeat_interleave(lens))
+        ws, d1, k = w[idx].double(), dl[idx] == 1, kd[idx]
+        exc = float(ws[d1 & (k == 0)].sum())
+        inh = float(ws[d1 & (k == 1)].sum())
+        n_ev = int(d1.sum())
+        del w, dl, kd, idx, ws
+        torch.cuda.empty_cache()
+        assert n_ev == int(ev[0]) > 0
+        tol = n_ev * 2.0 ** -(S + 1) + 1e-6 * (exc + inh)
+        for ch, want in ((0, exc), (1, exc), (2, inh), (3, inh)):
+            assert abs(J[ch].sum() - want) <= tol, (ch, J[ch].sum(), want, tol)
+    finally:
+        nb.close()
+
+
 def test_region_share_full_size_event_identity():
     """The 90-region share at full size (2e7 neurons, 1e10 synapses, delays
     1-40, external drive): the same closed-window identity as for the large
