@@ -86,6 +86,9 @@ def lib():
         "dtb_block_get": [_vp, _i32, _vp, _i64, _vp],
         "dtb_block_info": [_vp, _vp],
         "dtb_block_time": [_vp, _vp],
+        "dtb_block_state_bytes": [_vp, _vp],
+        "dtb_block_save_state": [_vp, _vp, _i64],
+        "dtb_block_load_state": [_vp, _vp, _i64],
         "dtb_block_set_props": [_vp, _i64, _vp, _vp, _vp],
         "dtb_block_mul_props_by_pop": [_vp, _i64, _vp, _vp, _vp],
         "dtb_block_gamma_props_by_pop": [_vp, _i64, _vp, _vp, _vp, _vp, _u64],
@@ -123,6 +126,7 @@ def exported_symbols():
     return ["dtb_last_error", "dtb_abi_version", "dtb_block_create", "dtb_block_create_fixed_indegree",
             "dtb_block_create_members", "dtb_block_create_regions",
             "dtb_block_destroy", "dtb_block_run", "dtb_block_get", "dtb_block_info", "dtb_block_time",
+            "dtb_block_state_bytes", "dtb_block_save_state", "dtb_block_load_state",
             "dtb_block_set_props", "dtb_block_mul_props_by_pop", "dtb_block_gamma_props_by_pop",
             "dtb_block_replace_props", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",
             "dtb_block_set_launch", "dtb_block_get_launch", "dtb_block_set_external",

@@ -2338,6 +2338,141 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
     return DTB_OK;
 }
 
+// State snapshot: a 256-byte header, then the device sections below, each
+// starting on a 256-byte boundary.  The sections are everything the next
+// steps read that a step writes; properties, connectivity, the launch shape
+// and the per-batch records are not part of it.
+namespace {
+constexpr uint64_t kSnapMagic = 0x31504e5342544455ull; // "UDTBSNP1"
+constexpr uint32_t kSnapVersion = 1;
+
+struct SnapHeader {
+    uint64_t magic;
+    uint32_t version;
+    uint32_t pair;
+    int64_t n;
+    int32_t D;
+    int32_t nvox;
+    int32_t bold_sps;
+    int32_t pad0;
+    int64_t spk_cap;
+    int64_t steps_done;
+    double t_host;
+    uint64_t seed;
+    int64_t bytes;
+};
+static_assert(sizeof(SnapHeader) <= 256, "snapshot header exceeds its slot");
+
+struct SnapSection {
+    void* p;
+    size_t bytes;
+};
+} // namespace
+
+static std::vector<SnapSection> snap_sections(Block* b)
+{
+    const size_t n = (size_t)b->n, L = (size_t)b->D + 1;
+    std::vector<SnapSection> v{
+        {b->ctl, sizeof(StepCtl)},
+        {b->V, 4 * n},
+        {b->Jp, 16 * n},
+        {b->acc, (b->pair ? 16 : 8) * n},
+        {b->ref_end, 4 * n},
+        {b->last_step, 4 * n},
+        {b->t_last, 8 * n},
+        {b->flags, n},
+        {b->spk, 4 * L * kSubLists * (size_t)b->spk_cap},
+        {b->spk_cnt, 4 * L * kSubLists * kCntStride},
+    };
+    if (b->nvox > 0) v.push_back({b->bold_state, 6 * sizeof(double) * (size_t)b->nvox});
+    return v;
+}
+
+static int64_t snap_bytes(Block* b)
+{
+    int64_t off = 256;
+    for (const SnapSection& x : snap_sections(b)) off += ((int64_t)x.bytes + 255) / 256 * 256;
+    return off;
+}
+
+static int save_state_impl(Block* b, void* dst, int64_t bytes)
+{
+    const int64_t need = snap_bytes(b);
+    DTB_REQUIRE(dst != nullptr, "dst is NULL");
+    DTB_REQUIRE(bytes >= need, "a state snapshot takes %lld bytes, got %lld", (long long)need, (long long)bytes);
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    SnapHeader h{};
+    h.magic = kSnapMagic;
+    h.version = kSnapVersion;
+    h.pair = b->pair ? 1u : 0u;
+    h.n = b->n;
+    h.D = b->D;
+    h.nvox = b->nvox;
+    h.bold_sps = b->bold_sps;
+    h.spk_cap = b->spk_cap;
+    h.steps_done = b->steps_done;
+    h.t_host = b->t_host;
+    h.seed = b->seed;
+    h.bytes = need;
+    char* out = static_cast<char*>(dst);
+    int st = copy_out(b, out, &h, sizeof(h));
+    int64_t off = 256;
+    for (const SnapSection& x : snap_sections(b)) {
+        if (st) break;
+        st = copy_out(b, out + off, x.p, x.bytes);
+        off += ((int64_t)x.bytes + 255) / 256 * 256;
+    }
+    // the header is read from this frame: wait for the copies before returning
+    const hipError_t e = hipStreamSynchronize(b->stream);
+    if (st) return st;
+    DTB_TRY_HIP(e);
+    return DTB_OK;
+}
+
+static int load_state_impl(Block* b, const void* src, int64_t bytes)
+{
+    DTB_REQUIRE(src != nullptr, "src is NULL");
+    DTB_REQUIRE(bytes >= 256, "a state snapshot is at least 256 bytes, got %lld", (long long)bytes);
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    SnapHeader h{};
+    DTB_TRY_HIP(hipMemcpy(&h, src, sizeof(h), hipMemcpyDefault));
+    DTB_REQUIRE(h.magic == kSnapMagic && h.version == kSnapVersion, "not a state snapshot (version %u)", kSnapVersion);
+    const int64_t need = snap_bytes(b);
+    DTB_REQUIRE(h.n == b->n && h.D == b->D && h.pair == (b->pair ? 1u : 0u) && h.spk_cap == b->spk_cap,
+                "snapshot of a block with n=%lld, max delay %d, pair %u; this block has n=%lld, max delay %d, pair %d",
+                (long long)h.n, h.D, h.pair, (long long)b->n, b->D, b->pair ? 1 : 0);
+    DTB_REQUIRE(h.nvox == b->nvox && h.bold_sps == b->bold_sps,
+                "snapshot taken with %d BOLD voxels every %d steps; this block has %d every %d", h.nvox, h.bold_sps,
+                b->nvox, b->bold_sps);
+    DTB_REQUIRE(h.bytes == need && bytes >= need, "snapshot size %lld (buffer %lld), expected %lld",
+                (long long)h.bytes, (long long)bytes, (long long)need);
+    DTB_REQUIRE(h.steps_done >= 0 && h.steps_done < 0x7fff0000ll, "snapshot step counter out of range");
+    const char* in = static_cast<const char*>(src);
+    int64_t off = 256;
+    int st = DTB_OK;
+    for (const SnapSection& x : snap_sections(b)) {
+        if (hipMemcpyAsync(x.p, in + off, x.bytes, hipMemcpyDefault, b->stream) != hipSuccess) {
+            set_error("state snapshot copy failed");
+            st = DTB_RUNTIME_ERROR;
+            break;
+        }
+        off += ((int64_t)x.bytes + 255) / 256 * 256;
+    }
+    const hipError_t e = hipStreamSynchronize(b->stream);
+    if (st) return st;
+    DTB_TRY_HIP(e);
+    if (h.seed != b->seed) {
+        // captured batches hold the seed as a kernel argument
+        for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        b->seed = h.seed;
+    }
+    b->steps_done = h.steps_done;
+    b->t_host = h.t_host;
+    return DTB_OK;
+}
+
 // Property requests grouped by column into [ncols][S] tables.
 template <typename T>
 static int group_requests(Block* b, int64_t m, const uint32_t* pop, const uint32_t* col, const T* val, T none,
@@ -2505,6 +2640,25 @@ int dtb_block_get(dtb_block* b, int32_t field, void* dst, int64_t bytes, int64_t
 {
     DTB_REQUIRE(b != nullptr, "block is NULL");
     return get_impl(b, field, dst, bytes, bytes_out);
+}
+
+int dtb_block_state_bytes(dtb_block* b, int64_t* bytes)
+{
+    DTB_REQUIRE(b != nullptr && bytes != nullptr, "block or bytes is NULL");
+    *bytes = snap_bytes(b);
+    return DTB_OK;
+}
+
+int dtb_block_save_state(dtb_block* b, void* dst, int64_t bytes)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    return save_state_impl(b, dst, bytes);
+}
+
+int dtb_block_load_state(dtb_block* b, const void* src, int64_t bytes)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    return load_state_impl(b, src, bytes);
 }
 
 int dtb_block_info(dtb_block* b, int64_t* info)

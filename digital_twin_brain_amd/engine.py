@@ -155,6 +155,32 @@ class NativeBlock:
         nat.check(nat.lib().dtb_block_time(self._h, nat.ptr(out)))
         return float(out[0])
 
+    # ------------------------------------------------------------ snapshot
+    def state_bytes(self):
+        out = ctypes.c_int64(0)
+        nat.check(nat.lib().dtb_block_state_bytes(self._h, ctypes.byref(out)))
+        return out.value
+
+    def save_state(self, out=None):
+        """Snapshot of the dynamic state (V, J, arrivals, refractory ends,
+        t_last, the spike lists of the delays in flight, step counter, time,
+        noise seed, BOLD state) into `out` (uint8 numpy array or torch tensor,
+        host or device, at least state_bytes() long); returns it."""
+        nbytes = self.state_bytes()
+        if out is None:
+            out = np.zeros(nbytes, dtype=np.uint8)
+        have = out.nbytes if isinstance(out, np.ndarray) else out.numel() * out.element_size()
+        nat.check(nat.lib().dtb_block_save_state(self._h, nat.ptr(out), int(have)))
+        return out
+
+    def load_state(self, src):
+        """Restore a snapshot taken by save_state on a block of the same shape;
+        the block then continues exactly as the saved one did."""
+        if isinstance(src, np.ndarray):
+            src = np.ascontiguousarray(src)
+        have = src.nbytes if isinstance(src, np.ndarray) else src.numel() * src.element_size()
+        nat.check(nat.lib().dtb_block_load_state(self._h, nat.ptr(src), int(have)))
+
     # ----------------------------------------------------------------- run
     def run(self, nsteps, noise_mode=nat.NOISE_SCALAR, noise_rate=0.0, uniforms=None, isolated=False, graph=True,
             freqs=None, vmeans=None, imeans=None, sample_spike=None, sample_v=None, events=None, spikes=None,

@@ -199,6 +199,23 @@ int dtb_block_info(dtb_block* b, int64_t* info);
 /* Simulated time t (models/block.py:58,133), accumulated in double. */
 int dtb_block_time(dtb_block* b, double* t);
 
+/* State checkpoint and restore.  No counterpart in the reference, which keeps
+ * no V, J or t_last across runs (its only offset, RunRequest.iter_offset in
+ * cuda/tools/dist_simulator.cpp:3811-3820, re-aligns input spike times); for
+ * restartable data-assimilation windows (data_assimilation/DataAssimilation.py:402-453).
+ * A snapshot holds everything the next steps read: V, J, the arrivals of the
+ * current step, refractory ends, t_last, spike flags, the spike lists of the
+ * last D+1 steps (the synaptic delays in flight), the step counter and
+ * simulated time, the noise seed (noise is counter-based: seed and step fix
+ * it) and, when attached, the BOLD integrator's state.  Properties,
+ * connectivity and the launch shape are not part of it.  dst and src may be
+ * host or device memory.  Loading checks that the snapshot comes from a
+ * block of the same size, maximum delay, weight form and BOLD attachment;
+ * after it the block continues exactly as the saved one did. */
+int dtb_block_state_bytes(dtb_block* b, int64_t* bytes);
+int dtb_block_save_state(dtb_block* b, void* dst, int64_t bytes);
+int dtb_block_load_state(dtb_block* b, const void* src, int64_t bytes);
+
 /* Point-wise property assignment, cuda/src/stage/property.cpp:17-120:
  * props[neuron[i]][col[i]] = val[i] (col is a property column 0..21). */
 int dtb_block_set_props(dtb_block* b, int64_t m, const uint32_t* neuron, const uint32_t* col, const float* val);
