@@ -163,17 +163,57 @@ class DataAssimilation(simulation):
             state = full.reshape(1, -1, 4)
         self.bold.state_update(state)
 
+    # ---------------------------------------------------------- checkpoint
+    def save_checkpoint(self, path, **extra):
+        """Resumable state between windows: the block (state snapshot, current
+        properties, relative-push factors), the hyper-parameters, the last
+        hidden state and the torch RNG streams that the random walk (CPU) and
+        the EnKF draw (device) use.  `extra` arrays are stored alongside.  The
+        file is replaced atomically."""
+        d = {"block_" + k: v for k, v in self.block_model.state_dict().items()}
+        d.update(hp_log=self._hp_log.numpy(), hp=self._hp.detach().cpu().numpy(),
+                 hidden_state=self._hidden_state.cpu().numpy(), rng_cpu=torch.get_rng_state().numpy(),
+                 rng_device=torch.cuda.get_rng_state(self.block_model._dev).numpy())
+        d.update(extra)
+        tmp = path + ".tmp"
+        with open(tmp, "wb") as f:
+            np.savez(f, **d)
+        os.replace(tmp, path)
+
+    def load_checkpoint(self, path):
+        """Restore what save_checkpoint wrote; returns all stored arrays."""
+        with np.load(path, allow_pickle=False) as z:
+            d = {k: z[k] for k in z.files}
+        self.block_model.load_state_dict({k[len("block_"):]: v for k, v in d.items() if k.startswith("block_")})
+        self._hp_log = torch.from_numpy(d["hp_log"].copy())
+        self._hp = torch.from_numpy(d["hp"].copy())
+        self._hidden_state = torch.from_numpy(d["hidden_state"].copy()).to(self.block_model._dev)
+        torch.set_rng_state(torch.from_numpy(d["rng_cpu"].copy()))
+        torch.cuda.set_rng_state(torch.from_numpy(d["rng_device"].copy()), self.block_model._dev)
+        return d
+
     def da_rest_run(self, bold_real, write_path, observation_times=None, steps=800, bold_sigma=1e-8,
-                    solo_rate=0.8, hp_sigma=0.25, save=True):
+                    solo_rate=0.8, hp_sigma=0.25, save=True, checkpoint=None):
         """data_assimilation/DataAssimilation.py:402-453: one analysis per window.
 
         `bold_real` is [T, voxels per member] or a callable(t) -> [voxels per member]
         (an observation produced while the windows run).  Returns the per-window
-        hidden states before and after the analysis and the window times."""
+        hidden states before and after the analysis and the window times.
+
+        With `checkpoint` (a file path) the run is restartable: the state is
+        saved there after every window, and a run that finds the file resumes
+        after its last window with the same results as an uninterrupted run
+        (bit-for-bit when delivery sums are order-independent, i.e. bucketed
+        delivery with fixed-point accumulation)."""
         T = (bold_real.shape[0] if observation_times is None else observation_times) if not callable(bold_real) \
             else observation_times
         w_save, w_fix = [], []
-        for t in range(T):
+        start = 0
+        if checkpoint is not None and os.path.exists(checkpoint):
+            d = self.load_checkpoint(checkpoint)
+            start = int(d["window"])
+            w_save, w_fix = list(d["w_save"]), list(d["w_fix"])
+        for t in range(start, T):
             t0 = time.perf_counter()
             self.da_evolve(steps, hp_sigma=hp_sigma)
             w_save.append(self._hidden_state.cpu().numpy())
@@ -182,6 +222,9 @@ class DataAssimilation(simulation):
             w_fix.append(self._hidden_state.cpu().numpy())
             torch.cuda.synchronize()
             self.timings.append(time.perf_counter() - t0)
+            if checkpoint is not None:
+                self.save_checkpoint(checkpoint, window=np.int64(t + 1), w_save=np.stack(w_save),
+                                     w_fix=np.stack(w_fix))
         w_save = np.stack(w_save)
         if save:
             os.makedirs(write_path, exist_ok=True)

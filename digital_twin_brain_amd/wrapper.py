@@ -211,6 +211,21 @@ class LocalBlockWrapper:
         self._gamma_seed += 1
         return True
 
+    # ---------------------------------------------------------- checkpoint
+    def state_dict(self):
+        """Everything later steps and property pushes depend on: the block's
+        state snapshot, its current properties and the factors that relative
+        property pushes divide by (numpy arrays)."""
+        return {"block_state": self.nb.save_state(), "props": self.nb.get("props"),
+                "last_hyper_parameter": self._last_hyper_parameter.numpy().copy(),
+                "gamma_seed": np.int64(self._gamma_seed)}
+
+    def load_state_dict(self, d):
+        self.nb.replace_props(d["props"])
+        self.nb.load_state(d["block_state"])
+        self._last_hyper_parameter = torch.from_numpy(np.array(d["last_hyper_parameter"], dtype=np.float32))
+        self._gamma_seed = int(d["gamma_seed"])
+
     def update_property(self, property_idx, property_weight, bid=None):
         """cuda/python/dist_blockwrapper_pytorch.py:322-382: point-wise assignment."""
         assert property_weight.dtype == torch.float32

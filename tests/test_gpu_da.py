@@ -152,6 +152,54 @@ def test_da_window_analysis_matches_oracle_enkf():
                                rtol=1e-12)
 
 
+def test_da_rest_run_resumes_from_a_checkpoint(tmp_path):
+    """A DA run stopped after 2 of 4 windows and resumed from its checkpoint
+    by a freshly built block and DataAssimilation (RNG streams scrambled)
+    gives the uninterrupted run's hidden states, before and after every
+    analysis, bit-for-bit: bucketed delivery with fixed-point sums makes the
+    windows independent of atomic order."""
+    from digital_twin_brain_amd import networks
+    from digital_twin_brain_amd.data_assimilation import DataAssimilation
+
+    E, V = 4, 3
+
+    def make():
+        mem = networks.ensemble_members(E=E, seed=1, n_lo=400, n_hi=1500, k_lo=30, k_hi=60, truth_n=600, truth_k=40)
+        props = networks.ensemble_props(mem, voxels=V)
+        nb = networks.NativeBlock.members(props, mem, max_delay=3, stats_exclude_blocked=True)
+        nb.set_launch(deliver_mode=1, deliver_grid=256)
+        assert nb.get_launch()["accum_fixed"] >= 16
+        da = DataAssimilation(network=nb, column=False, ensemble=E, dt=0.1, bold_steps=10, da_populations=E * 2 * V,
+                              da_voxels=E * V, write_path=str(tmp_path))
+        low, high = torch.full((2 * V, 2), 0.5), torch.full((2 * V, 2), 2.0)
+        hp = da.hp_random_initialize(low, high, rng=np.random.default_rng(0))
+        da.hp_index2hidden_state()
+        da.da_property_initialize([10, 12], None, hp)
+        return da, nb
+
+    y = np.random.default_rng(5).uniform(0.01, 0.03, size=(4, V))
+    kw = dict(steps=200, bold_sigma=1e-6, hp_sigma=0.25, save=False)
+    da, nb = make()
+    torch.manual_seed(11)
+    w_all, f_all, _ = da.da_rest_run(y, str(tmp_path), **kw)
+    nb.close()
+    assert not np.array_equal(w_all[0], w_all[3])
+
+    ck = str(tmp_path / "da_checkpoint.npz")
+    da, nb = make()
+    torch.manual_seed(11)
+    w2, f2, _ = da.da_rest_run(y[:2], str(tmp_path), checkpoint=ck, **kw)
+    nb.close()
+    assert np.array_equal(w2, w_all[:2]) and np.array_equal(f2, f_all[:2])
+
+    da, nb = make()
+    torch.manual_seed(12345)
+    w3, f3, _ = da.da_rest_run(y, str(tmp_path), checkpoint=ck, **kw)
+    assert np.array_equal(w3, w_all) and np.array_equal(f3, f_all)
+    assert nb.steps == 4 * 200
+    nb.close()
+
+
 def test_ensemble_da_runner_small():
     from digital_twin_brain_amd.da_runner import run_ensemble_da
 
