@@ -208,7 +208,8 @@ int dtb_block_time(dtb_block* b, double* t);
  * last D+1 steps (the synaptic delays in flight), the step counter and
  * simulated time, the noise seed (noise is counter-based: seed and step fix
  * it) and, when attached, the BOLD integrator's state.  Properties,
- * connectivity and the launch shape are not part of it.  dst and src may be
+ * connectivity, the external drive's settings and the launch shape are not
+ * part of it.  dst and src may be
  * host or device memory.  Loading checks that the snapshot comes from a
  * block of the same size, maximum delay, weight form and BOLD attachment;
  * after it the block continues exactly as the saved one did. */
