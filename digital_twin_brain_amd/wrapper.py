@@ -69,6 +69,53 @@ def load_block_dir(path):
                        w1=None if np.array_equal(w0, w1) else w1)
 
 
+def save_block_dir(path, props, csr, sizes):
+    """Write a network as reference block_{i}.npz files (the fields and dtypes
+    of generation/make_block.py:473-479, read back by
+    cuda/src/configuration.cpp:586-594,783-792,972-975 and load_block_dir).
+
+    `csr` is a presynaptic-major CSR dict (rowptr, post, kind, w0, optional w1,
+    optional delay); `sizes` splits the neurons into consecutive blocks.  Each
+    block holds the synapses onto its own neurons, ordered by (post, pre):
+    output_neuron_idx (u32, local), input_block_idx (i16, source block minus
+    this block), input_neuron_idx (u32, local in the source block),
+    input_channel_offset (u8: 0 excitatory, 2 inhibitory), weight (f32 [nnz,2])
+    and property (f32 [n,22]).  The format has no delay field, so delays other
+    than 1 are refused."""
+    props = np.asarray(props, dtype=np.float32)
+    sizes = np.asarray(sizes, dtype=np.int64)
+    n = props.shape[0]
+    if sizes.ndim != 1 or (sizes <= 0).any() or int(sizes.sum()) != n:
+        raise ValueError(f"block sizes must be positive and sum to {n}")
+    if len(sizes) > 32767:
+        raise ValueError("at most 32767 blocks (input_block_idx is int16)")
+    rowptr = np.asarray(csr["rowptr"], dtype=np.int64)
+    post = np.asarray(csr["post"], dtype=np.int64)
+    if rowptr.shape != (n + 1,) or rowptr[-1] != post.shape[0]:
+        raise ValueError("rowptr does not match props and post")
+    delay = csr.get("delay")
+    if delay is not None and (np.asarray(delay) != 1).any():
+        raise ValueError("block npz files have no delay field: every delay must be 1")
+    kind = np.asarray(csr["kind"]).astype(np.uint8)
+    w0 = np.asarray(csr["w0"], dtype=np.float32)
+    w1 = w0 if csr.get("w1") is None else np.asarray(csr["w1"], dtype=np.float32)
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
+    bases = np.concatenate([[0], np.cumsum(sizes)])
+    blk_pre = np.searchsorted(bases, pre, side="right") - 1
+    order = np.lexsort((pre, post))
+    os.makedirs(path, exist_ok=True)
+    bounds = np.searchsorted(post[order], bases)
+    for i in range(len(sizes)):
+        sel = order[bounds[i]:bounds[i + 1]]
+        np.savez(os.path.join(path, f"block_{i}.npz"),
+                 property=np.ascontiguousarray(props[bases[i]:bases[i + 1]]),
+                 output_neuron_idx=(post[sel] - bases[i]).astype(np.uint32),
+                 input_block_idx=(blk_pre[sel] - i).astype(np.int16),
+                 input_neuron_idx=(pre[sel] - bases[blk_pre[sel]]).astype(np.uint32),
+                 input_channel_offset=(2 * kind[sel]).astype(np.uint8),
+                 weight=np.ascontiguousarray(np.stack([w0[sel], w1[sel]], axis=1)))
+
+
 class LocalBlockWrapper:
     """`BlockWrapper` surface over an in-process libdtb block."""
 
