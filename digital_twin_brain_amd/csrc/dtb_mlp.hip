@@ -32,6 +32,13 @@ __device__ inline uint16_t f2bf(float f)
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // act: 0 identity, 1 ReLU
+// A workgroup of four wavefronts (2x2) computes a (64 TM) x (64 TN) tile of C;
+// each wavefront TM x TN sub-tiles of 32x32, one v_mfma_f32_32x32x16_bf16 per
+// sub-tile and 16-wide K step, so each fragment it loads feeds TN (or TM)
+// products.  The fragments of the next K step are loaded before the products
+// of the current one.  M and N are multiples of 32 only: a sub-tile past the
+// edge loads clamped rows (inside A and B) and is not stored.
+template <int TM, int TN>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(int32_t M, int32_t N, int32_t K, const uint16_t* __restrict__ A,
                                                       int64_t lda, const uint16_t* __restrict__ B, int64_t ldb,
                                                       const float* __restrict__ bias, int32_t act,
@@ -40,30 +47,65 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int32_t M, int32_t N, int3
 {
     const int wid = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int m0 = blockIdx.y * 64 + (wid >> 1) * 32;
-    const int n0 = blockIdx.x * 64 + (wid & 1) * 32;
-    if (m0 >= M || n0 >= N) return; // whole 32x32 sub-tiles only (M, N multiples of 32)
+    const int m0 = blockIdx.y * (64 * TM) + (wid >> 1) * (32 * TM);
+    const int n0 = blockIdx.x * (64 * TN) + (wid & 1) * (32 * TN);
+    if (m0 >= M || n0 >= N) return; // wavefront-uniform
     const int r = lane & 31;
     const int kh = (lane >> 5) * 8;
-    const uint16_t* pa = A + (int64_t)(m0 + r) * lda + kh;
-    const uint16_t* pb = B + (int64_t)(n0 + r) * ldb + kh;
-    f32x16 acc;
+    const uint16_t* pa[TM];
+    const uint16_t* pb[TN];
 #pragma unroll
-    for (int x = 0; x < 16; ++x) acc[x] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + k0);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + k0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    for (int i = 0; i < TM; ++i) pa[i] = A + (int64_t)(min(m0 + 32 * i, M - 32) + r) * lda + kh;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) pb[j] = B + (int64_t)(min(n0 + 32 * j, N - 32) + r) * ldb + kh;
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int x = 0; x < 16; ++x) acc[i][j][x] = 0.0f;
+    bf16x8 a[TM], b[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) a[i] = *reinterpret_cast<const bf16x8*>(pa[i]);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) b[j] = *reinterpret_cast<const bf16x8*>(pb[j]);
+    for (int k0 = 16; k0 < K; k0 += 16) {
+        bf16x8 an[TM], bn[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) an[i] = *reinterpret_cast<const bf16x8*>(pa[i] + k0);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bn[j] = *reinterpret_cast<const bf16x8*>(pb[j] + k0);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = an[i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = bn[j];
     }
-    const int col = n0 + r;
-    const float bv = bias ? bias[col] : 0.0f;
 #pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        const int row = m0 + 8 * (x >> 2) + 4 * (lane >> 5) + (x & 3);
-        float v = acc[x] + bv;
-        if (act == 1) v = v > 0.0f ? v : 0.0f;
-        if (C) C[(int64_t)row * ldc + col] = v;
-        if (Cb) Cb[(int64_t)row * ldcb + col] = f2bf(v);
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        if (n0 + 32 * j >= N) continue;
+        const int col = n0 + 32 * j + r;
+        const float bv = bias ? bias[col] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            if (m0 + 32 * i >= M) continue;
+#pragma unroll
+            for (int x = 0; x < 16; ++x) {
+                const int row = m0 + 32 * i + 8 * (x >> 2) + 4 * (lane >> 5) + (x & 3);
+                float v = acc[i][j][x] + bv;
+                if (act == 1) v = v > 0.0f ? v : 0.0f;
+                if (C) C[(int64_t)row * ldc + col] = v;
+                if (Cb) Cb[(int64_t)row * ldcb + col] = f2bf(v);
+            }
+        }
     }
 }
 
@@ -249,9 +291,18 @@ int dtb_mlp_gemm_bf16_nt(int32_t M, int32_t N, int32_t K, const void* A, int64_t
     DTB_REQUIRE(act == 0 || act == 1, "act must be 0 (identity) or 1 (ReLU)");
     DTB_REQUIRE((reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0,
                 "A and B must be 16-byte aligned");
-    dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
-    hipLaunchKernelGGL(gemm_nt_kernel, grid, dim3(256), 0, (hipStream_t)stream, M, N, K, (const uint16_t*)A, lda,
-                       (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
+    // 128x128 workgroup tiles while they still give every compute unit a
+    // workgroup, else 64x64 (the surrogate's own products are that small)
+    const int64_t big = (int64_t)((N + 127) / 128) * ((M + 127) / 128);
+    if (big >= 256) {
+        dim3 grid((unsigned)((N + 127) / 128), (unsigned)((M + 127) / 128));
+        hipLaunchKernelGGL((gemm_nt_kernel<2, 2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
+                           (const uint16_t*)A, lda, (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
+    } else {
+        dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+        hipLaunchKernelGGL((gemm_nt_kernel<1, 1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
+                           (const uint16_t*)A, lda, (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
+    }
     DTB_TRY_HIP(hipGetLastError());
     return DTB_OK;
 }

@@ -423,3 +423,52 @@ class GRUSurrogate:
 
     def params_numpy(self):
         return {k: v.cpu().double().numpy() for k, v in self.p.items()}
+
+
+def gemm_throughput(M, N, K, reps=50, warmup=5, device=0):
+    """Time dtb_mlp_gemm_bf16_nt (C = A·Bᵀ, bf16 operands, FP32 out) on
+    random operands: HIP events on the stream the kernel is launched on.
+    The product is first checked against torch in FP32 (bf16 inputs are
+    exact in FP32; only the summation order differs).  Returns a dict with
+    the average microseconds per launch and TFLOP/s (2·M·N·K per launch)."""
+    nat.require_gpu()
+    if M % 32 or N % 32 or K % 16 or min(M, N, K) <= 0:
+        raise ValueError("M and N must be positive multiples of 32 and K of 16")
+    lib = nat.lib()
+    dev = torch.device("cuda", device)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    A = torch.randn(M, K, generator=g).to(torch.bfloat16).to(dev)
+    B = torch.randn(N, K, generator=g).to(torch.bfloat16).to(dev)
+    C = torch.empty(M, N, dtype=torch.float32, device=dev)
+    s = torch.cuda.current_stream(dev)
+
+    def launch():
+        nat.check(lib.dtb_mlp_gemm_bf16_nt(M, N, K, A.data_ptr(), K, B.data_ptr(), K, None, 0, C.data_ptr(), N, None,
+                                           0, s.cuda_stream))
+
+    launch()
+    ref = A.float() @ B.float().T
+    err = float((C - ref).abs().max() / ref.abs().max())
+    if not err < 1e-5:
+        raise RuntimeError(f"GEMM {M}x{N}x{K} differs from torch: relative error {err:.3g}")
+    for _ in range(warmup):
+        launch()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        launch()
+    e1.record(s)
+    e1.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    return {"M": M, "N": N, "K": K, "us": us, "tflops": 2.0 * M * N * K / us * 1e-6, "max_rel_err": err}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="surrogate GEMM throughput")
+    ap.add_argument("--gemm", type=int, nargs=3, action="append", metavar=("M", "N", "K"), required=True)
+    ap.add_argument("--reps", type=int, default=50)
+    for shape in ap.parse_args().gemm:
+        print(json.dumps(gemm_throughput(*shape, reps=ap.parse_args().reps)), flush=True)
