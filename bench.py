@@ -13,7 +13,8 @@ Prints one JSON line.  The CPU baseline times the oracle's FP32 restatement
 of the reference algorithm (a full pass over every synapse per step, as
 torch.sparse.mm in models/block.py:76-81) with OpenMP on a bounded sample of
 the same workload.  Beside the main line's fields: `da` (surrogate-assisted
-against simulator-only DA windows on a 64-member ensemble) and `regions` (the
+against simulator-only DA windows on a 64-member ensemble, with the
+surrogate's matrix-core GEMM throughput in `da.surrogate_gemm`) and `regions` (the
 90-region, 2e7-neuron share of a whole-brain model, timed the same way);
 --da-windows 0 and --region-steps 0 skip them.
 """
@@ -32,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix-core peak
 B_SEGMENT = 16  # rowptr begin (8) + delay-segment ends (4) + spike-list entry (4) per delivered (row, delay)
 B_SYNAPSE = 8   # packed int32 target + fp32 weight per synaptic event
 
@@ -272,6 +274,17 @@ def main():
               "gain": c5["gain"], "config": "64 uneven members (2k-200k neurons, in-degree 100-500) + truth, 90 voxels, 800 ms windows, EnKF FP64",
               "detail": c5, "total_s": time.perf_counter() - t0}
         del rec
+        # the surrogate's matrix-core product, live: one layer of the MLP and a large square product
+        from digital_twin_brain_amd.surrogate import gemm_throughput
+
+        g_small = gemm_throughput(256, 512, 512, reps=50)
+        g_big = gemm_throughput(8192, 8192, 8192, reps=10)
+        da["surrogate_gemm"] = {"kernel": "gemm_nt_kernel (bf16 MFMA, FP32 accumulate)", "bound": "mfma",
+                                "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "layer_256x512x512_us": g_small["us"],
+                                "square_8192_tflops": g_big["tflops"],
+                                "square_8192_frac": g_big["tflops"] / MFMA_BF16_PEAK_TFLOPS}
+        log(f"surrogate GEMM: 256x512x512 {g_small['us']:.1f} us, 8192^3 {g_big['tflops']:.0f} TFLOP/s")
+        torch.cuda.empty_cache()
 
     regions = None
     if a.region_steps > 0 and a.workload == "large":
