@@ -268,8 +268,10 @@ class LocalBlockWrapper:
                 "gamma_seed": np.int64(self._gamma_seed)}
 
     def load_state_dict(self, d):
-        self.nb.replace_props(d["props"])
+        # the snapshot is validated before anything changes; the properties
+        # then re-derive the refractory ends from the restored t_last
         self.nb.load_state(d["block_state"])
+        self.nb.replace_props(d["props"])
         self._last_hyper_parameter = torch.from_numpy(np.array(d["last_hyper_parameter"], dtype=np.float32))
         self._gamma_seed = int(d["gamma_seed"])
 
