@@ -39,12 +39,14 @@ def load_block_dir(path):
     if not ids or ids != list(range(len(ids))):
         raise FileNotFoundError(f"no contiguous block_0..block_k .npz files in {path}")
     files = [np.load(os.path.join(path, f"block_{i}.npz")) for i in ids]
+    for i, f in enumerate(files):
+        if "src_timestamp" in f.files or "src_neuron_idx" in f.files:
+            raise NotImplementedError(f"block_{i}.npz is an input block (src_timestamp, src_neuron_idx): "
+                                      "timed spike injection is not supported yet")
     sizes = [f["property"].shape[0] for f in files]
     bases = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     props, pre, post, kind, w0, w1 = [], [], [], [], [], []
     for i, f in enumerate(files):
-        if "src_timestamp" in f.files or "src_neuron_idx" in f.files:
-            raise NotImplementedError("input blocks with src_timestamp are not supported yet")
         props.append(np.asarray(f["property"], dtype=np.float32)[:, :nat.NPROP])
         blk = np.asarray(f["input_block_idx"]).astype(np.int64)
         pre.append(bases[i + blk] + np.asarray(f["input_neuron_idx"]).astype(np.int64))

@@ -70,3 +70,16 @@ def test_writer_refuses_what_the_format_cannot_hold(tmp_path):
     with pytest.raises(ValueError, match="sum"):
         save_block_dir(str(tmp_path), props, ref, [100, 100])
     save_block_dir(str(tmp_path), props, ref, [n])
+
+
+def test_reader_refuses_input_blocks(tmp_path):
+    """Input blocks (src_timestamp, src_neuron_idx: timed spike injection,
+    cuda/src/configuration.cpp:753-762) are not supported: the reader says so
+    instead of loading a network without them."""
+    n = 200
+    ref = orc.gen_fixed_indegree(n, 160, 10, 1, 0.0, 1.0, False, seed=2)
+    save_block_dir(str(tmp_path), np.zeros((n, 22), np.float32), ref, [n])
+    np.savez(os.path.join(tmp_path, "block_1.npz"), src_timestamp=np.arange(4, dtype=np.uint32),
+             src_neuron_idx=np.zeros(4, np.uint32))
+    with pytest.raises(NotImplementedError, match="src_timestamp"):
+        load_block_dir(str(tmp_path))
