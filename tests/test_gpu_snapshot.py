@@ -160,6 +160,34 @@ def test_restore_with_external_drive():
         nb.close()
 
 
+def test_restore_full_size_block():
+    """The benchmark's 5e9-synapse block in its benchmark launch shape
+    (bucketed, LDS-sorted staging, fixed-point sums): a device snapshot
+    restores it, and 100 further steps repeat bit-for-bit."""
+    from digital_twin_brain_amd import networks
+
+    nb, _ = networks.build_large()
+    try:
+        nb.set_launch(deliver_mode=1, deliver_grid=1024, stage_segs=256, stage_ept=16, stage_variant=2)
+        assert nb.get_launch()["accum_fixed"] >= 16
+        nb.run(100, noise_mode=1, graph=True)
+        snap = nb.save_state(torch.empty(nb.state_bytes(), dtype=torch.uint8, device="cuda"))
+
+        def window():
+            ev = np.zeros(100, np.uint64)
+            fr = np.zeros((100, nb.S), np.uint32)
+            nb.run(100, noise_mode=1, graph=True, events=ev, freqs=fr)
+            return ev, fr, nb.get("V"), nb.get("J"), nb.get("spike_list")
+
+        first = window()
+        assert first[0].min() > 1_000_000
+        nb.load_state(snap)
+        for x, y in zip(first, window()):
+            assert np.array_equal(x, y)
+    finally:
+        nb.close()
+
+
 def test_load_rejects_foreign_or_short_snapshots():
     a = _block(False, 0, noise_seed=4)
     b = _block(False, 0, noise_seed=4, bold=True)
