@@ -109,6 +109,61 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int32_t M, int32_t N, int3
     }
 }
 
+// Small products (too few 64x64 tiles to fill the chip): a workgroup computes
+// one 32x32 tile of C and its four wavefronts split K in four consecutive
+// ranges; wavefronts 1-3 hand their partial sums to wavefront 0 through LDS,
+// which adds them in wavefront order (a fixed order: results do not depend on
+// timing).  Each wavefront's serial chain of K steps is a quarter as long.
+__global__ __launch_bounds__(256) void gemm_nt_splitk_kernel(int32_t M, int32_t N, int32_t K,
+                                                             const uint16_t* __restrict__ A, int64_t lda,
+                                                             const uint16_t* __restrict__ B, int64_t ldb,
+                                                             const float* __restrict__ bias, int32_t act,
+                                                             float* __restrict__ C, int64_t ldc,
+                                                             uint16_t* __restrict__ Cb, int64_t ldcb)
+{
+    __shared__ float red[3][16][64];
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int m0 = blockIdx.y * 32;
+    const int n0 = blockIdx.x * 32;
+    const int r = lane & 31;
+    const int kh = (lane >> 5) * 8;
+    const int steps = K / 16;
+    const int per = (steps + 3) / 4;
+    const int kb = min(wid * per, steps) * 16;
+    const int ke = min((wid + 1) * per, steps) * 16;
+    const uint16_t* pa = A + (int64_t)(m0 + r) * lda + kh;
+    const uint16_t* pb = B + (int64_t)(n0 + r) * ldb + kh;
+    f32x16 acc;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) acc[x] = 0.0f;
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + k0);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + k0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (wid > 0) {
+#pragma unroll
+        for (int x = 0; x < 16; ++x) red[wid - 1][x][lane] = acc[x];
+    }
+    __syncthreads();
+    if (wid != 0) return;
+#pragma unroll
+    for (int w = 0; w < 3; ++w)
+#pragma unroll
+        for (int x = 0; x < 16; ++x) acc[x] += red[w][x][lane];
+    const int col = n0 + r;
+    const float bv = bias ? bias[col] : 0.0f;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+        const int row = m0 + 8 * (x >> 2) + 4 * (lane >> 5) + (x & 3);
+        float v = acc[x] + bv;
+        if (act == 1) v = v > 0.0f ? v : 0.0f;
+        if (C) C[(int64_t)row * ldc + col] = v;
+        if (Cb) Cb[(int64_t)row * ldcb + col] = f2bf(v);
+    }
+}
+
 // dZ = dY * relu'(Z)  (Z > 0), written in FP32 and bf16
 __global__ void relu_backward_kernel(int64_t n, const float* __restrict__ dY, const float* __restrict__ Z,
                                      float* __restrict__ dZ, uint16_t* __restrict__ dZb)
@@ -292,15 +347,20 @@ int dtb_mlp_gemm_bf16_nt(int32_t M, int32_t N, int32_t K, const void* A, int64_t
     DTB_REQUIRE((reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0,
                 "A and B must be 16-byte aligned");
     // 128x128 workgroup tiles while they still give every compute unit a
-    // workgroup, else 64x64 (the surrogate's own products are that small)
+    // workgroup, else 64x64, else 32x32 tiles with K split over the wavefronts
+    // (the surrogate's own products are that small)
     const int64_t big = (int64_t)((N + 127) / 128) * ((M + 127) / 128);
     if (big >= 256) {
         dim3 grid((unsigned)((N + 127) / 128), (unsigned)((M + 127) / 128));
         hipLaunchKernelGGL((gemm_nt_kernel<2, 2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
                            (const uint16_t*)A, lda, (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
-    } else {
+    } else if ((int64_t)((N + 63) / 64) * ((M + 63) / 64) >= 256) {
         dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
         hipLaunchKernelGGL((gemm_nt_kernel<1, 1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
+                           (const uint16_t*)A, lda, (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
+    } else {
+        dim3 grid((unsigned)(N / 32), (unsigned)(M / 32));
+        hipLaunchKernelGGL(gemm_nt_splitk_kernel, grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
                            (const uint16_t*)A, lda, (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
     }
     DTB_TRY_HIP(hipGetLastError());

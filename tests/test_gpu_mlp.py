@@ -33,10 +33,11 @@ def gemm(A, B, bias=None, act=0, want_bf16=False, lda=None, ldb=None):
     return (C, Cb) if want_bf16 else C
 
 
-# the last two take the 128x128 workgroup tiles (at least 256 of them), with a
-# 32-row and a 64-column edge band
-@pytest.mark.parametrize("M,N,K", [(32, 32, 16), (96, 160, 48), (256, 512, 384), (64, 96, 1040), (2080, 2080, 64),
-                                   (2048, 2112, 48)])
+# the first four take 32x32 tiles with K split over the wavefronts (K = 16 gives
+# three wavefronts an empty range); 1024x1024 takes 64x64 tiles; the last two
+# take 128x128 tiles (at least 256 of them), with a 32-row and a 64-column edge band
+@pytest.mark.parametrize("M,N,K", [(32, 32, 16), (96, 160, 48), (256, 512, 384), (64, 96, 1040), (1024, 1024, 48),
+                                   (2080, 2080, 64), (2048, 2112, 48)])
 def test_gemm_bit_exact_on_integer_operands(M, N, K):
     g = torch.Generator().manual_seed(M * 7 + N + K)
     A = torch.randint(-4, 5, (M, K), generator=g).to(torch.bfloat16)
