@@ -93,6 +93,8 @@ def lib():
         "dtb_block_mul_props_by_pop": [_vp, _i64, _vp, _vp, _vp],
         "dtb_block_gamma_props_by_pop": [_vp, _i64, _vp, _vp, _vp, _vp, _u64],
         "dtb_block_replace_props": [_vp, _vp],
+        "dtb_block_set_weights": [_vp, _i64, _vp, _vp, _vp, _vp],
+        "dtb_block_set_sources": [_vp, _i64, _vp, _vp, _i64, _vp, _vp],
         "dtb_block_set_samples": [_vp, _i64, _vp],
         "dtb_block_attach_bold": [_vp, _vp, _i32, _i32, _vp, _vp],
         "dtb_block_set_launch": [_vp, _i32, _i32],
@@ -128,7 +130,7 @@ def exported_symbols():
             "dtb_block_destroy", "dtb_block_run", "dtb_block_get", "dtb_block_info", "dtb_block_time",
             "dtb_block_state_bytes", "dtb_block_save_state", "dtb_block_load_state",
             "dtb_block_set_props", "dtb_block_mul_props_by_pop", "dtb_block_gamma_props_by_pop",
-            "dtb_block_replace_props", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",
+            "dtb_block_replace_props", "dtb_block_set_weights", "dtb_block_set_sources", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",
             "dtb_block_set_launch", "dtb_block_get_launch", "dtb_block_set_external",
             "dtb_bold_step_f32", "dtb_bold_step_f64",
             "dtb_enkf_analysis_f64",

@@ -12,7 +12,16 @@ Differences that follow from running on the GPU:
 - the state is FP32 and time is accumulated in double, so the refractory test
   `t >= t_last + T_ref` (models/block.py:100) is decided exactly as in a
   float64 run of the reference;
-- `src` forced-spike inputs are not supported yet (NotImplementedError).
+- `src` forced spikes (models/block.py:38-50,104-109): the sources are the
+  neurons with no afferents, and column t of `src` drives the step whose
+  0-based index is t, as in the reference.  The reference also reads column
+  iter + 1 after advancing iter, for a write to V that the next line
+  discards (:105-107), so it raises IndexError at its T-th step with T
+  columns; this block runs all T steps and raises IndexError at step T + 1;
+- the weights are stored per channel (pair form), so `update_conn_weight`
+  can set AMPA and NMDA (or GABAa and GABAb) of one synapse independently.
+  Setting a w_uij entry that has no synapse yet rebuilds the network on the
+  GPU with that synapse added; the dynamic state carries over exactly.
 """
 from __future__ import annotations
 
@@ -88,25 +97,42 @@ class block:
     """GPU LIF block with the interface of models/block.py:5-177."""
 
     def __init__(self, node_property, w_uij, delta_t=1, src=None, seed=0, device=0):
-        if src is not None:
-            raise NotImplementedError("forced-spike `src` inputs (models/block.py:38-50) are not supported yet")
         assert len(w_uij.shape) == 3
         self.w_uij = w_uij
         self.src = src
-        self.src_neuron = None
-        self.iter = None
         self.delta_t = delta_t
         self._out_device = w_uij.device
         self._dtype = torch.float32
+        self._seed = int(seed)
+        self._device = device
         csr = csr_from_w(w_uij)
-        self._csr = csr
+        if csr["w1"] is None:
+            csr["w1"] = csr["w0"].copy()
         props = node_property.detach().cpu().numpy().astype(np.float32)
         self._nb = NativeBlock.from_csr(props, csr["rowptr"], csr["post"], csr["kind"], csr["w0"], csr["w1"],
-                                        dt=float(delta_t), noise_seed=int(seed), device=device)
+                                        dt=float(delta_t), noise_seed=self._seed, device=device)
         self.N = self._nb.n
+        if src is None:
+            self.src_neuron = None
+            self.iter = None
+        else:
+            # sources: the neurons no synapse targets (models/block.py:42-48)
+            has_in = np.zeros(self.N, bool)
+            has_in[csr["post"]] = True
+            src_neuron = np.flatnonzero(~has_in)
+            sched = torch.as_tensor(src).detach().cpu().numpy().astype(bool)
+            assert sched.ndim == 2 and src_neuron.shape[0] == sched.shape[0]
+            self.src_neuron = torch.from_numpy(src_neuron.astype(np.int64))
+            self.iter = 0
+            self._sched = sched
+            self._apply_sources(self._nb)
         self._set_property_attrs(node_property)
         self._freq = np.zeros(self._nb.S, np.uint32)
         self._vmean = np.zeros(self._nb.S, np.float32)
+
+    def _apply_sources(self, nb):
+        k, t = np.nonzero(self._sched)
+        nb.set_sources(self.src_neuron.numpy(), steps=t, neurons=self.src_neuron.numpy()[k])
 
     # reference: update_property (models/block.py:152-170)
     def _set_property_attrs(self, node_property):
@@ -131,7 +157,76 @@ class block:
         return True
 
     def update_conn_weight(self, conn_idx, conn_weight):
-        raise NotImplementedError("update_conn_weight needs the CSR weight-update entry point (not built yet)")
+        """w_uij[conn_idx] = conn_weight (models/block.py:172-177).  conn_idx
+        holds the (channel, post, pre) indices of the w_uij entries, as three
+        index arrays (a tuple/list, or one [3, X'] array); conn_weight is [X']."""
+        if isinstance(conn_idx, (tuple, list)):
+            idx = [torch.as_tensor(x).detach().cpu().numpy().reshape(-1) for x in conn_idx]
+        else:
+            a = torch.as_tensor(conn_idx).detach().cpu().numpy()
+            if a.ndim != 2 or a.shape[0] != 3:
+                raise ValueError("conn_idx must index the three dimensions of w_uij: (channel, post, pre)")
+            idx = list(a)
+        if len(idx) != 3:
+            raise ValueError("conn_idx must index the three dimensions of w_uij: (channel, post, pre)")
+        ch, post, pre = (np.asarray(x, np.int64) for x in idx)
+        w = torch.as_tensor(conn_weight).detach().cpu().numpy().astype(np.float32).reshape(-1)
+        w = np.broadcast_to(w, ch.shape).copy()
+        if ((ch < 0) | (ch > 3)).any() or ((post < 0) | (post >= self.N) | (pre < 0) | (pre >= self.N)).any():
+            raise IndexError("conn_idx out of range")
+        try:
+            self._nb.set_weights(pre, post, ch, w)
+        except nat.DTBError as e:
+            if e.code != nat.DTB_INVALID_PARAMETER or "no synapse" not in str(e):
+                raise
+            self._rebuild_with(ch, post, pre, w)
+        if not self.w_uij.is_sparse:
+            with torch.no_grad():
+                self.w_uij[torch.from_numpy(ch), torch.from_numpy(post), torch.from_numpy(pre)] = \
+                    torch.from_numpy(w).to(self.w_uij.dtype).to(self.w_uij.device)
+        return True
+
+    def _rebuild_with(self, ch, post, pre, w):
+        """New network from the current one with the requested entries set
+        (missing synapses added), continuing from the current state."""
+        nb = self._nb
+        rp = nb.get("rowptr")
+        cur = dict(post=nb.get("post").astype(np.int64), kind=nb.get("kind"), w0=nb.get("w0"), w1=nb.get("w1"))
+        pre_c = np.repeat(np.arange(self.N, dtype=np.int64), np.diff(rp))
+        key = (pre_c * self.N + cur["post"]) * 2 + cur["kind"]
+        w0, w1 = cur["w0"].copy(), cur["w1"].copy()
+        add = {}
+        order = np.argsort(key, kind="stable")
+        for c, q, p, x in zip(ch, post, pre, w):
+            k = (int(p) * self.N + int(q)) * 2 + int(c) // 2
+            lo = np.searchsorted(key[order], k, side="left")
+            hi = np.searchsorted(key[order], k, side="right")
+            if hi > lo:
+                hit = order[lo:hi]
+                arr = w0 if c % 2 == 0 else w1
+                arr[hit] = 0.0
+                arr[hit[0]] = x
+            else:
+                a = add.setdefault(k, [0.0, 0.0])
+                a[int(c) % 2] = x
+        ks = np.array(sorted(add), dtype=np.int64)
+        pre_all = np.concatenate([pre_c, ks // 2 // self.N])
+        post_all = np.concatenate([cur["post"], ks // 2 % self.N])
+        kind_all = np.concatenate([cur["kind"], (ks % 2).astype(np.uint8)])
+        w0_all = np.concatenate([w0, np.array([add[k][0] for k in ks], np.float32)])
+        w1_all = np.concatenate([w1, np.array([add[k][1] for k in ks], np.float32)])
+        o = np.lexsort((kind_all, post_all, pre_all))
+        rowptr = np.zeros(self.N + 1, np.int64)
+        np.add.at(rowptr, pre_all + 1, 1)
+        rowptr = np.cumsum(rowptr)
+        snap = nb.save_state()
+        new = NativeBlock.from_csr(nb.get("props"), rowptr, post_all[o].astype(np.int32), kind_all[o], w0_all[o],
+                                   w1_all[o], dt=float(self.delta_t), noise_seed=self._seed, device=self._device)
+        if self.src_neuron is not None:
+            self._apply_sources(new)
+        new.load_state(snap)
+        nb.close()
+        self._nb = new
 
     def _t(self, a):
         return torch.from_numpy(a).to(self._out_device)
@@ -146,8 +241,12 @@ class block:
         if uniforms is not None:
             mode = nat.NOISE_UNIFORMS
             uniforms = np.ascontiguousarray(np.asarray(uniforms, dtype=np.float32).reshape(-1))
+        if self.src_neuron is not None and self.iter >= self._sched.shape[1]:
+            raise IndexError(f"src has {self._sched.shape[1]} columns: no forced spikes for step {self.iter + 1}")
         self._nb.run(1, noise_mode=mode, noise_rate=float(noise_rate), uniforms=uniforms, isolated=isolated,
                      graph=False, freqs=self._freq, vmeans=self._vmean)
+        if self.src_neuron is not None:
+            self.iter += 1
         return self._t(self._freq.astype(np.float32)), self._t(self._vmean.copy())
 
     # state attributes, read from the device on access

@@ -28,6 +28,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <tuple>
 
 #include "dtb_block.hpp"
 
@@ -102,6 +103,8 @@ struct NeuronArgs {
     const float* __restrict__ iext;
     const float* __restrict__ tref;
     const float* __restrict__ noisep;
+    const int8_t* __restrict__ jexp; // [n][4] input exponents (read for the external drive)
+    uint8_t* __restrict__ src;       // [n] forced-spike source flags (NULL: no sources)
     const int32_t* __restrict__ pop;
     const uint8_t* __restrict__ pset; // parameter set of each neuron (PSET kernels)
     const float* __restrict__ ptab;   // [npsets][16]: V_u[4], decay[4], g_L, V_L, dt/C, V_th, V_reset, I_ext
@@ -220,6 +223,8 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
         const int32_t pp = a.pop[i];
         const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
         const float un = a.noise_mode == DTB_NOISE_UNIFORMS ? a.uniforms[(int64_t)k * a.n + i] : 0.0f;
+        const char2 ee = a.ext.n ? *reinterpret_cast<const char2*>(a.jexp + 4 * i) : make_char2(0, 0);
+        const uint8_t ss = a.src ? a.src[i] : (uint8_t)0;
         J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
         float4 vu, dc;
         float gL, vL, dtC, vth, vrst, iext;
@@ -243,24 +248,31 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
         const float vi = not_ref ? vn : vrst;
         act = vi >= vth;
         vout = vi > vth ? vth : vi;
+        if (ss & 1) {
+            // a source's threshold spike comes from the schedule (models/block.py:104-107);
+            // a frozen one (an input block's neuron) also keeps its V
+            act = (ss & 4) != 0;
+            if (ss & 2) vout = V;
+        }
         if (a.clear_acc) {
             if (PAIR) reinterpret_cast<float4*>(a.acc)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             else reinterpret_cast<float2*>(a.acc)[i] = make_float2(0.f, 0.f);
         }
         a.V[i] = vout;
+        if (ss & 4) a.src[i] = ss & 3;
         if (act) {
             a.ref_end[i] = refractory_end(step, t, a.tref[i], a.dt);
             a.last_step[i] = (int32_t)step;
             a.t_last[i] = t;
         }
         const float u = a.noise_mode == DTB_NOISE_UNIFORMS ? un : noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
-        spike = (u < pr) || (act && !a.isolated);
+        spike = ((u < pr) && !(ss & 2)) || (act && !a.isolated);
         float4 jn = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
         if (a.ext.n) {
             // external arrivals of this step: part of J at the next step, like this step's deliveries
             const float x = (float)ext_count(a.ext, a.seed, (uint64_t)step, (uint64_t)i) * a.ext.w;
-            jn.x += x;
-            jn.y += x;
+            jn.x += ldexpf(x, ee.x);
+            jn.y += ldexpf(x, ee.y);
         }
         a.Jp[i] = jn;
         a.flags[i] = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
@@ -527,8 +539,6 @@ struct StageArgs {
     void* stage;      // [nbk][kStageParts][cap] uint2 (compact) / uint4 (pair)
     void* acc2;       // overflow accumulator [n]
     uint32_t* ovf;    // [nbk] overflow flags
-    float fscale;     // fixed-point accumulation: weight * 2^S as int32
-    double inv_scale; // 2^-S
 };
 
 
@@ -940,81 +950,245 @@ static void launch_accum(const Block* b, const StageArgs& sa)
     else launch_accum_f<PAIR, false>(b, sa);
 }
 
-// Per-step sum bound: for every (post, channel), the sum of |w| over its
-// incoming synapses (each synapse delivers at most once per step).
+// Input exponents (Block::jexp).  A neuron's arrivals on channel c in one
+// step are bounded by B = the sum of |w| over its inputs on c (a synapse
+// delivers at most once per step).  B is summed as an integer -- each |w|
+// rounded up to a multiple of 2^-sg, 64-bit integer atomics -- so it does not
+// depend on the order of the atomics, and the exponents a network gets are
+// the same on every run.  e = 30 + sg - ceil(log2(B)) puts B x 2^e in
+// (2^29, 2^30].
+
+// nullable exponent table: NULL reads as 0
+__device__ inline int jexp_at(const int8_t* j, int64_t k) { return j ? (int)j[k] : 0; }
+
+template <bool PAIR>
+__global__ void wmax_kernel(int64_t nnz, const uint32_t* __restrict__ col, const float* __restrict__ w,
+                            const float2* __restrict__ w2, const int8_t* __restrict__ jexp, uint32_t idx_mask,
+                            uint32_t* __restrict__ out)
+{
+    uint32_t m = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) {
+        const uint32_t c = col[e];
+        const int64_t k = (int64_t)(c & idx_mask) * 4 + ((c >> 31) << 1);
+        if (PAIR) {
+            const float2 v = w2[e];
+            m = max(m, __float_as_uint(fabsf(ldexpf(v.x, -jexp_at(jexp, k)))));
+            m = max(m, __float_as_uint(fabsf(ldexpf(v.y, -jexp_at(jexp, k + 1)))));
+        } else {
+            m = max(m, __float_as_uint(fabsf(ldexpf(w[e], -jexp_at(jexp, k)))));
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) m = max(m, (uint32_t)__shfl_xor(m, off)); // non-negative floats order like their bits
+    if (__lane_id() == 0) atomicMax(out, m);
+}
+
 template <bool PAIR>
 __global__ void insum_kernel(int64_t nnz, const uint32_t* __restrict__ col, const float* __restrict__ w,
-                             const float2* __restrict__ w2, uint32_t idx_mask, float* __restrict__ sums)
+                             const float2* __restrict__ w2, const int8_t* __restrict__ jexp, uint32_t idx_mask,
+                             int32_t sg, unsigned long long* __restrict__ sums)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) {
         const uint32_t c = col[e];
-        const int64_t post = c & idx_mask;
-        const uint32_t kind = c >> 31;
-        if (PAIR) {
-            const float2 v = w2[e];
-            unsafeAtomicAdd(&sums[post * 4 + kind * 2], fabsf(v.x));
-            unsafeAtomicAdd(&sums[post * 4 + kind * 2 + 1], fabsf(v.y));
-        } else {
-            unsafeAtomicAdd(&sums[post * 2 + kind], fabsf(w[e]));
+        const int64_t k = (int64_t)(c & idx_mask) * 4 + ((c >> 31) << 1);
+        // compact weights: one sum per receptor pair, kept on its first channel
+        const int nw = PAIR ? 2 : 1;
+        for (int x = 0; x < nw; ++x) {
+            const float v = PAIR ? (x ? w2[e].y : w2[e].x) : w[e];
+            const double a = ldexp((double)fabsf(v), sg - jexp_at(jexp, k + x));
+            const unsigned long long q = (unsigned long long)ceil(a);
+            if (q) atomicAdd(&sums[k + x], q);
         }
     }
 }
 
-__global__ void max_nonneg_kernel(int64_t n, const float* __restrict__ v, uint32_t* __restrict__ out)
+// out[0] = smallest exponent over channels with inputs (atomicMin, starts at
+// 127), out[1] = 1 if some channel's bound overflows even at the lowest
+// exponent
+__global__ void jexp_kernel(int64_t n, int32_t pair, int32_t sg, const unsigned long long* __restrict__ sums,
+                            int8_t* __restrict__ jnew, int32_t* __restrict__ out)
 {
-    uint32_t m = 0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 4; ++c) {
+        const unsigned long long B = sums[i * 4 + (pair ? c : (c & 2))];
+        int e = 0;
+        if (B) {
+            const int cl = 64 - __clzll(B - 1ull); // ceil(log2(B))
+            e = 30 + sg - cl;
+            if (e < -64) {
+                e = -64;
+                if (cl - sg + e > 30) out[1] = 1;
+            }
+            e = min(e, 64);
+            atomicMin(&out[0], e);
+        }
+        jnew[i * 4 + c] = (int8_t)e;
+    }
+}
+
+template <bool PAIR>
+__global__ void rescale_w_kernel(int64_t nnz, const uint32_t* __restrict__ col, float* __restrict__ w,
+                                 float2* __restrict__ w2, const int8_t* __restrict__ jold,
+                                 const int8_t* __restrict__ jnew, uint32_t idx_mask)
+{
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        m = max(m, __float_as_uint(v[i])); // non-negative floats order like their bits
-    m = max(m, (uint32_t)__shfl_xor(m, 1));
-    m = max(m, (uint32_t)__shfl_xor(m, 2));
-    m = max(m, (uint32_t)__shfl_xor(m, 4));
-    m = max(m, (uint32_t)__shfl_xor(m, 8));
-    m = max(m, (uint32_t)__shfl_xor(m, 16));
-    m = max(m, (uint32_t)__shfl_xor(m, 32));
-    if (__lane_id() == 0) atomicMax(out, m);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) {
+        const uint32_t c = col[e];
+        const int64_t k = (int64_t)(c & idx_mask) * 4 + ((c >> 31) << 1);
+        const int d0 = jexp_at(jnew, k) - jexp_at(jold, k);
+        if (PAIR) {
+            const int d1 = jexp_at(jnew, k + 1) - jexp_at(jold, k + 1);
+            if (d0 | d1) {
+                const float2 v = w2[e];
+                w2[e] = make_float2(ldexpf(v.x, d0), ldexpf(v.y, d1));
+            }
+        } else if (d0) {
+            w[e] = ldexpf(w[e], d0);
+        }
+    }
 }
 
-// Fixed-point accumulation scale: S such that the largest possible per-step
-// sum times 2^S stays below 2^30.  Usable when S >= 16.
-static int fixed_scale(Block* b)
+// Jp, acc (and the overflow accumulator) from exponents jold to jnew
+template <bool PAIR>
+__global__ void rescale_state_kernel(int64_t n, float4* __restrict__ Jp, void* __restrict__ acc,
+                                     void* __restrict__ acc2, const int8_t* __restrict__ jold,
+                                     const int8_t* __restrict__ jnew)
 {
-    const int C = b->pair ? 4 : 2;
-    float* sums = nullptr;
-    uint32_t* mx = nullptr;
-    float bound = 0.0f;
-    if (b->nnz > 0) {
-        DTB_TRY_HIP(hipMalloc(&sums, sizeof(float) * (size_t)b->n * C));
-        hipError_t e = hipMalloc(&mx, sizeof(uint32_t));
-        if (e == hipSuccess) e = hipMemsetAsync(sums, 0, sizeof(float) * (size_t)b->n * C, b->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(mx, 0, sizeof(uint32_t), b->stream);
-        if (e == hipSuccess) {
-            const unsigned g = (unsigned)std::min<int64_t>(65536, (b->nnz + 255) / 256);
-            if (b->pair)
-                hipLaunchKernelGGL(insum_kernel<true>, dim3(g), dim3(256), 0, b->stream, b->nnz, b->col, b->w, b->w2,
-                                   b->pk.idx_mask, sums);
-            else
-                hipLaunchKernelGGL(insum_kernel<false>, dim3(g), dim3(256), 0, b->stream, b->nnz, b->col, b->w, b->w2,
-                                   b->pk.idx_mask, sums);
-            const unsigned g2 = (unsigned)std::min<int64_t>(4096, (b->n * C + 255) / 256);
-            hipLaunchKernelGGL(max_nonneg_kernel, dim3(g2), dim3(256), 0, b->stream, b->n * C, sums, mx);
-            uint32_t bits = 0;
-            e = hipMemcpyAsync(&bits, mx, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-            memcpy(&bound, &bits, sizeof(float));
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) d[c] = jexp_at(jnew, 4 * i + c) - jexp_at(jold, 4 * i + c);
+    if (!(d[0] | d[1] | d[2] | d[3])) return;
+    const float4 j = Jp[i];
+    Jp[i] = make_float4(ldexpf(j.x, d[0]), ldexpf(j.y, d[1]), ldexpf(j.z, d[2]), ldexpf(j.w, d[3]));
+    void* bufs[2] = {acc, acc2};
+    for (int x = 0; x < 2; ++x) {
+        if (!bufs[x]) continue;
+        if (PAIR) {
+            float4* a = reinterpret_cast<float4*>(bufs[x]) + i;
+            const float4 v = *a;
+            *a = make_float4(ldexpf(v.x, d[0]), ldexpf(v.y, d[1]), ldexpf(v.z, d[2]), ldexpf(v.w, d[3]));
+        } else {
+            float2* a = reinterpret_cast<float2*>(bufs[x]) + i;
+            const float2 v = *a;
+            *a = make_float2(ldexpf(v.x, d[0]), ldexpf(v.y, d[2]));
         }
-        hipFree(sums);
-        if (mx) hipFree(mx);
-        DTB_TRY_HIP(e);
     }
-    // margin for the float rounding of the bound itself
-    const double lim = (double)(1 << 30) / std::max(1e-30, (double)bound * 1.01 + 1e-30);
-    int S = (int)std::floor(std::log2(lim));
-    S = std::min(S, 60);
-    b->acc_shift = S;
-    b->fixed_ok = std::isfinite(bound) && S >= 16;
-    b->accum_fixed = b->fixed_ok;
+}
+
+// unscaled weights of one pair member (field W0: which = 0, W1: which = 1)
+template <bool PAIR>
+__global__ void weight_out_kernel(int64_t nnz, const uint32_t* __restrict__ col, const float* __restrict__ w,
+                                  const float2* __restrict__ w2, const int8_t* __restrict__ jexp, uint32_t idx_mask,
+                                  int32_t which, float* __restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) {
+        const uint32_t c = col[e];
+        const int64_t k = (int64_t)(c & idx_mask) * 4 + ((c >> 31) << 1) + (PAIR ? which : 0);
+        const float v = PAIR ? (which ? w2[e].y : w2[e].x) : w[e];
+        out[e] = ldexpf(v, -jexp_at(jexp, k));
+    }
+}
+
+static inline unsigned grid_syn(int64_t nnz);
+static inline unsigned grid_for(int64_t n, int bs);
+
+// (Re)compute the input exponents from the current weights and rescale the
+// weights (and, with_state, J and the pending arrivals) to them.  Exact:
+// every change is a multiplication by a power of two.
+static int update_jexp(Block* b, bool with_state)
+{
+    const int64_t n = b->n, nnz = b->nnz;
+    if (!b->jexp) {
+        DTB_TRY(block_alloc(b, reinterpret_cast<void**>(&b->jexp), 4 * (size_t)n));
+        DTB_TRY_HIP(hipMemsetAsync(b->jexp, 0, 4 * (size_t)n, b->stream));
+    }
+    int8_t* jnew = nullptr;
+    unsigned long long* sums = nullptr;
+    uint32_t* mx = nullptr;
+    int32_t* mm = nullptr;
+    int st = DTB_OK;
+    hipError_t e = hipMalloc(&jnew, 4 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&sums, 8 * 4 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&mx, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&mm, 2 * sizeof(int32_t));
+    do {
+        if (e != hipSuccess) break;
+        const int32_t mm0[2] = {127, 0};
+        if ((e = hipMemsetAsync(mx, 0, sizeof(uint32_t), b->stream)) != hipSuccess) break;
+        if ((e = hipMemsetAsync(sums, 0, 8 * 4 * (size_t)n, b->stream)) != hipSuccess) break;
+        if ((e = hipMemcpyAsync(mm, mm0, sizeof(mm0), hipMemcpyHostToDevice, b->stream)) != hipSuccess) break;
+        float maxw = 0.0f;
+        if (nnz > 0) {
+            if (b->pair)
+                hipLaunchKernelGGL(wmax_kernel<true>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col, b->w,
+                                   b->w2, b->jexp, b->pk.idx_mask, mx);
+            else
+                hipLaunchKernelGGL(wmax_kernel<false>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col, b->w,
+                                   b->w2, b->jexp, b->pk.idx_mask, mx);
+            uint32_t bits = 0;
+            if ((e = hipMemcpyAsync(&bits, mx, 4, hipMemcpyDeviceToHost, b->stream)) != hipSuccess) break;
+            if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) break;
+            std::memcpy(&maxw, &bits, 4);
+        }
+        if (!std::isfinite(maxw)) {
+            set_error("synaptic weights must be finite");
+            st = DTB_INVALID_PARAMETER;
+            break;
+        }
+        // sums stay below 2^63: nnz * (maxw * 2^sg + 1) < 2^62
+        int32_t sg = 0;
+        if (maxw > 0.0f) {
+            const int lw = (int)std::ceil(std::log2((double)maxw));
+            const int ln = (int)std::ceil(std::log2((double)nnz + 1.0));
+            sg = std::max(-200, std::min(200, 61 - ln - lw));
+            if (nnz > 0) {
+                if (b->pair)
+                    hipLaunchKernelGGL(insum_kernel<true>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col,
+                                       b->w, b->w2, b->jexp, b->pk.idx_mask, sg, sums);
+                else
+                    hipLaunchKernelGGL(insum_kernel<false>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col,
+                                       b->w, b->w2, b->jexp, b->pk.idx_mask, sg, sums);
+            }
+        }
+        hipLaunchKernelGGL(jexp_kernel, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, (int32_t)b->pair, sg,
+                           sums, jnew, mm);
+        if (nnz > 0) {
+            if (b->pair)
+                hipLaunchKernelGGL(rescale_w_kernel<true>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col,
+                                   b->w, b->w2, b->jexp, jnew, b->pk.idx_mask);
+            else
+                hipLaunchKernelGGL(rescale_w_kernel<false>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col,
+                                   b->w, b->w2, b->jexp, jnew, b->pk.idx_mask);
+        }
+        if (with_state) {
+            if (b->pair)
+                hipLaunchKernelGGL(rescale_state_kernel<true>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n,
+                                   b->Jp, b->acc, b->acc2, b->jexp, jnew);
+            else
+                hipLaunchKernelGGL(rescale_state_kernel<false>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n,
+                                   b->Jp, b->acc, b->acc2, b->jexp, jnew);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        if ((e = hipMemcpyAsync(b->jexp, jnew, 4 * (size_t)n, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess) break;
+        int32_t hm[2] = {0, 0};
+        if ((e = hipMemcpyAsync(hm, mm, sizeof(hm), hipMemcpyDeviceToHost, b->stream)) != hipSuccess) break;
+        if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) break;
+        b->acc_shift = hm[0] == 127 ? 0 : hm[0];
+        b->fixed_ok = hm[1] == 0;
+    } while (0);
+    if (jnew) hipFree(jnew);
+    if (sums) hipFree(sums);
+    if (mx) hipFree(mx);
+    if (mm) hipFree(mm);
+    if (st) return st;
+    DTB_TRY_HIP(e);
+    if (!b->fixed_ok) b->accum_fixed = false;
     return DTB_OK;
 }
 
@@ -1080,8 +1254,8 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
                 const uint32_t o = (v[u].x & 0x7fffffffu) * 4 + ((v[u].x >> 31) << 1);
                 if (FIXED) {
                     int* si = reinterpret_cast<int*>(slice);
-                    atomicAdd(&si[o], __float2int_rn(__uint_as_float(v[u].y) * s.fscale));
-                    atomicAdd(&si[o + 1], __float2int_rn(__uint_as_float(v[u].z) * s.fscale));
+                    atomicAdd(&si[o], __float2int_rn(__uint_as_float(v[u].y)));
+                    atomicAdd(&si[o + 1], __float2int_rn(__uint_as_float(v[u].z)));
                 } else {
                     atomicAdd(&slice[o], __uint_as_float(v[u].y));
                     atomicAdd(&slice[o + 1], __uint_as_float(v[u].z));
@@ -1099,7 +1273,7 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
                 if (base + (uint32_t)u * blockDim.x >= cnt) break;
                 const uint32_t o = (v[u].x & 0x7fffffffu) * 2 + (v[u].x >> 31);
                 if (FIXED)
-                    atomicAdd(&reinterpret_cast<int*>(slice)[o], __float2int_rn(__uint_as_float(v[u].y) * s.fscale));
+                    atomicAdd(&reinterpret_cast<int*>(slice)[o], __float2int_rn(__uint_as_float(v[u].y)));
                 else
                     atomicAdd(&slice[o], __uint_as_float(v[u].y));
             }
@@ -1115,8 +1289,7 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
             float4 v;
             if (FIXED) {
                 const int* si = reinterpret_cast<const int*>(slice) + 4 * x;
-                v = make_float4((float)((double)si[0] * s.inv_scale), (float)((double)si[1] * s.inv_scale),
-                                (float)((double)si[2] * s.inv_scale), (float)((double)si[3] * s.inv_scale));
+                v = make_float4((float)si[0], (float)si[1], (float)si[2], (float)si[3]);
             } else {
                 v = make_float4(slice[4 * x], slice[4 * x + 1], slice[4 * x + 2], slice[4 * x + 3]);
             }
@@ -1134,7 +1307,7 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
             float2 v;
             if (FIXED) {
                 const int* si = reinterpret_cast<const int*>(slice) + 2 * x;
-                v = make_float2((float)((double)si[0] * s.inv_scale), (float)((double)si[1] * s.inv_scale));
+                v = make_float2((float)si[0], (float)si[1]);
             } else {
                 v = make_float2(slice[2 * x], slice[2 * x + 1]);
             }
@@ -1149,6 +1322,25 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     __syncthreads();
     if (t < kStageParts) s.cursor[((size_t)b * kStageParts + t) * kCursorStride] = 0u;
     if (t == 0) s.ovf[(size_t)b * kCursorStride] = 0u;
+}
+
+// Forced spikes of batch step k: the sources listed for the step's 0-based
+// index (nbase + k) get their fire mark, which the neuron kernel consumes.
+__global__ void src_mark_kernel(const StepCtl* __restrict__ ctl, int32_t k, int64_t nts,
+                                const uint32_t* __restrict__ ts, const int64_t* __restrict__ ptr,
+                                const uint32_t* __restrict__ ids, uint8_t* __restrict__ src)
+{
+    const int64_t step0 = ctl->nbase + k;
+    int64_t lo = 0, hi = nts; // first timestamp >= step0
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)ts[mid] < step0) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= nts || (int64_t)ts[lo] != step0) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t x = ptr[lo] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < ptr[lo + 1]; x += stride)
+        src[ids[x]] |= (uint8_t)4;
 }
 
 // Batch prologue: times of the K steps, step counters, record clearing.
@@ -1278,7 +1470,8 @@ __global__ void scale_kernel(float* x, const float* inv, int64_t K, int32_t S)
 __global__ void derive_kernel(int64_t n, const float* __restrict__ props, double dt, int32_t excl, float4* g,
                               float4* vu, float4* decay, float* gL, float* vL, float* dtC, float* vth, float* vrst,
                               float* iext, float* tref, float* noisep, int32_t* pop, int32_t* sub,
-                              int32_t* ref_end, const int32_t* last_step, const double* t_last)
+                              int32_t* ref_end, const int32_t* last_step, const double* t_last,
+                              const int8_t* __restrict__ jexp)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1294,7 +1487,9 @@ __global__ void derive_kernel(int64_t n, const float* __restrict__ props, double
     vL[i] = p[7];
     vth[i] = p[8];
     vrst[i] = p[9];
-    g[i] = make_float4(p[10], p[11], p[12], p[13]);
+    // J is held x 2^e per channel, so g x 2^-e gives the same products exactly
+    g[i] = make_float4(ldexpf(p[10], -jexp_at(jexp, 4 * i)), ldexpf(p[11], -jexp_at(jexp, 4 * i + 1)),
+                       ldexpf(p[12], -jexp_at(jexp, 4 * i + 2)), ldexpf(p[13], -jexp_at(jexp, 4 * i + 3)));
     vu[i] = make_float4(p[14], p[15], p[16], p[17]);
     decay[i] = make_float4((float)exp(-dt / (double)p[18]), (float)exp(-dt / (double)p[19]),
                            (float)exp(-dt / (double)p[20]), (float)exp(-dt / (double)p[21]));
@@ -1363,7 +1558,8 @@ __global__ void unpack_kernel(int64_t nnz, const uint32_t* __restrict__ col, Syn
 template <bool PAIR>
 __global__ void state_kernel(int64_t n, const float* __restrict__ V, const float4* __restrict__ Jp,
                              const void* __restrict__ acc, const float4* __restrict__ g,
-                             const float4* __restrict__ vu, float* J_out, float* I_out, float* Isyn_out)
+                             const float4* __restrict__ vu, const int8_t* __restrict__ jexp, float* J_out,
+                             float* I_out, float* Isyn_out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1381,7 +1577,10 @@ __global__ void state_kernel(int64_t n, const float* __restrict__ V, const float
     const float i0 = gg.x * (uu.x - v) * J.x, i1 = gg.y * (uu.y - v) * J.y;
     const float i2 = gg.z * (uu.z - v) * J.z, i3 = gg.w * (uu.w - v) * J.w;
     if (J_out) {
-        J_out[i] = J.x; J_out[n + i] = J.y; J_out[2 * n + i] = J.z; J_out[3 * n + i] = J.w;
+        J_out[i] = ldexpf(J.x, -jexp_at(jexp, 4 * i));
+        J_out[n + i] = ldexpf(J.y, -jexp_at(jexp, 4 * i + 1));
+        J_out[2 * n + i] = ldexpf(J.z, -jexp_at(jexp, 4 * i + 2));
+        J_out[3 * n + i] = ldexpf(J.w, -jexp_at(jexp, 4 * i + 3));
     }
     if (I_out) {
         I_out[i] = i0; I_out[n + i] = i1; I_out[2 * n + i] = i2; I_out[3 * n + i] = i3;
@@ -1554,7 +1753,7 @@ int derive_props(Block* b, bool psets_dirty)
 {
     hipLaunchKernelGGL(derive_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->props, b->dt,
                        b->stats_excl, b->g, b->vu, b->decay, b->gL, b->vL, b->dtC, b->vth, b->vrst, b->iext, b->tref,
-                       b->noisep, b->pop, b->sub, b->ref_end, b->last_step, b->t_last);
+                       b->noisep, b->pop, b->sub, b->ref_end, b->last_step, b->t_last, b->jexp);
     DTB_TRY_HIP(hipGetLastError());
     // statistics populations and their sizes (host copy of sub_idx / pop)
     std::vector<int32_t> pop((size_t)b->n);
@@ -1753,7 +1952,7 @@ static int ensure_bucketed(Block* b)
     DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)nbk_max * kStageParts * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)nbk_max * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->acc2, 0, (b->pair ? 16 : 8) * (size_t)b->n, b->stream));
-    DTB_TRY(fixed_scale(b));
+    b->accum_fixed = b->fixed_ok;
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     return DTB_OK;
 }
@@ -1763,6 +1962,7 @@ static int finish_create(Block* b, const float* props)
     DTB_TRY(upload(b, b->props, props, sizeof(float) * (size_t)b->n * DTB_NPROP));
     hipLaunchKernelGGL(init_state_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n, b->props, b->V);
     DTB_TRY_HIP(hipGetLastError());
+    DTB_TRY(update_jexp(b, false));
     DTB_TRY(derive_props(b));
     DTB_TRY(alloc_ring_and_records(b));
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
@@ -1962,7 +2162,8 @@ static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* unif
     NeuronArgs x;
     x.n = b->n; x.D = b->D; x.S = b->S;
     x.g = b->g; x.vu = b->vu; x.decay = b->decay; x.gL = b->gL; x.vL = b->vL; x.dtC = b->dtC; x.vth = b->vth;
-    x.vrst = b->vrst; x.iext = b->iext; x.tref = b->tref; x.noisep = b->noisep; x.pop = b->pop;
+    x.vrst = b->vrst; x.iext = b->iext; x.tref = b->tref; x.noisep = b->noisep; x.jexp = b->jexp; x.pop = b->pop;
+    x.src = b->src;
     x.pset = b->pset; x.ptab = b->ptab; x.npsets = b->npsets;
     x.V = b->V; x.Jp = b->Jp; x.ref_end = b->ref_end; x.last_step = b->last_step; x.t_last = b->t_last;
     x.flags = b->flags; x.acc = b->acc; x.spk = b->spk; x.spk_cnt = b->spk_cnt; x.spk_cap = b->spk_cap;
@@ -2010,6 +2211,10 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
     const DeliverArgs da = deliver_args(b);
     const unsigned ng = grid_for(b->n, 256);
     for (int32_t k = 0; k < K; ++k) {
+        if (b->src_nts > 0)
+            hipLaunchKernelGGL(src_mark_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (b->src_max + 255) / 256))),
+                               dim3(256), 0, b->stream, b->ctl, k, b->src_nts, b->src_ts, b->src_ptr, b->src_ids,
+                               b->src);
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k)], b->stream));
         if (b->pair) {
             if (b->npsets)
@@ -2024,8 +2229,8 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         }
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
         if (b->deliver_mode == 1) {
-            StageArgs sa{da,     b->nbk,  b->bsize, b->bmagic, b->bshift, b->stage_cap, b->cursor,
-                         b->stage, b->acc2, b->ovf, std::ldexp(1.0f, b->acc_shift), std::ldexp(1.0, -b->acc_shift)};
+            StageArgs sa{da, b->nbk, b->bsize, b->bmagic, b->bshift, b->stage_cap, b->cursor, b->stage, b->acc2,
+                         b->ovf};
             if (b->pair) {
                 launch_stage<true>(b, sa, k);
                 launch_accum<true>(b, sa);
@@ -2240,10 +2445,10 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
         float* Is = field == DTB_FIELD_I_SYN ? (float*)tmp : nullptr;
         if (b->pair)
             hipLaunchKernelGGL(state_kernel<true>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, b->V, b->Jp,
-                               b->acc, b->g, b->vu, J, I, Is);
+                               b->acc, b->g, b->vu, b->jexp, J, I, Is);
         else
             hipLaunchKernelGGL(state_kernel<false>, dim3(grid_for(n, 256)), dim3(256), 0, b->stream, n, b->V, b->Jp,
-                               b->acc, b->g, b->vu, J, I, Is);
+                               b->acc, b->g, b->vu, b->jexp, J, I, Is);
         st = copy_out(b, dst, tmp, (size_t)sz);
         break;
     }
@@ -2312,12 +2517,15 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
     case DTB_FIELD_W1:
         DTB_TRY(need(4 * nnz));
         if (nnz == 0) break;
-        if (b->pair) {
-            DTB_TRY_HIP(hipMemcpy2DAsync(dst, 4, reinterpret_cast<const float*>(b->w2) + (field == DTB_FIELD_W1), 8, 4,
-                                         (size_t)nnz, hipMemcpyDefault, b->stream));
-        } else {
-            DTB_TRY(copy_out(b, dst, b->w, 4 * (size_t)nnz));
-        }
+        // the stored weights are scaled by the input exponents of their targets
+        DTB_TRY_HIP(hipMalloc(&tmp, 4 * (size_t)nnz));
+        if (b->pair)
+            hipLaunchKernelGGL(weight_out_kernel<true>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col,
+                               b->w, b->w2, b->jexp, b->pk.idx_mask, (int32_t)(field == DTB_FIELD_W1), (float*)tmp);
+        else
+            hipLaunchKernelGGL(weight_out_kernel<false>, dim3(grid_syn(nnz)), dim3(256), 0, b->stream, nnz, b->col,
+                               b->w, b->w2, b->jexp, b->pk.idx_mask, (int32_t)(field == DTB_FIELD_W1), (float*)tmp);
+        st = copy_out(b, dst, tmp, 4 * (size_t)nnz);
         break;
     case DTB_FIELD_T_LAST:
         DTB_TRY(need(8 * n));
@@ -2341,10 +2549,12 @@ static int get_impl(Block* b, int32_t field, void* dst, int64_t bytes, int64_t* 
 // State snapshot: a 256-byte header, then the device sections below, each
 // starting on a 256-byte boundary.  The sections are everything the next
 // steps read that a step writes; properties, connectivity, the launch shape
-// and the per-batch records are not part of it.
+// and the per-batch records are not part of it.  Jp and acc are stored in the
+// saving block's input scaling, so the input exponents are the last section;
+// a loading block re-expresses them in its own (a power-of-two rescale).
 namespace {
 constexpr uint64_t kSnapMagic = 0x31504e5342544455ull; // "UDTBSNP1"
-constexpr uint32_t kSnapVersion = 1;
+constexpr uint32_t kSnapVersion = 2;
 
 struct SnapHeader {
     uint64_t magic;
@@ -2385,6 +2595,7 @@ static std::vector<SnapSection> snap_sections(Block* b)
         {b->spk_cnt, 4 * L * kSubLists * kCntStride},
     };
     if (b->nvox > 0) v.push_back({b->bold_state, 6 * sizeof(double) * (size_t)b->nvox});
+    v.push_back({b->jexp, 4 * n}); // last: loaded into a scratch copy, not over the block's own
     return v;
 }
 
@@ -2451,15 +2662,29 @@ static int load_state_impl(Block* b, const void* src, int64_t bytes)
     const char* in = static_cast<const char*>(src);
     int64_t off = 256;
     int st = DTB_OK;
-    for (const SnapSection& x : snap_sections(b)) {
-        if (hipMemcpyAsync(x.p, in + off, x.bytes, hipMemcpyDefault, b->stream) != hipSuccess) {
+    int8_t* jsnap = nullptr;
+    DTB_TRY_HIP(hipMalloc(&jsnap, 4 * (size_t)b->n));
+    const std::vector<SnapSection> secs = snap_sections(b);
+    for (size_t k = 0; k < secs.size(); ++k) {
+        const SnapSection& x = secs[k];
+        void* to = k + 1 == secs.size() ? (void*)jsnap : x.p;
+        if (hipMemcpyAsync(to, in + off, x.bytes, hipMemcpyDefault, b->stream) != hipSuccess) {
             set_error("state snapshot copy failed");
             st = DTB_RUNTIME_ERROR;
             break;
         }
         off += ((int64_t)x.bytes + 255) / 256 * 256;
     }
+    if (!st) {
+        if (b->pair)
+            hipLaunchKernelGGL(rescale_state_kernel<true>, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n,
+                               b->Jp, b->acc, nullptr, jsnap, b->jexp);
+        else
+            hipLaunchKernelGGL(rescale_state_kernel<false>, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n,
+                               b->Jp, b->acc, nullptr, jsnap, b->jexp);
+    }
     const hipError_t e = hipStreamSynchronize(b->stream);
+    hipFree(jsnap);
     if (st) return st;
     DTB_TRY_HIP(e);
     if (h.seed != b->seed) {
@@ -2506,6 +2731,287 @@ static int group_requests(Block* b, int64_t m, const uint32_t* pop, const uint32
 } // namespace dtb
 
 using namespace dtb;
+
+// ------------------------------------------------------- weight updates
+// Requests name synapses by (pre, post, channel); the rows of a presynaptic
+// neuron are short (at most a few thousand synapses), so each request scans
+// its row.  Pass 1 counts the matches; pass 2 writes the first match and
+// zeroes the others, so several synapses joining one pair on one receptor
+// group sum to the requested value, as one dense w_uij entry does.
+__global__ void find_syn_kernel(int64_t m, const uint32_t* __restrict__ pre, const uint32_t* __restrict__ post,
+                                const uint8_t* __restrict__ chan, const int64_t* __restrict__ rowptr,
+                                const uint32_t* __restrict__ col, uint32_t idx_mask, int64_t* __restrict__ first)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const uint32_t want = post[r] | ((uint32_t)(chan[r] >> 1) << 31);
+    int64_t f = -1;
+    for (int64_t e = rowptr[pre[r]]; e < rowptr[pre[r] + 1]; ++e) {
+        const uint32_t c = col[e];
+        if (((c & idx_mask) | (c & 0x80000000u)) == want) {
+            f = e;
+            break;
+        }
+    }
+    first[r] = f;
+}
+
+__global__ void write_syn_kernel(int64_t m, const uint32_t* __restrict__ pre, const uint32_t* __restrict__ post,
+                                 const uint8_t* __restrict__ chan, const float* __restrict__ val,
+                                 const int64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                 uint32_t idx_mask, const int8_t* __restrict__ jexp, int32_t pair, float* w)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const uint32_t ch = chan[r];
+    const uint32_t want = post[r] | ((ch >> 1) << 31);
+    // stored in the target's input scaling (exact)
+    const float x = ldexpf(val[r], jexp[(int64_t)post[r] * 4 + ch]);
+    bool first = true;
+    for (int64_t e = rowptr[pre[r]]; e < rowptr[pre[r] + 1]; ++e) {
+        const uint32_t c = col[e];
+        if (((c & idx_mask) | (c & 0x80000000u)) != want) continue;
+        w[pair ? 2 * e + (ch & 1) : e] = first ? x : 0.0f;
+        first = false;
+    }
+}
+
+__global__ void dup_pair_kernel(int64_t nnz, const float* __restrict__ w, float2* __restrict__ w2)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) w2[e] = make_float2(w[e], w[e]);
+}
+
+// compact arrivals (E, I) -> per channel (E, E, I, I)
+__global__ void widen_acc_kernel(int64_t n, const float2* __restrict__ a2, float4* __restrict__ a4)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 v = a2[i];
+        a4[i] = make_float4(v.x, v.x, v.y, v.y);
+    }
+}
+
+static void block_free(Block* b, void* p)
+{
+    if (!p) return;
+    auto it = std::find(b->allocs.begin(), b->allocs.end(), p);
+    if (it != b->allocs.end()) b->allocs.erase(it);
+    hipFree(p);
+}
+
+// Compact weights (one per synapse, shared by the receptor pair) to pair
+// weights (one per channel): weights, arrivals and the bucketed staging take
+// the pair layout; results continue unchanged.
+static int make_pair(Block* b)
+{
+    if (b->pair) return DTB_OK;
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    float2* w2 = nullptr;
+    DTB_TRY(alloc_n(b, &w2, std::max<int64_t>(b->nnz, 1)));
+    if (b->nnz > 0)
+        hipLaunchKernelGGL(dup_pair_kernel, dim3(grid_syn(b->nnz)), dim3(256), 0, b->stream, b->nnz, b->w, w2);
+    void* acc4 = nullptr;
+    DTB_TRY(block_alloc(b, &acc4, sizeof(float4) * (size_t)b->n));
+    hipLaunchKernelGGL(widen_acc_kernel, dim3(grid_for(b->n, 256)), dim3(256), 0, b->stream, b->n,
+                       (const float2*)b->acc, (float4*)acc4);
+    DTB_TRY_HIP(hipGetLastError());
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    block_free(b, b->w);
+    block_free(b, b->acc);
+    b->w = nullptr;
+    b->w2 = w2;
+    b->acc = acc4;
+    // staging holds 16-byte events for pair weights: rebuilt on demand
+    const bool had = b->stage != nullptr;
+    block_free(b, b->stage);
+    block_free(b, b->cursor);
+    block_free(b, b->ovf);
+    block_free(b, b->acc2);
+    b->stage = nullptr;
+    b->cursor = nullptr;
+    b->ovf = nullptr;
+    b->acc2 = nullptr;
+    b->pair = true;
+    b->stage_ept = 16;
+    for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    if (had || b->deliver_mode == 1) {
+        const bool fixed = b->accum_fixed;
+        DTB_TRY(ensure_bucketed(b));
+        b->accum_fixed = fixed && b->fixed_ok;
+        if (b->stage_variant == 2 && !b->sorted_ok) b->stage_variant = 0;
+    }
+    return DTB_OK;
+}
+
+static int set_weights_impl(Block* b, int64_t m, const uint32_t* pre, const uint32_t* post, const uint8_t* chan,
+                            const float* val)
+{
+    DTB_REQUIRE(m >= 0, "m must be >= 0");
+    if (m == 0) return DTB_OK;
+    DTB_REQUIRE(pre && post && chan && val, "pre, post, channel and w are required");
+    std::vector<uint32_t> hp((size_t)m), hq((size_t)m);
+    std::vector<uint8_t> hc((size_t)m);
+    std::vector<float> hv((size_t)m);
+    DTB_TRY_HIP(hipMemcpy(hp.data(), pre, 4 * (size_t)m, hipMemcpyDefault));
+    DTB_TRY_HIP(hipMemcpy(hq.data(), post, 4 * (size_t)m, hipMemcpyDefault));
+    DTB_TRY_HIP(hipMemcpy(hc.data(), chan, (size_t)m, hipMemcpyDefault));
+    DTB_TRY_HIP(hipMemcpy(hv.data(), val, 4 * (size_t)m, hipMemcpyDefault));
+    // validate; the last request for a (pre, post, channel) wins
+    std::map<std::tuple<uint32_t, uint32_t, uint8_t>, float> req;
+    for (int64_t r = 0; r < m; ++r) {
+        DTB_REQUIRE(hp[(size_t)r] < (uint64_t)b->n && hq[(size_t)r] < (uint64_t)b->n,
+                    "request %lld: neuron index out of range", (long long)r);
+        DTB_REQUIRE(hc[(size_t)r] <= 3, "request %lld: channel must be 0..3", (long long)r);
+        DTB_REQUIRE(std::isfinite(hv[(size_t)r]), "request %lld: weight is not finite", (long long)r);
+        req[std::make_tuple(hp[(size_t)r], hq[(size_t)r], hc[(size_t)r])] = hv[(size_t)r];
+    }
+    // compact weights hold one value per receptor pair: a request that sets
+    // the two channels of a pair differently (or only one of them) needs pair weights
+    bool split = false;
+    if (!b->pair) {
+        for (const auto& kv : req) {
+            const auto& [p, q, c] = kv.first;
+            auto it = req.find(std::make_tuple(p, q, (uint8_t)(c ^ 1)));
+            if (it == req.end() || it->second != kv.second) {
+                split = true;
+                break;
+            }
+        }
+    }
+    const int64_t u = (int64_t)req.size();
+    std::vector<uint32_t> up, uq;
+    std::vector<uint8_t> uc;
+    std::vector<float> uv;
+    for (const auto& kv : req) {
+        up.push_back(std::get<0>(kv.first));
+        uq.push_back(std::get<1>(kv.first));
+        uc.push_back(std::get<2>(kv.first));
+        uv.push_back(kv.second);
+    }
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    char* dbuf = nullptr;
+    const size_t off_q = 4 * (size_t)u, off_c = 8 * (size_t)u, off_v = (9 * (size_t)u + 7) & ~(size_t)7,
+                 off_f = (off_v + 4 * (size_t)u + 7) & ~(size_t)7;
+    DTB_TRY_HIP(hipMalloc(&dbuf, off_f + 8 * (size_t)u));
+    int st = DTB_OK;
+    do {
+        if ((st = upload(b, dbuf, up.data(), 4 * (size_t)u))) break;
+        if ((st = upload(b, dbuf + off_q, uq.data(), 4 * (size_t)u))) break;
+        if ((st = upload(b, dbuf + off_c, uc.data(), (size_t)u))) break;
+        if ((st = upload(b, dbuf + off_v, uv.data(), 4 * (size_t)u))) break;
+        const uint32_t* dp = (const uint32_t*)dbuf;
+        const uint32_t* dq = (const uint32_t*)(dbuf + off_q);
+        const uint8_t* dc = (const uint8_t*)(dbuf + off_c);
+        const float* dv = (const float*)(dbuf + off_v);
+        int64_t* df = (int64_t*)(dbuf + off_f);
+        hipLaunchKernelGGL(find_syn_kernel, dim3(grid_for(u, 256)), dim3(256), 0, b->stream, u, dp, dq, dc, b->rowptr,
+                           b->col, b->pk.idx_mask, df);
+        std::vector<int64_t> hf((size_t)u);
+        if (hipMemcpyAsync(hf.data(), df, 8 * (size_t)u, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+            hipStreamSynchronize(b->stream) != hipSuccess) {
+            set_error("synapse search failed");
+            st = DTB_RUNTIME_ERROR;
+            break;
+        }
+        for (int64_t r = 0; r < u; ++r) {
+            if (hf[(size_t)r] < 0) {
+                set_error("no synapse from neuron %u to neuron %u on channel %u: only existing synapses can be "
+                          "changed", up[(size_t)r], uq[(size_t)r], (unsigned)uc[(size_t)r]);
+                st = DTB_INVALID_PARAMETER;
+                break;
+            }
+        }
+        if (st) break;
+        if (split && (st = make_pair(b))) break;
+        float* wp = b->pair ? reinterpret_cast<float*>(b->w2) : b->w;
+        hipLaunchKernelGGL(write_syn_kernel, dim3(grid_for(u, 256)), dim3(256), 0, b->stream, u, dp, dq, dc, dv,
+                           b->rowptr, b->col, b->pk.idx_mask, b->jexp, (int32_t)b->pair, wp);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
+            set_error("weight update failed");
+            st = DTB_RUNTIME_ERROR;
+        }
+    } while (0);
+    hipFree(dbuf);
+    if (st) return st;
+    // the input bounds changed: new exponents, weights and state rescaled, g re-derived
+    DTB_TRY(update_jexp(b, true));
+    return derive_props(b, false);
+}
+
+// ------------------------------------------------------- forced spikes
+static int set_sources_impl(Block* b, int64_t n_src, const uint32_t* sources, const uint8_t* frozen,
+                            int64_t n_events, const uint32_t* step, const uint32_t* neuron)
+{
+    DTB_REQUIRE(n_src >= 0 && n_events >= 0, "counts must be >= 0");
+    DTB_REQUIRE(n_src == 0 || sources, "sources is NULL");
+    DTB_REQUIRE(n_events == 0 || (step && neuron), "step and neuron are required with events");
+    DTB_REQUIRE(n_src > 0 || n_events == 0, "events need sources");
+    const int64_t n = b->n;
+    std::vector<uint32_t> hs((size_t)n_src), ht((size_t)n_events), hn((size_t)n_events);
+    std::vector<uint8_t> hf((size_t)n_src, 0);
+    if (n_src) DTB_TRY_HIP(hipMemcpy(hs.data(), sources, 4 * (size_t)n_src, hipMemcpyDefault));
+    if (n_src && frozen) DTB_TRY_HIP(hipMemcpy(hf.data(), frozen, (size_t)n_src, hipMemcpyDefault));
+    if (n_events) {
+        DTB_TRY_HIP(hipMemcpy(ht.data(), step, 4 * (size_t)n_events, hipMemcpyDefault));
+        DTB_TRY_HIP(hipMemcpy(hn.data(), neuron, 4 * (size_t)n_events, hipMemcpyDefault));
+    }
+    std::vector<uint8_t> flag((size_t)n, 0);
+    for (int64_t k = 0; k < n_src; ++k) {
+        DTB_REQUIRE(hs[(size_t)k] < (uint64_t)n, "source %lld: neuron %u out of range", (long long)k, hs[(size_t)k]);
+        flag[hs[(size_t)k]] = (uint8_t)(1 | (hf[(size_t)k] ? 2 : 0));
+    }
+    std::vector<int64_t> order((size_t)n_events);
+    for (int64_t k = 0; k < n_events; ++k) {
+        DTB_REQUIRE(hn[(size_t)k] < (uint64_t)n && (flag[hn[(size_t)k]] & 1),
+                    "event %lld: neuron %u is not a source", (long long)k, hn[(size_t)k]);
+        DTB_REQUIRE(ht[(size_t)k] < 0x7fff0000u, "event %lld: step index out of range", (long long)k);
+        order[(size_t)k] = k;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return ht[(size_t)x] < ht[(size_t)y]; });
+    std::vector<uint32_t> ts, ids((size_t)n_events);
+    std::vector<int64_t> ptr{0};
+    int64_t mx = 0;
+    for (int64_t k = 0; k < n_events; ++k) {
+        const uint32_t t = ht[(size_t)order[(size_t)k]];
+        if (ts.empty() || ts.back() != t) {
+            if (!ts.empty()) ptr.push_back(k);
+            ts.push_back(t);
+        }
+        ids[(size_t)k] = hn[(size_t)order[(size_t)k]];
+    }
+    if (!ts.empty()) ptr.push_back(n_events);
+    for (size_t k = 0; k + 1 < ptr.size(); ++k) mx = std::max<int64_t>(mx, ptr[k + 1] - ptr[k]);
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    block_free(b, b->src);
+    block_free(b, b->src_ts);
+    block_free(b, b->src_ptr);
+    block_free(b, b->src_ids);
+    b->src = nullptr;
+    b->src_ts = nullptr;
+    b->src_ptr = nullptr;
+    b->src_ids = nullptr;
+    b->src_nts = 0;
+    b->src_max = 0;
+    for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    if (n_src == 0) return DTB_OK;
+    DTB_TRY(alloc_n(b, &b->src, n));
+    DTB_TRY(upload(b, b->src, flag.data(), (size_t)n));
+    if (!ts.empty()) {
+        DTB_TRY(alloc_n(b, &b->src_ts, (int64_t)ts.size()));
+        DTB_TRY(alloc_n(b, &b->src_ptr, (int64_t)ptr.size()));
+        DTB_TRY(alloc_n(b, &b->src_ids, n_events));
+        DTB_TRY(upload(b, b->src_ts, ts.data(), 4 * ts.size()));
+        DTB_TRY(upload(b, b->src_ptr, ptr.data(), 8 * ptr.size()));
+        DTB_TRY(upload(b, b->src_ids, ids.data(), 4 * ids.size()));
+    }
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    b->src_nts = (int64_t)ts.size();
+    b->src_max = mx;
+    return DTB_OK;
+}
 
 struct dtb_block : public Block {};
 
@@ -2775,6 +3281,22 @@ int dtb_block_gamma_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, c
     hipFree(dt);
     hipFree(dc);
     return st;
+}
+
+int dtb_block_set_weights(dtb_block* b, int64_t m, const uint32_t* pre, const uint32_t* post,
+                          const uint8_t* channel, const float* w)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    return set_weights_impl(b, m, pre, post, channel, w);
+}
+
+int dtb_block_set_sources(dtb_block* b, int64_t n_src, const uint32_t* sources, const uint8_t* frozen,
+                          int64_t n_events, const uint32_t* step, const uint32_t* neuron)
+{
+    DTB_REQUIRE(b != nullptr, "block is NULL");
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    return set_sources_impl(b, n_src, sources, frozen, n_events, step, neuron);
 }
 
 int dtb_block_replace_props(dtb_block* b, const float* props)

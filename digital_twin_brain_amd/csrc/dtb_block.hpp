@@ -93,6 +93,18 @@ struct Block {
     uint32_t* samples = nullptr;
     int64_t ns = 0;
 
+    // forced-spike sources: src[i] bit0 = source (threshold spikes from the
+    // schedule), bit1 = frozen (no membrane dynamics, no noise), bit2 = fires
+    // at the current step (set by the mark kernel, cleared by the neuron
+    // kernel); schedule: nts ascending step indices src_ts, neurons
+    // src_ids[src_ptr[k] .. src_ptr[k+1]) fire at the step whose 0-based index is src_ts[k]
+    uint8_t* src = nullptr;
+    uint32_t* src_ts = nullptr;
+    int64_t* src_ptr = nullptr;
+    uint32_t* src_ids = nullptr;
+    int64_t src_nts = 0;
+    int64_t src_max = 0; // most neurons of one timestamp
+
     std::vector<int64_t> pop_size;
     int64_t steps_done = 0;
     double t_host = 0.0;
@@ -105,9 +117,15 @@ struct Block {
     bool sorted_ok = false;     // the sorted variant's LDS fits
     int32_t acc_occ = 1;        // bucketed: accumulation workgroups per CU (bucket size follows)
     int32_t accum_unroll = 8;   // bucketed: staged events in flight per accumulation lane
-    bool accum_fixed = false;   // bucketed: int32 fixed-point accumulation (scale 2^acc_shift)
-    bool fixed_ok = false;      // acc_shift >= 16
-    int32_t acc_shift = 0;
+    bool accum_fixed = false;   // bucketed: int32 fixed-point accumulation of the scaled weights
+    bool fixed_ok = false;      // no neuron's largest per-step sum can overflow int32
+    int32_t acc_shift = 0;      // smallest input exponent over neurons with inputs (coarsest quantum 2^-acc_shift)
+    // Input exponents: for neuron i and receptor channel c, e = jexp[4i + c] is
+    // chosen so that the sum of |w| over i's inputs on c, times 2^e, lies in
+    // [2^29, 2^30].  CSR weights, Jp and acc hold values x 2^e and g holds
+    // g x 2^-e (exact power-of-two scalings), so an int32 sum of rounded
+    // scaled weights resolves every neuron relative to its own input scale.
+    int8_t* jexp = nullptr;
     int32_t ncu = 0;            // compute units of the device
     int32_t lds_max = 0;        // LDS bytes a workgroup may use
     ExtDrive ext{};             // external Poisson drive (n = 0: off)

@@ -300,6 +300,37 @@ class NativeBlock:
             raise ValueError(f"props must be [{self.n}, {nat.NPROP}]")
         nat.check(nat.lib().dtb_block_replace_props(self._h, nat.ptr(props)))
 
+    def set_weights(self, pre, post, channel, w):
+        """w_uij[channel][post][pre] = w for existing synapses (channel 0..3:
+        AMPA, NMDA, GABAa, GABAb); a compact block switches to pair weights
+        when a call sets the two channels of a pair differently."""
+        pre = _as_np(pre, np.uint32).reshape(-1)
+        post = _as_np(post, np.uint32).reshape(-1)
+        channel = _as_np(channel, np.uint8).reshape(-1)
+        w = _as_np(w, np.float32).reshape(-1)
+        m = pre.shape[0]
+        if not (post.shape[0] == channel.shape[0] == w.shape[0] == m):
+            raise ValueError("pre, post, channel and w must have the same length")
+        nat.check(nat.lib().dtb_block_set_weights(self._h, int(m), nat.ptr(pre), nat.ptr(post), nat.ptr(channel),
+                                                  nat.ptr(w)))
+        self.pair = bool(self.info()[6])
+
+    def set_sources(self, sources, steps=(), neurons=(), frozen=None):
+        """Forced-spike sources: neurons[k] fires at the step whose 0-based
+        index since creation is steps[k]; sources fire at no other step.
+        frozen (bool per source, or one bool for all) keeps a source's V and
+        turns its noise off (the neurons of an input block)."""
+        src = _as_np(sources, np.uint32).reshape(-1)
+        fr = None
+        if frozen is not None:
+            fr = np.ascontiguousarray(np.broadcast_to(np.asarray(frozen, dtype=np.uint8), src.shape))
+        st = _as_np(np.asarray(steps), np.uint32).reshape(-1)
+        nu = _as_np(np.asarray(neurons), np.uint32).reshape(-1)
+        if st.shape != nu.shape:
+            raise ValueError("steps and neurons must have the same length")
+        nat.check(nat.lib().dtb_block_set_sources(self._h, int(src.shape[0]), nat.ptr(src), nat.ptr(fr),
+                                                  int(st.shape[0]), nat.ptr(st), nat.ptr(nu)))
+
     def set_samples(self, idx):
         idx = _as_np(idx, np.uint32)
         nat.check(nat.lib().dtb_block_set_samples(self._h, int(idx.shape[0]), nat.ptr(idx)))

@@ -425,6 +425,17 @@ class GRUSurrogate:
         return {k: v.cpu().double().numpy() for k, v in self.p.items()}
 
 
+def gemm_kernel_name(M, N, K):
+    """The kernel dtb_mlp_gemm_bf16_nt dispatches an M x N x K product to:
+    128x128 tiles while they give at least 256 workgroups, else 64x64 tiles,
+    else 32x32 tiles with K split over the four wavefronts."""
+    if ((N + 127) // 128) * ((M + 127) // 128) >= 256:
+        return "gemm_nt_kernel<2,2> (128x128 tiles, bf16 MFMA, FP32 accumulate)"
+    if ((N + 63) // 64) * ((M + 63) // 64) >= 256:
+        return "gemm_nt_kernel<1,1> (64x64 tiles, bf16 MFMA, FP32 accumulate)"
+    return "gemm_nt_splitk_kernel (32x32 tiles, K split over 4 wavefronts, bf16 MFMA, FP32 accumulate)"
+
+
 def gemm_throughput(M, N, K, reps=50, warmup=5, device=0):
     """Time dtb_mlp_gemm_bf16_nt (C = A·Bᵀ, bf16 operands, FP32 out) on
     random operands: HIP events on the stream the kernel is launched on.

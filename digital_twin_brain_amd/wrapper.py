@@ -33,21 +33,43 @@ def load_block_dir(path):
     pre = bases[i + input_block_idx] + input_neuron_idx, post = bases[i] +
     output_neuron_idx, kind = input_channel_offset // 2, weights (w0, w1) the
     two columns of `weight` (generation/read_block.py:64-90).
+
+    Input blocks (src_timestamp, src_neuron_idx instead of connections,
+    cuda/src/configuration.cpp:753-762; generation/read_block.py:53-61) are
+    neurons driven by a spike schedule: csr["inputs"] then holds
+    dict(sources, steps, neurons) -- every neuron of an input block is a
+    source, and source neuron bases[i] + src_neuron_idx[k] fires at the step
+    of 0-based index src_timestamp[k] (cuda/src/block.cpp:195-221).  Without
+    input blocks csr["inputs"] is None.
     """
     pat = re.compile(r"block_([0-9]+)\.npz")
     ids = sorted(int(pat.fullmatch(f).group(1)) for f in os.listdir(path) if pat.fullmatch(f))
     if not ids or ids != list(range(len(ids))):
         raise FileNotFoundError(f"no contiguous block_0..block_k .npz files in {path}")
     files = [np.load(os.path.join(path, f"block_{i}.npz")) for i in ids]
+    is_input = []
     for i, f in enumerate(files):
-        if "src_timestamp" in f.files or "src_neuron_idx" in f.files:
-            raise NotImplementedError(f"block_{i}.npz is an input block (src_timestamp, src_neuron_idx): "
-                                      "timed spike injection is not supported yet")
+        has = ("src_timestamp" in f.files, "src_neuron_idx" in f.files)
+        if has[0] != has[1]:
+            raise ValueError(f"block_{i}.npz has only one of src_timestamp and src_neuron_idx")
+        is_input.append(has[0])
     sizes = [f["property"].shape[0] for f in files]
     bases = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     props, pre, post, kind, w0, w1 = [], [], [], [], [], []
+    srcs, ev_t, ev_n = [], [], []
     for i, f in enumerate(files):
         props.append(np.asarray(f["property"], dtype=np.float32)[:, :nat.NPROP])
+        if is_input[i]:
+            ts = np.asarray(f["src_timestamp"]).astype(np.int64)
+            nid = np.asarray(f["src_neuron_idx"]).astype(np.int64)
+            if ts.shape != nid.shape:
+                raise ValueError(f"block_{i}.npz: src_timestamp and src_neuron_idx differ in length")
+            if nid.size and (nid.min() < 0 or nid.max() >= sizes[i] or ts.min() < 0):
+                raise ValueError(f"block_{i}.npz: src_neuron_idx outside the block or negative timestamps")
+            srcs.append(bases[i] + np.arange(sizes[i], dtype=np.int64))
+            ev_t.append(ts)
+            ev_n.append(bases[i] + nid)
+            continue
         blk = np.asarray(f["input_block_idx"]).astype(np.int64)
         pre.append(bases[i + blk] + np.asarray(f["input_neuron_idx"]).astype(np.int64))
         post.append(bases[i] + np.asarray(f["output_neuron_idx"]).astype(np.int64))
@@ -67,11 +89,14 @@ def load_block_dir(path):
     np.add.at(rowptr, pre + 1, 1)
     rowptr = np.cumsum(rowptr)
     w0, w1 = w0[order], w1[order]
+    inputs = None
+    if srcs:
+        inputs = dict(sources=np.concatenate(srcs), steps=np.concatenate(ev_t), neurons=np.concatenate(ev_n))
     return props, dict(rowptr=rowptr, post=post[order].astype(np.int32), kind=kind[order], w0=w0,
-                       w1=None if np.array_equal(w0, w1) else w1)
+                       w1=None if np.array_equal(w0, w1) else w1, inputs=inputs)
 
 
-def save_block_dir(path, props, csr, sizes):
+def save_block_dir(path, props, csr, sizes, inputs=None):
     """Write a network as reference block_{i}.npz files (the fields and dtypes
     of generation/make_block.py:473-479, read back by
     cuda/src/configuration.cpp:586-594,783-792,972-975 and load_block_dir).
@@ -83,7 +108,10 @@ def save_block_dir(path, props, csr, sizes):
     this block), input_neuron_idx (u32, local in the source block),
     input_channel_offset (u8: 0 excitatory, 2 inhibitory), weight (f32 [nnz,2])
     and property (f32 [n,22]).  The format has no delay field, so delays other
-    than 1 are refused."""
+    than 1 are refused.  `inputs` maps a block index to its spike schedule
+    (src_timestamp, src_neuron_idx as local ids): that block is written as an
+    input block (property, src_timestamp u32, src_neuron_idx u32) and must
+    receive no synapses."""
     props = np.asarray(props, dtype=np.float32)
     sizes = np.asarray(sizes, dtype=np.int64)
     n = props.shape[0]
@@ -107,8 +135,17 @@ def save_block_dir(path, props, csr, sizes):
     order = np.lexsort((pre, post))
     os.makedirs(path, exist_ok=True)
     bounds = np.searchsorted(post[order], bases)
+    inputs = inputs or {}
     for i in range(len(sizes)):
         sel = order[bounds[i]:bounds[i + 1]]
+        if i in inputs:
+            if sel.size:
+                raise ValueError(f"block {i} is an input block but receives {sel.size} synapses")
+            ts, nid = (np.asarray(x) for x in inputs[i])
+            np.savez(os.path.join(path, f"block_{i}.npz"),
+                     property=np.ascontiguousarray(props[bases[i]:bases[i + 1]]),
+                     src_timestamp=ts.astype(np.uint32), src_neuron_idx=nid.astype(np.uint32))
+            continue
         np.savez(os.path.join(path, f"block_{i}.npz"),
                  property=np.ascontiguousarray(props[bases[i]:bases[i + 1]]),
                  output_neuron_idx=(post[sel] - bases[i]).astype(np.uint32),
@@ -130,6 +167,10 @@ class LocalBlockWrapper:
             network = NativeBlock.from_csr(props, csr["rowptr"], csr["post"], csr["kind"], csr["w0"], csr["w1"],
                                            dt=float(delta_t), noise_seed=noise_seed, stats_exclude_blocked=True,
                                            device=device)
+            if csr["inputs"] is not None:
+                # input blocks: neurons without membrane dynamics driven by their schedule
+                inp = csr["inputs"]
+                network.set_sources(inp["sources"], steps=inp["steps"], neurons=inp["neurons"], frozen=True)
         self.nb = network
         self.print_stat = print_stat
         self.delta_t = float(delta_t)

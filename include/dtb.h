@@ -232,6 +232,36 @@ int dtb_block_mul_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, con
 int dtb_block_gamma_props_by_pop(dtb_block* b, int64_t m, const uint32_t* pop, const uint32_t* col,
                                  const float* alpha, const float* beta, uint64_t seed);
 
+/* Forced-spike sources: the threshold spikes of the listed neurons come from
+ * a schedule instead of their membrane potential -- block(src=...)
+ * (models/block.py:38-50,104-109) and the input blocks of the block npz
+ * format (src_timestamp, src_neuron_idx: cuda/src/configuration.cpp:753-762,
+ * timed injection in cuda/src/block.cpp:195-221).  Event k fires neuron[k]
+ * at the step whose 0-based index since the block's creation is step[k]
+ * (the first step run has index 0, as the reference's iter and timestamps
+ * count); at every other step a source does not fire.  A source keeps its
+ * membrane dynamics and background noise (block.py); a frozen source
+ * (frozen[k] != 0, an input block's neuron) keeps its V and draws no noise.
+ * Refractoriness, t_last and the statistics follow the forced spikes.  Each
+ * call replaces the previous sources and schedule; n_src = 0 removes them.
+ * frozen may be NULL (none frozen). */
+int dtb_block_set_sources(dtb_block* b, int64_t n_src, const uint32_t* sources, const uint8_t* frozen,
+                          int64_t n_events, const uint32_t* step, const uint32_t* neuron);
+
+/* Change existing synaptic weights: w_uij[channel][post][pre] = w for each
+ * request (block.update_conn_weight, models/block.py:172-177, with the
+ * request as the (channel, post, pre) index of w_uij).  Channels 0/1 are
+ * the AMPA/NMDA weights of a kind-0 synapse, 2/3 the GABAa/GABAb weights of
+ * a kind-1 synapse.  Where several synapses join one pair on one receptor
+ * group, the first takes w and the others 0, so they sum to w as the dense
+ * w_uij entry does; the last of several requests for the same entry wins.
+ * A block with compact weights switches to pair weights when a call sets
+ * the two channels of a pair differently (or only one of them).  A request
+ * naming no existing synapse fails with DTB_INVALID_PARAMETER and changes
+ * nothing.  The dynamic state continues unchanged. */
+int dtb_block_set_weights(dtb_block* b, int64_t m, const uint32_t* pre, const uint32_t* post, const uint8_t* channel,
+                          const float* w);
+
 /* Replace every property (block.update_property, models/block.py:152-170). */
 int dtb_block_replace_props(dtb_block* b, const float* props);
 

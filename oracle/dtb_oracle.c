@@ -231,6 +231,15 @@ typedef struct {
     int32_t ext_n;
     float ext_w;
     uint32_t ext_thr[16];
+    /* forced-spike sources (models/block.py:38-50,104-109): srcflag bit0 =
+     * source, bit1 = frozen (an input block's neuron: no membrane update and
+     * no noise, cuda/src/block.cpp:195-221,277-283); events (ev_step,
+     * ev_neuron) sorted by step, fire[] marks the current step's */
+    uint8_t* srcflag;
+    uint8_t* fire;
+    int64_t nev;
+    uint32_t* ev_step;
+    uint32_t* ev_neuron;
 } orc_block;
 
 #define P(b, i, c) ((REAL)(b)->props[(int64_t)(i) * 22 + (c)])
@@ -319,6 +328,7 @@ void RSUFFIX(orc_block_destroy)(void* h)
     free(b->prow); free(b->ppre); free(b->pkind); free(b->pdelay); free(b->pw0); free(b->pw1); free(b->shist);
     free(b->V); free(b->J); free(b->I_ui); free(b->I_syn); free(b->t_last); free(b->ring);
     free(b->active); free(b->spike); free(b->sub);
+    free(b->srcflag); free(b->fire); free(b->ev_step); free(b->ev_neuron);
     free(b);
 }
 
@@ -364,6 +374,12 @@ void RSUFFIX(orc_block_step)(void* h, int32_t noise_mode, float noise_rate, cons
     b->t += dt; /* models/block.py:133 */
     const REAL t = b->t;
     const int64_t step = b->step;
+    /* forced spikes of this step: src[:, iter] with iter = step - 1 (models/block.py:105) */
+    if (b->srcflag) {
+        memset(b->fire, 0, (size_t)n);
+        for (int64_t k = 0; k < b->nev; ++k)
+            if ((int64_t)b->ev_step[k] == step - 1) b->fire[b->ev_neuron[k]] = 1;
+    }
     /* update_Vi, models/block.py:89-111 */
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
@@ -376,11 +392,17 @@ void RSUFFIX(orc_block_step)(void* h, int32_t noise_mode, float noise_rate, cons
         REAL Vi = not_sat ? Vn : P(b, i, 9);
         uint8_t act = Vi >= P(b, i, 8);
         REAL vth = P(b, i, 8);
-        b->V[i] = Vi < vth ? Vi : vth;
+        REAL Vnew = Vi < vth ? Vi : vth;
+        uint8_t sf = b->srcflag ? b->srcflag[i] : 0;
+        if (sf & 1) {
+            act = b->fire[i]; /* active[src_neuron] = src[:, iter] */
+            if (sf & 2) Vnew = V;
+        }
+        b->V[i] = Vnew;
         b->active[i] = act;
         float u = uniforms ? uniforms[i] : orc_noise_uniform(seed, (uint64_t)step, (uint64_t)i);
         float p = noise_mode == 1 ? b->props[i * 22 + 0] : noise_rate;
-        uint8_t noise = u < p;
+        uint8_t noise = (u < p) && !(sf & 2);
         b->spike[i] = isolated ? noise : (uint8_t)(noise | act);
     }
     /* update_J_ui, models/block.py:68-74: J = J*exp(-dt/tau) + W.s */
@@ -487,6 +509,74 @@ void RSUFFIX(orc_block_get)(void* h, int32_t field, void* dst)
     case 6: memcpy(dst, b->spike, (size_t)n); break;
     case 7: memcpy(dst, &b->t, sizeof(REAL)); break;
     default: break;
+    }
+}
+
+/* New connectivity, state kept (push form only): w_uij replaced between
+ * steps, as block.update_conn_weight does (models/block.py:172-177). */
+void RSUFFIX(orc_block_replace_csr)(void* h, int64_t nnz, const int64_t* rowptr, const int32_t* post,
+                                    const uint8_t* kind, const float* w0, const float* w1)
+{
+    orc_block* b = (orc_block*)h;
+    free(b->post); free(b->kind); free(b->delay); free(b->w0); free(b->w1);
+    b->nnz = nnz;
+    memcpy(b->rowptr, rowptr, sizeof(int64_t) * (size_t)(b->n + 1));
+    b->post = (int32_t*)xcalloc((size_t)nnz, sizeof(int32_t));
+    b->kind = (uint8_t*)xcalloc((size_t)nnz, 1);
+    b->delay = (uint8_t*)xcalloc((size_t)nnz, 1);
+    b->w0 = (float*)xcalloc((size_t)nnz, sizeof(float));
+    b->w1 = (float*)xcalloc((size_t)nnz, sizeof(float));
+    memcpy(b->post, post, sizeof(int32_t) * (size_t)nnz);
+    memcpy(b->kind, kind, (size_t)nnz);
+    memset(b->delay, 1, (size_t)nnz);
+    memcpy(b->w0, w0, sizeof(float) * (size_t)nnz);
+    memcpy(b->w1, w1 ? w1 : w0, sizeof(float) * (size_t)nnz);
+}
+
+/* Forced-spike sources and their schedule: event k fires neuron[k] at the
+ * step whose 0-based index is step[k]; frozen[k] != 0 marks an input-block
+ * neuron (no membrane update, no noise).  n_src = 0 removes them. */
+void RSUFFIX(orc_block_set_sources)(void* h, int64_t n_src, const uint32_t* sources, const uint8_t* frozen,
+                                    int64_t n_events, const uint32_t* step, const uint32_t* neuron)
+{
+    orc_block* b = (orc_block*)h;
+    free(b->srcflag); free(b->fire); free(b->ev_step); free(b->ev_neuron);
+    b->srcflag = NULL; b->fire = NULL; b->ev_step = NULL; b->ev_neuron = NULL; b->nev = 0;
+    if (n_src <= 0) return;
+    b->srcflag = (uint8_t*)xcalloc((size_t)b->n, 1);
+    b->fire = (uint8_t*)xcalloc((size_t)b->n, 1);
+    for (int64_t k = 0; k < n_src; ++k) b->srcflag[sources[k]] = (uint8_t)(1 | (frozen && frozen[k] ? 2 : 0));
+    b->nev = n_events;
+    b->ev_step = (uint32_t*)xcalloc((size_t)n_events, sizeof(uint32_t));
+    b->ev_neuron = (uint32_t*)xcalloc((size_t)n_events, sizeof(uint32_t));
+    memcpy(b->ev_step, step, sizeof(uint32_t) * (size_t)n_events);
+    memcpy(b->ev_neuron, neuron, sizeof(uint32_t) * (size_t)n_events);
+}
+
+/* Replace part of the state between steps (test infrastructure: re-aligns
+ * the oracle with another implementation's state after a threshold tie so
+ * the following steps are compared from the same starting point).
+ * field: 0 V[n], 1 J[4n], 4 t_last[n] (REAL).  I_ui and I_syn are then
+ * recomputed from V and J as update_I_syn does (models/block.py:83-87). */
+void RSUFFIX(orc_block_set)(void* h, int32_t field, const void* src)
+{
+    orc_block* b = (orc_block*)h;
+    int64_t n = b->n;
+    switch (field) {
+    case 0: memcpy(b->V, src, sizeof(REAL) * (size_t)n); break;
+    case 1: memcpy(b->J, src, sizeof(REAL) * (size_t)n * 4); break;
+    case 4: memcpy(b->t_last, src, sizeof(REAL) * (size_t)n); break;
+    default: return;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        REAL V = b->V[i];
+        REAL sum = 0;
+        for (int k = 0; k < 4; ++k) {
+            REAL I = P(b, i, 10 + k) * (P(b, i, 14 + k) - V) * b->J[(int64_t)k * n + i];
+            b->I_ui[(int64_t)k * n + i] = I;
+            sum += I;
+        }
+        b->I_syn[i] = sum;
     }
 }
 

@@ -57,6 +57,9 @@ def _lib(real="f64"):
     getattr(lib, "orc_block_num_sub" + sfx).argtypes = [_vp]
     getattr(lib, "orc_block_num_sub" + sfx).restype = _c_i32
     getattr(lib, "orc_block_get" + sfx).argtypes = [_vp, _c_i32, _vp]
+    getattr(lib, "orc_block_set" + sfx).argtypes = [_vp, _c_i32, _vp]
+    getattr(lib, "orc_block_set_sources" + sfx).argtypes = [_vp, _c_i64, _vp, _vp, _c_i64, _vp, _vp]
+    getattr(lib, "orc_block_replace_csr" + sfx).argtypes = [_vp, _c_i64, _vp, _vp, _vp, _vp, _vp]
     getattr(lib, "orc_block_set_external" + sfx).argtypes = [_vp, _c_i32, _vp, _c_f32]
     getattr(lib, "orc_ext_count" + sfx).argtypes = [_c_u64, _c_u64, _c_u64, _c_i32, _vp]
     getattr(lib, "orc_ext_count" + sfx).restype = ctypes.c_uint32
@@ -213,6 +216,45 @@ class OracleBlock:
             out = np.zeros(self.n, dtype=self.dtype)
         getattr(self.lib, "orc_block_get" + self.sfx)(self.h, f, _p(out))
         return out
+
+    def replace_csr(self, csr):
+        """New connectivity (delay 1, push form), state kept: the reference's
+        w_uij assignment between steps (models/block.py:172-177)."""
+        assert not self.pull
+        rowptr = np.ascontiguousarray(csr["rowptr"], dtype=np.int64)
+        assert rowptr.shape == (self.n + 1,)
+        post = np.ascontiguousarray(csr["post"], dtype=np.int32)
+        kind = np.ascontiguousarray(csr["kind"], dtype=np.uint8)
+        w0 = np.ascontiguousarray(csr["w0"], dtype=np.float32)
+        w1 = csr.get("w1")
+        w1 = None if w1 is None else np.ascontiguousarray(w1, dtype=np.float32)
+        self.nnz = post.shape[0]
+        getattr(self.lib, "orc_block_replace_csr" + self.sfx)(self.h, self.nnz, _p(rowptr), _p(post), _p(kind),
+                                                               _p(w0), _p(w1))
+
+    def set_sources(self, sources, steps=(), neurons=(), frozen=None):
+        """Forced-spike sources (models/block.py:38-50,104-109): neurons[k]
+        fires at the step whose 0-based index is steps[k]; frozen sources
+        (input-block neurons, cuda/src/block.cpp:195-221) keep V and draw no
+        noise."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        fr = None if frozen is None else np.ascontiguousarray(np.broadcast_to(frozen, src.shape), dtype=np.uint8)
+        st = np.ascontiguousarray(steps, dtype=np.uint32)
+        nu = np.ascontiguousarray(neurons, dtype=np.uint32)
+        assert st.shape == nu.shape
+        self._keep["src"] = (src, fr, st, nu)
+        getattr(self.lib, "orc_block_set_sources" + self.sfx)(self.h, src.shape[0], _p(src), _p(fr), st.shape[0],
+                                                               _p(st), _p(nu))
+
+    def set(self, name, values):
+        """Overwrite V, J ([4, n]) or t_last between steps; I_ui and I_syn are
+        recomputed from V and J (test infrastructure: re-aligning the oracle
+        with the GPU state after a threshold tie)."""
+        if name not in ("V", "J", "t_last"):
+            raise ValueError("only V, J and t_last can be set")
+        shape = (4, self.n) if name == "J" else (self.n,)
+        v = np.ascontiguousarray(np.asarray(values, dtype=self.dtype).reshape(shape))
+        getattr(self.lib, "orc_block_set" + self.sfx)(self.h, self.FIELDS[name], _p(v))
 
     @property
     def t(self):

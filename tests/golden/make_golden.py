@@ -66,6 +66,19 @@ def csr_to_torch_sparse(n, csr):
     return torch.sparse_coo_tensor(idx, torch.from_numpy(vals), (4, n, n)).coalesce()
 
 
+def drop_inputs(n, csr, targets):
+    """The CSR without the synapses onto `targets` (neurons left with no
+    afferents, which models/block.py:38-48 takes as the spike sources)."""
+    rowptr = csr["rowptr"]
+    row = np.repeat(np.arange(n), np.diff(rowptr))
+    keep = ~np.isin(csr["post"], targets)
+    out = {k: (v[keep] if isinstance(v, np.ndarray) and v.shape[:1] == keep.shape else v) for k, v in csr.items()}
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, row[keep] + 1, 1)
+    out["rowptr"] = np.cumsum(rp)
+    return out
+
+
 def dyadic(csr, bits=8):
     """Round weights to multiples of 2**-bits so every float32 partial sum of
     W.s is exact (the reference's float32 product then equals the FP64 sum)."""
@@ -77,12 +90,13 @@ def dyadic(csr, bits=8):
 
 
 def run_block_case(ref_block, name, n, n_exc, csr, props, steps, noise_rate, isolated, dt, seed, dense,
-                   keep_every=1, snap_steps=()):
+                   keep_every=1, snap_steps=(), src=None):
     torch.set_default_dtype(torch.float64)
     W = csr_to_torch_sparse(n, csr)
     if dense:
         W = W.to_dense()
-    B = ref_block(node_property=torch.from_numpy(props.astype(np.float64)), w_uij=W, delta_t=dt)
+    kw = {} if src is None else dict(src=torch.from_numpy(src.astype(bool)))
+    B = ref_block(node_property=torch.from_numpy(props.astype(np.float64)), w_uij=W, delta_t=dt, **kw)
     step_box = [0]
 
     def fake_rand(*size, **kw):
@@ -107,13 +121,24 @@ def run_block_case(ref_block, name, n, n_exc, csr, props, steps, noise_rate, iso
                 snaps[f"J_{s}"] = B.J_ui.numpy().copy()
                 snaps[f"I_ui_{s}"] = B.I_ui.numpy().copy()
                 snaps[f"I_syn_{s}"] = B.I_syn.numpy().copy()
+        extra = {}
+        t_final = float(B.t)
+        if src is not None:
+            # the step at which the reference stops: update_Vi reads src[:, iter] once more
+            # after advancing iter (models/block.py:105-107)
+            extra = dict(src=src.astype(np.uint8), src_neuron=B.src_neuron.numpy().astype(np.int64), raises_at=-1)
+            step_box[0] = steps + 1
+            try:
+                B.run(noise_rate=noise_rate, isolated=isolated)
+            except IndexError:
+                extra["raises_at"] = steps + 1
     out = dict(
         n=n, n_exc=n_exc, dt=dt, seed=seed, steps=steps, noise_rate=noise_rate, isolated=int(isolated),
         keep_every=keep_every, dense=int(dense), props=props, rowptr=csr["rowptr"], post=csr["post"],
         kind=csr["kind"], w0=csr["w0"], w1=csr["w1"] if csr.get("w1") is not None else csr["w0"],
         V=np.stack(V_hist), active=np.packbits(np.stack(act_hist), axis=1),
         spike=np.packbits(np.stack(spk_hist), axis=1), sum_act=np.stack(sum_hist), mean_v=np.stack(mean_hist),
-        t_final=float(B.t), snap_steps=np.array(sorted(snap_steps), dtype=np.int64), **snaps)
+        t_final=t_final, snap_steps=np.array(sorted(snap_steps), dtype=np.int64), **snaps, **extra)
     np.savez_compressed(os.path.join(HERE, f"block_{name}.npz"), **out)
     nsp = int(np.stack(act_hist).sum())
     print(f"{name}: n={n} steps={steps} threshold spikes={nsp} delivered={int(np.stack(spk_hist).sum())}")
@@ -173,6 +198,18 @@ def main(ref_root):
     props = make_props(n, ne, rng, 0.02, 0.1, T_ref=5.0)
     run_block_case(ref_block, "isolated64", n, ne, csr, props, steps=200, noise_rate=2.0 ** -4, isolated=True,
                    dt=0.5, seed=13, dense=True, snap_steps=(1, 100, 200))
+    # 4) forced-spike sources (models/block.py:38-50,104-109): six neurons with no afferents driven by a
+    # spike schedule of 201 columns; the reference runs 200 steps and raises at the 201st.  W is sparse:
+    # with current torch, isinstance(w, torch.sparse.Tensor) holds for dense tensors too, and the
+    # dense branch of models/block.py:42-45 is never taken
+    n, ne = 64, 48
+    csr = dyadic(orc.gen_fixed_indegree(n, ne, 12, 1, 0.0, 1.0, True, seed=41))
+    srcs = np.array([0, 7, 20, 33, 50, 63])
+    csr = drop_inputs(n, csr, srcs)
+    props = make_props(n, ne, rng, 0.01, 0.08)
+    sched = (rng.random((srcs.shape[0], 201)) < 0.08).astype(np.uint8)
+    run_block_case(ref_block, "src64", n, ne, csr, props, steps=200, noise_rate=2.0 ** -6, isolated=False,
+                   dt=0.1, seed=17, dense=False, snap_steps=(1, 100, 200), src=sched)
     run_bold_case(ref_bold, "v18", 18, 600, seed=3)
 
 

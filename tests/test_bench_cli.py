@@ -22,3 +22,9 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 def test_a_multi_rank_launch_is_refused_too():
     r = _bench("--gpus", "1", env={"WORLD_SIZE": "8"})
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_steps_that_are_not_whole_bold_samples_are_refused():
+    for steps in ("0", "5", "25"):
+        r = _bench("--steps", steps)
+        assert r.returncode == 2 and "multiple of 10" in r.stderr and r.stdout == "", steps

@@ -72,14 +72,66 @@ def test_writer_refuses_what_the_format_cannot_hold(tmp_path):
     save_block_dir(str(tmp_path), props, ref, [n])
 
 
-def test_reader_refuses_input_blocks(tmp_path):
-    """Input blocks (src_timestamp, src_neuron_idx: timed spike injection,
-    cuda/src/configuration.cpp:753-762) are not supported: the reader says so
-    instead of loading a network without them."""
-    n = 200
-    ref = orc.gen_fixed_indegree(n, 160, 10, 1, 0.0, 1.0, False, seed=2)
-    save_block_dir(str(tmp_path), np.zeros((n, 22), np.float32), ref, [n])
-    np.savez(os.path.join(tmp_path, "block_1.npz"), src_timestamp=np.arange(4, dtype=np.uint32),
-             src_neuron_idx=np.zeros(4, np.uint32))
-    with pytest.raises(NotImplementedError, match="src_timestamp"):
+def _with_input_block(n_in, n, seed):
+    """A network whose first n_in neurons receive no synapses (an input block)
+    and project into the other n neurons."""
+    rng = np.random.default_rng(seed)
+    ref = orc.gen_fixed_indegree(n, int(0.8 * n), 10, 1, 0.0, 1.0, False, seed=seed)
+    # shift the recurrent network behind the input block and add input synapses from it
+    post = ref["post"].astype(np.int64) + n_in
+    pre = np.repeat(np.arange(n), np.diff(ref["rowptr"])) + n_in
+    k_in = 3
+    ipre = rng.integers(0, n_in, size=n * k_in)
+    ipost = np.repeat(np.arange(n_in, n_in + n), k_in)
+    pre = np.concatenate([pre, ipre])
+    post = np.concatenate([post, ipost])
+    kind = np.concatenate([ref["kind"], np.zeros(n * k_in, np.uint8)])
+    w0 = np.concatenate([ref["w0"], rng.uniform(0, 1, n * k_in).astype(np.float32)])
+    o = np.lexsort((post, pre))
+    rowptr = np.zeros(n_in + n + 1, np.int64)
+    np.add.at(rowptr, pre + 1, 1)
+    csr = dict(rowptr=np.cumsum(rowptr), post=post[o].astype(np.int32), kind=kind[o], w0=w0[o], w1=None,
+               delay=None)
+    ts = np.sort(rng.integers(0, 300, size=120)).astype(np.uint32)
+    nid = rng.integers(0, n_in, size=120).astype(np.uint32)
+    return csr, ts, nid
+
+
+def test_input_blocks_round_trip(tmp_path):
+    """Input blocks (property, src_timestamp, src_neuron_idx: timed spike
+    injection, cuda/src/configuration.cpp:753-762, generation/read_block.py:53-61)
+    are written and read back: every neuron of the input block is a source and
+    the schedule keeps its (timestamp, neuron) pairs."""
+    n_in, n = 40, 200
+    csr, ts, nid = _with_input_block(n_in, n, seed=3)
+    props = np.zeros((n_in + n, 22), np.float32)
+    props[:, 3] = np.arange(n_in + n) % 3
+    save_block_dir(str(tmp_path), props, csr, [n_in, n], inputs={0: (ts, nid)})
+    z = np.load(os.path.join(tmp_path, "block_0.npz"))
+    assert set(z.files) == {"property", "src_timestamp", "src_neuron_idx"}
+    assert z["src_timestamp"].dtype == np.uint32 and z["src_neuron_idx"].dtype == np.uint32
+    p2, csr2 = load_block_dir(str(tmp_path))
+    assert np.array_equal(p2, props)
+    for k in ("rowptr", "post", "kind", "w0"):
+        assert np.array_equal(csr2[k], csr[k]), k
+    inp = csr2["inputs"]
+    assert np.array_equal(inp["sources"], np.arange(n_in))
+    assert np.array_equal(inp["steps"], ts) and np.array_equal(inp["neurons"], nid)
+    # the reference's dense form of the same schedule (generation/read_block.py:60-61)
+    dense = np.zeros((n_in, int(ts.max()) + 1), np.uint8)
+    dense[nid, ts] = 1
+    mine = np.zeros_like(dense)
+    mine[inp["neurons"], inp["steps"]] = 1
+    assert np.array_equal(dense, mine)
+
+
+def test_input_block_refusals(tmp_path):
+    n_in, n = 40, 200
+    csr, ts, nid = _with_input_block(n_in, n, seed=4)
+    props = np.zeros((n_in + n, 22), np.float32)
+    with pytest.raises(ValueError, match="input block"):
+        save_block_dir(str(tmp_path), props, csr, [n_in, n], inputs={1: (ts, nid)})
+    save_block_dir(str(tmp_path), props, csr, [n_in, n], inputs={0: (ts, nid)})
+    np.savez(os.path.join(tmp_path, "block_0.npz"), property=props[:n_in], src_timestamp=ts)
+    with pytest.raises(ValueError, match="only one of"):
         load_block_dir(str(tmp_path))

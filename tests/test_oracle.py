@@ -27,6 +27,13 @@ def case_csr(c):
     return dict(rowptr=c["rowptr"], post=c["post"], kind=c["kind"], w0=c["w0"], w1=c["w1"], delay=None)
 
 
+def case_schedule(c):
+    """Forced-spike schedule of a golden case with sources: src[k, t] = 1 fires
+    src_neuron[k] at the step of 0-based index t (models/block.py:105)."""
+    k, t = np.nonzero(c["src"])
+    return c["src_neuron"].astype(np.uint32), t.astype(np.uint32), c["src_neuron"][k].astype(np.uint32)
+
+
 # ---------------------------------------------------------------- Philox
 def test_philox_known_answers():
     kat = [
@@ -91,12 +98,19 @@ def test_generator_members_stay_inside_their_member():
 
 
 # ------------------------------------------------- block step vs golden
-@pytest.mark.parametrize("name,exact", [("dense64", True), ("isolated64", True), ("sparse400", False)])
+@pytest.mark.parametrize("name,exact", [("dense64", True), ("isolated64", True), ("sparse400", False),
+                                        ("src64", True)])
 @pytest.mark.parametrize("pull", [False, True])
 def test_oracle_block_matches_reference_golden(golden_dir, name, exact, pull):
     c = load_case(golden_dir, name)
     n, steps, keep = int(c["n"]), int(c["steps"]), int(c["keep_every"])
     ob = orc.OracleBlock(c["props"], case_csr(c), float(c["dt"]), pull=pull)
+    if "src" in c:
+        # the reference's sources are exactly the neurons with no afferents (models/block.py:42-48)
+        assert np.array_equal(np.setdiff1d(np.arange(n), c["post"]), c["src_neuron"])
+        # it reads one schedule column past the one it uses, so T columns drive T - 1 steps
+        assert int(c["raises_at"]) == c["src"].shape[1] == steps + 1
+        ob.set_sources(*case_schedule(c))
     act = np.unpackbits(c["active"], axis=1)[:, :n]
     spk = np.unpackbits(c["spike"], axis=1)[:, :n]
     snaps = set(int(s) for s in c["snap_steps"])
