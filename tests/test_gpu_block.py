@@ -734,3 +734,80 @@ def test_fixed_point_accumulation_matches_float(pair):
             Jf, Jr = outs[0][0], outs[1][0]
             np.testing.assert_allclose(Jf, Jr, rtol=1e-6, atol=1e-6 * np.abs(Jr).max())
             np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-5)
+
+
+def _heterogeneous_csr(n, ne, seed):
+    """Per-target in-degree log-uniform in [10, 1000] (sources uniform, not
+    the target itself), delays 1-3, weights U[0, 1) times a per-target scale
+    log-uniform in [1e-3, 1]: per-step input sums that differ by about six
+    orders of magnitude between the neurons of one block."""
+    rng = np.random.default_rng(seed)
+    k_of = np.exp(rng.uniform(np.log(10), np.log(1000), n)).astype(np.int64)
+    scale = np.exp(rng.uniform(np.log(1e-3), 0.0, n))
+    post = np.repeat(np.arange(n), k_of)
+    pre = rng.integers(0, n - 1, post.shape[0])
+    pre = pre + (pre >= post)
+    o = np.argsort(pre, kind="stable")
+    pre, post = pre[o], post[o]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, pre + 1, 1)
+    w = (rng.random(post.shape[0]) * scale[post]).astype(np.float32)
+    csr = dict(rowptr=np.cumsum(rp), post=post.astype(np.int32), kind=(pre >= ne).astype(np.uint8),
+               delay=rng.integers(1, 4, post.shape[0]).astype(np.uint8), w0=w, w1=None)
+    return csr, k_of, scale
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_fixed_point_per_neuron_scale_on_heterogeneous_inputs(mode):
+    """Targets whose inputs differ in scale by 1e3 and in count by 1e2 within
+    one block: every neuron's J is within 1e-5 of the FP64 oracle relative to
+    that neuron's own input scale (its mean incoming weight on the channel),
+    for bucketed fixed-point delivery (mode 1, the default for large blocks)
+    and direct float atomics (mode 0)."""
+    from digital_twin_brain_amd import NativeBlock
+
+    n, ne = 30000, 24000
+    csr, k_of, scale = _heterogeneous_csr(n, ne, 21)
+    rng = np.random.default_rng(22)
+    props = _plain_like_props(n, ne, rng, g_scale=0.5)
+    nb = NativeBlock.from_csr(props, csr["rowptr"], csr["post"], csr["kind"], csr["w0"], None, csr["delay"], dt=0.1,
+                              noise_seed=5)
+    nb.set_launch(deliver_mode=mode, deliver_grid=256)
+    if mode == 1:
+        assert nb.get_launch()["accum_fixed"] >= 16, "fixed-point accumulation not selected"
+    # the stored weights read back unscaled, bit for bit
+    lib_csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
+                   w0=nb.get("w0"), w1=None)
+    ref = rows_by_delay(csr)
+    for k in ("rowptr", "post", "kind", "delay", "w0"):
+        assert np.array_equal(lib_csr[k], ref[k]), f"CSR {k} read back differs"
+    ob = orc.OracleBlock(props, lib_csr, 0.1)
+    # per (neuron, receptor group): mean incoming weight = the neuron's own input scale
+    post = lib_csr["post"].astype(np.int64)
+    grp = lib_csr["kind"].astype(np.int64)
+    wsum = np.zeros((2, n))
+    wcnt = np.zeros((2, n))
+    np.add.at(wsum, (grp, post), lib_csr["w0"].astype(np.float64))
+    np.add.at(wcnt, (grp, post), 1.0)
+    wbar = np.where(wcnt > 0, wsum / np.maximum(wcnt, 1), 0.0)
+    wbar4 = np.stack([wbar[0], wbar[0], wbar[1], wbar[1]])
+    ev = np.zeros(50, np.uint64)
+    worst = 0.0
+    for b in range(4):
+        # noise spikes only: the arrivals do not feed back into the raster
+        nb.run(50, noise_rate=0.02, isolated=True, graph=True, events=ev)
+        for _ in range(50):
+            ob.run(noise_rate=0.02, seed=5, isolated=True)
+        Jg, Jo = nb.get("J").astype(np.float64), ob.get("J")
+        err = np.abs(Jg - Jo)
+        tol = RTOL * (np.abs(Jo) + wbar4)
+        assert (err <= tol).all(), (
+            f"batch {b}: J beyond 1e-5 of the neuron's own scale at {np.argwhere(err > tol)[:4].tolist()}, worst "
+            f"{float((err / np.maximum(tol, 1e-300)).max()):.3g} x tolerance")
+        worst = max(worst, float((err / np.maximum(tol, 1e-300)).max()))
+        np.testing.assert_allclose(nb.get("V"), ob.get("V"), rtol=RTOL)
+    # the test spans the scales it claims: small- and large-scale neurons both received input
+    got = np.abs(Jo).sum(0) > 0
+    assert scale[got].min() < 3e-3 and scale[got].max() > 0.3 and k_of[got].min() < 30
+    print(f"mode {mode}: worst J error {worst:.3g} x tolerance, launch {nb.get_launch()}")
+    nb.close()

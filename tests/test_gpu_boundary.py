@@ -66,6 +66,15 @@ def _drop_inputs(n, csr, targets):
     return out
 
 
+def _keep_synapses(n, csr, keep):
+    row = np.repeat(np.arange(n), np.diff(csr["rowptr"]))
+    out = {k: (v[keep] if isinstance(v, np.ndarray) and v.shape[:1] == keep.shape else v) for k, v in csr.items()}
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, row[keep] + 1, 1)
+    out["rowptr"] = np.cumsum(rp)
+    return out
+
+
 def test_dropin_block_src_matches_reference_golden(golden_dir):
     from digital_twin_brain_amd.block import block
 
@@ -174,6 +183,12 @@ def test_update_conn_weight_matches_oracle():
     csr = orc.gen_fixed_indegree(n, ne, 12, 1, 0.0, 1.0, True, seed=32)
     for k in ("w0", "w1"):
         csr[k] = (np.floor(csr[k] * 256) / 256).astype(np.float32)
+    # a dense w_uij holds one entry per (channel, post, pre): keep the first of repeated pairs
+    pre = np.repeat(np.arange(n), np.diff(csr["rowptr"]))
+    _, first = np.unique(pre * n + csr["post"], return_index=True)
+    keep = np.zeros(pre.shape[0], bool)
+    keep[first] = True
+    csr = _keep_synapses(n, csr, keep)
     p = _lif_props(n, rng, g_scale=1.5)
     pre = np.repeat(np.arange(n), np.diff(csr["rowptr"]))
     W = torch.zeros(4, n, n, dtype=torch.float32)

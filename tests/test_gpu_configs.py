@@ -5,12 +5,16 @@ U[0, 1], delay 1, noise 10 Hz, dt = 0.1 ms): 100 warm-up + 8,000 steps
 against the FP64 oracle (models/block.py:116-150) on the same synapses and
 the same Philox noise.  Bar, per step:
 - delivered noise spikes identical (same uniforms);
-- threshold rasters identical until the first divergence, which is
-  reported; a divergence is accepted only at a threshold tie, where the
-  oracle's V lies within the V tolerance (1e-5 relative) of V_th.  The
-  oracle is then re-aligned to the GPU state (V, J, t_last) and the
-  comparison continues, so every one of the 8,100 steps is checked;
-- V within 1e-5 relative, J within 1e-5 relative (+1e-12);
+- per step, from a common state (the oracle is set to the GPU's V, J and
+  t_last after every step): threshold rasters identical except at a
+  threshold tie, where the oracle's V lies within the V tolerance (1e-5
+  relative) of V_th; V within 1e-5 relative, J within 1e-5 relative
+  (+1e-12);
+- free running (a second oracle never re-aligned): rasters identical until
+  the first divergence, which is reported and must come after step 50.
+  This network is strongly coupled, so FP32 and FP64 trajectories separate
+  within a few hundred steps (an FP32 build of the oracle itself diverges
+  from the FP64 one at step 84);
 - spike lists (compaction order) and event counts bit-exact;
 - per-population threshold counts exact, mean V within 1e-5 relative.
 
@@ -49,13 +53,17 @@ def test_plain_block_8000_steps_match_oracle():
     g = props[:, 10:14]
     assert (g >= 0.001).all() and (g <= 0.05).all()
     outdeg = np.diff(csr["rowptr"])
+    # ob: re-aligned to the GPU state after every step (each step checked from a common state);
+    # free: never re-aligned, followed until its first threshold divergence, which is reported
     ob = orc.OracleBlock(props, csr, 0.1)
+    free = orc.OracleBlock(props, csr, 0.1)
     vth = props[:, 8].astype(np.float64)
     steps = 8100
     freqs = np.zeros(nb.S, np.uint32)
     vmeans = np.zeros(nb.S, np.float32)
     ev = np.zeros(1, np.uint64)
     first_div = None
+    free_worst_v = 0.0
     ties = []
     worst_v = worst_j = 0.0
     n_spk = n_thr = 0
@@ -66,38 +74,45 @@ def test_plain_block_8000_steps_match_oracle():
         s_g, s_o = nb.get("spike").astype(bool), ob.get("spike").astype(bool)
         V_g, V_o = nb.get("V").astype(np.float64), ob.get("V")
         J_g, J_o = nb.get("J").astype(np.float64), ob.get("J")
+        if first_div is None:
+            free.run(noise_mode=1, seed=3)
+            if np.array_equal(a_g, free.get("active").astype(bool)):
+                Vf = free.get("V")
+                free_worst_v = max(free_worst_v, float((np.abs(V_g - Vf) / np.abs(Vf)).max()))
+            else:
+                first_div = s
         # noise spikes use the same uniforms: delivered non-threshold spikes agree everywhere
         assert np.array_equal(s_g & ~a_g, s_o & ~a_o), f"noise spikes differ at step {s}"
         # events: the delivered rows' out-degrees, and the compacted list in ascending order
         assert int(ev[0]) == int(outdeg[s_g].sum()), f"event count at step {s}"
         assert np.array_equal(nb.get("spike_list"), np.flatnonzero(s_g).astype(np.int32)), f"spike list at step {s}"
+        n_thr += int(a_g.sum())
+        n_spk += int(s_g.sum())
         if not np.array_equal(a_g, a_o):
             bad = np.flatnonzero(a_g != a_o)
             # a threshold tie: the oracle's V (clamped at V_th when it fired) sits within tolerance of V_th
             gap = np.abs(np.where(a_o[bad], V_g[bad], V_o[bad]) - vth[bad])
             assert (gap <= RTOL * np.abs(vth[bad])).all(), (
                 f"step {s}: threshold rasters differ at {bad[:8]} away from a tie (gap {gap.max():.3g} mV)")
-            if first_div is None:
-                first_div = s
             ties.append((s, int(bad.shape[0])))
-            ob.set("V", V_g)
-            ob.set("J", J_g)
-            ob.set("t_last", nb.get("t_last"))
-            continue
-        n_thr += int(a_g.sum())
-        n_spk += int(s_g.sum())
-        dv = float((np.abs(V_g - V_o) / np.abs(V_o)).max())
-        dj = float((np.abs(J_g - J_o) - RTOL * np.abs(J_o)).max())
-        worst_v = max(worst_v, dv)
-        worst_j = max(worst_j, float((np.abs(J_g - J_o) / (np.abs(J_o) + 1e-12)).max()))
-        assert dv <= RTOL, f"V differs by {dv:.3g} relative at step {s}"
-        assert dj <= 1e-12, f"J differs beyond 1e-5 relative at step {s}"
-        assert np.array_equal(freqs.astype(np.int64), sa.astype(np.int64)), f"population counts at step {s}"
-        np.testing.assert_allclose(vmeans, mv, rtol=RTOL, err_msg=f"population mean V at step {s}")
-    print(f"plain block: {steps} steps, {n_thr} threshold and {n_spk} delivered spikes compared; first "
-          f"threshold divergence {'none' if first_div is None else f'at step {first_div}'}; "
-          f"{len(ties)} ties re-aligned {ties[:10]}; worst V {worst_v:.2e}, worst J {worst_j:.2e} relative")
+        else:
+            dv = float((np.abs(V_g - V_o) / np.abs(V_o)).max())
+            dj = float((np.abs(J_g - J_o) - RTOL * np.abs(J_o)).max())
+            worst_v = max(worst_v, dv)
+            worst_j = max(worst_j, float((np.abs(J_g - J_o) / (np.abs(J_o) + 1e-12)).max()))
+            assert dv <= RTOL, f"V differs by {dv:.3g} relative at step {s}"
+            assert dj <= 1e-12, f"J differs beyond 1e-5 relative at step {s}"
+            assert np.array_equal(freqs.astype(np.int64), sa.astype(np.int64)), f"population counts at step {s}"
+            np.testing.assert_allclose(vmeans, mv, rtol=RTOL, err_msg=f"population mean V at step {s}")
+        ob.set("V", V_g)
+        ob.set("J", J_g)
+        ob.set("t_last", nb.get("t_last"))
+    print(f"plain block: {steps} steps, {n_thr} threshold and {n_spk} delivered spikes compared; free-running "
+          f"first threshold divergence {'none' if first_div is None else f'at step {first_div}'} (worst V "
+          f"{free_worst_v:.2e} relative before it); per step: {len(ties)} ties {ties[:10]}, worst V {worst_v:.2e}, "
+          f"worst J {worst_j:.2e} relative")
     assert n_thr > 0, "the plain block never fired: the comparison is vacuous"
+    assert first_div is None or first_div > 50, f"free-running rasters diverge at step {first_div}"
     assert len(ties) <= steps // 100, f"{len(ties)} threshold ties in {steps} steps"
     nb.close()
 
