@@ -126,6 +126,36 @@ __global__ void enkf_update_kernel(int32_t E, int32_t V, int32_t S, double rho, 
     w[x] += rho * innov[(int64_t)e * V + v] * gsum[vs] + (1.0 - rho) * acc;
 }
 
+// The six analysis kernels on a caller-chosen scratch of enkf_scratch_doubles
+// doubles: mean [V*S], wd [E*V*S], temp [E*V], innov [E*V], gsum [V*S],
+// A [E*E].  Every scratch element is written before it is read.
+static size_t enkf_scratch_doubles(int32_t E, int32_t V, int32_t S)
+{
+    const size_t EVS = (size_t)E * V * S, VS = (size_t)V * S, EV = (size_t)E * V, EE = (size_t)E * E;
+    return VS + EVS + 2 * EV + VS + EE;
+}
+
+static int enkf_launch(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y, double rho,
+                       const double* xi, hipStream_t st, double* scratch)
+{
+    const size_t EVS = (size_t)E * V * S, VS = (size_t)V * S, EV = (size_t)E * V, EE = (size_t)E * E;
+    double* mean = scratch;
+    double* wd = mean + VS;
+    double* temp = wd + EVS;
+    double* innov = temp + EV;
+    double* gsum = innov + EV;
+    double* A = gsum + VS;
+    auto g = [](size_t n) { return dim3((unsigned)std::max<size_t>(1, (n + 255) / 256)); };
+    hipLaunchKernelGGL(enkf_mean_kernel, g(VS), dim3(256), 0, st, E, V, S, w, mean);
+    hipLaunchKernelGGL(enkf_diff_kernel, g(EVS), dim3(256), 0, st, E, V, S, w, mean, wd);
+    hipLaunchKernelGGL(enkf_gain_kernel, g((size_t)V), dim3(256), 0, st, E, V, S, w, wd, sigma, y, xi, temp, innov);
+    hipLaunchKernelGGL(enkf_local_kernel, g(VS), dim3(256), 0, st, E, V, S, wd, temp, gsum);
+    hipLaunchKernelGGL(enkf_global_kernel, g(EE), dim3(256), 0, st, E, V, innov, temp, A);
+    hipLaunchKernelGGL(enkf_update_kernel, g(EVS), dim3(256), 0, st, E, V, S, rho, wd, innov, gsum, A, w);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
+}
+
 // Scratch for the analysis, kept per device and grown on demand.
 struct EnkfScratch {
     double* p = nullptr;
@@ -142,8 +172,7 @@ static int enkf(int32_t E, int32_t V, int32_t S, double* w, double sigma, const 
     int dev = 0;
     DTB_TRY_HIP(hipGetDevice(&dev));
     DTB_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
-    const size_t EVS = (size_t)E * V * S, VS = (size_t)V * S, EV = (size_t)E * V, EE = (size_t)E * E;
-    const size_t need = VS /*mean*/ + EVS /*wd*/ + 2 * EV /*temp, innov*/ + VS /*gsum*/ + EE /*A*/;
+    const size_t need = enkf_scratch_doubles(E, V, S);
     std::lock_guard<std::mutex> lk(g_enkf_mu);
     EnkfScratch& sc = g_enkf[dev];
     if (sc.cap < need) {
@@ -156,24 +185,53 @@ static int enkf(int32_t E, int32_t V, int32_t S, double* w, double sigma, const 
         DTB_TRY_HIP(hipMalloc(&sc.p, need * sizeof(double)));
         sc.cap = need;
     }
-    double* mean = sc.p;
-    double* wd = mean + VS;
-    double* temp = wd + EVS;
-    double* innov = temp + EV;
-    double* gsum = innov + EV;
-    double* A = gsum + VS;
     hipStream_t st = (hipStream_t)stream;
-    auto g = [](size_t n) { return dim3((unsigned)std::max<size_t>(1, (n + 255) / 256)); };
-    hipLaunchKernelGGL(enkf_mean_kernel, g(VS), dim3(256), 0, st, E, V, S, w, mean);
-    hipLaunchKernelGGL(enkf_diff_kernel, g(EVS), dim3(256), 0, st, E, V, S, w, mean, wd);
-    hipLaunchKernelGGL(enkf_gain_kernel, g((size_t)V), dim3(256), 0, st, E, V, S, w, wd, sigma, y, xi, temp, innov);
-    hipLaunchKernelGGL(enkf_local_kernel, g(VS), dim3(256), 0, st, E, V, S, wd, temp, gsum);
-    hipLaunchKernelGGL(enkf_global_kernel, g(EE), dim3(256), 0, st, E, V, innov, temp, A);
-    hipLaunchKernelGGL(enkf_update_kernel, g(EVS), dim3(256), 0, st, E, V, S, rho, wd, innov, gsum, A, w);
-    DTB_TRY_HIP(hipGetLastError());
+    DTB_TRY(enkf_launch(E, V, S, w, sigma, y, rho, xi, st, sc.p));
     // the scratch is shared by later calls on this device
     DTB_TRY_HIP(hipStreamSynchronize(st));
     return DTB_OK;
+}
+
+// Allocation-free analysis: the scratch is a fixed device array, one copy per
+// GPU, held until the process ends.  Its host-side name is not a device
+// address, so each device's address comes from hipGetSymbolAddress, taken
+// once per device with that device current, and only that address is handed
+// to the kernels.
+constexpr size_t kEnkfScratchMax = (size_t)1 << 19; // doubles (4 MiB): E = 256, V = 90, S = 10 needs 343,816
+static __device__ __align__(256) double scratch[kEnkfScratchMax];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {nullptr};
+
+static int enkf_noalloc(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y, double rho,
+                        const double* xi, void* stream)
+{
+    DTB_REQUIRE(E >= 2 && V >= 1 && S >= 1, "EnKF needs E >= 2, V >= 1, S >= 1");
+    DTB_REQUIRE(w && y && xi, "w, y and xi are required");
+    DTB_REQUIRE((size_t)E <= kEnkfScratchMax && (size_t)V <= kEnkfScratchMax && (size_t)S <= kEnkfScratchMax,
+                "allocation-free EnKF: E=%d, V=%d, S=%d exceeds the fixed scratch of %zu doubles "
+                "(use dtb_enkf_analysis_f64)", E, V, S, kEnkfScratchMax);
+    const size_t need = enkf_scratch_doubles(E, V, S);
+    DTB_REQUIRE(need <= kEnkfScratchMax,
+                "allocation-free EnKF: E=%d, V=%d, S=%d needs %zu scratch doubles, the fixed scratch holds %zu "
+                "(use dtb_enkf_analysis_f64)", E, V, S, need, kEnkfScratchMax);
+    int dev = 0;
+    DTB_TRY_HIP(hipGetDevice(&dev));
+    DTB_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+    double* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* a = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !a) {
+                set_error("hipGetSymbolAddress of the EnKF scratch failed on device %d: %s", dev, hipGetErrorString(e));
+                return DTB_RUNTIME_ERROR;
+            }
+            g_scratch_ptr[dev] = static_cast<double*>(a);
+        }
+        p = g_scratch_ptr[dev];
+    }
+    return enkf_launch(E, V, S, w, sigma, y, rho, xi, (hipStream_t)stream, p);
 }
 
 } // namespace dtb
@@ -196,6 +254,12 @@ int dtb_enkf_analysis_f64(int32_t E, int32_t V, int32_t S, double* w, double sig
                           const double* xi, void* stream)
 {
     return dtb::enkf(E, V, S, w, sigma, y, rho, xi, stream);
+}
+
+int dtb_enkf_analysis_f64_noalloc(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y,
+                                  double rho, const double* xi, void* stream)
+{
+    return dtb::enkf_noalloc(E, V, S, w, sigma, y, rho, xi, stream);
 }
 
 } // extern "C"

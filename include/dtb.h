@@ -320,6 +320,17 @@ int dtb_bold_step_f64(int64_t n, const double* par, double* s, double* q, double
 int dtb_enkf_analysis_f64(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y, double rho,
                           const double* xi, void* stream);
 
+/* The same analysis without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The scratch is a fixed per-GPU device array
+ * of 2^19 doubles (4 MiB), held until the process ends; sizes whose scratch
+ * (V*S + E*V*S + 2*E*V + V*S + E*E doubles) exceeds it return
+ * DTB_INVALID_PARAMETER.  Asynchronous on `stream`; every call on one GPU
+ * shares that GPU's scratch, so calls that overlap on one GPU (other streams,
+ * other threads) race. */
+int dtb_enkf_analysis_f64_noalloc(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y,
+                                  double rho, const double* xi, void* stream);
+
 /* Learned surrogate kernels (no counterpart in the reference: the surrogate
  * is new in this library).  bf16 operands are raw uint16 bit patterns on the
  * device.  All calls are asynchronous on `stream` (hipStream_t or NULL). */
