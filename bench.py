@@ -18,7 +18,8 @@ block's own stream; they give the per-kernel times of `roofline` and
 `kernels` and are not part of `value`.
 
 Beside the main line's fields: `plain` (the 10,000-neuron plain block,
-in-degree 100, 8,000 steps after 100 warm-up steps -- the workload the CPU
+in-degree 100, 8,000 steps after 100 warm-up steps and delivery tuning,
+which picks the persistent launch for this block -- the workload the CPU
 comparison is quoted on), `cpu_baseline` (the oracle's FP32 restatement of
 the reference algorithm -- a full pass over every synapse per step, as
 torch.sparse.mm in models/block.py:76-81 -- with OpenMP on the host cores,
@@ -185,12 +186,19 @@ def plain_line(steps, warmup, log):
     in-degree 100, weights U[0,1], delay 1, noise 10 Hz, dt = 0.1 ms."""
     from digital_twin_brain_amd import networks
 
+    from digital_twin_brain_amd.autotune import tune_delivery
+
     nb, _ = networks.build_plain()
     nb.run(warmup, noise_mode=1, graph=True)
+    s0 = nb.steps
+    best, table = tune_delivery(nb, dict(noise_mode=1), steps=20, use_cache=False)
+    tuning = nb.steps - s0
+    log(f"plain block: delivery {best}, persistent {table.get('persistent step us')} us/step against "
+        f"{table.get('best shape step us')} us/step")
     cap = capture(nb, steps, noise_mode=1)
     wall, ev, sp = run_counted(nb, steps, noise_mode=1)
     out = {"metric": "synaptic events/s", "value": float(ev.sum()) / wall, "unit": "events/s", "steps": steps,
-           "warmup": warmup, "capture_steps": cap, "ms_per_step": wall / steps * 1e3,
+           "warmup": warmup, "tuning_steps": tuning, "capture_steps": cap, "ms_per_step": wall / steps * 1e3,
            "events_per_step": float(ev.mean()), "spikes_per_step": float(sp.mean()),
            "config": {"workload": "plain block", "neurons": int(nb.n), "in_degree": 100, "synapses": int(nb.nnz),
                       "g_ui": "U[0.001, 0.05]", "weights": "U[0, 1] compact fp32", "delay_steps": 1, "dt_ms": 0.1,

@@ -182,9 +182,116 @@ __device__ inline uint4 ld_stream(const uint4* p)
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
+// One neuron's step (models/block.py:89-114,136-138): J += this step's
+// arrivals, I_syn from the stored V, the Euler step or the refractory hold,
+// threshold, noise spike, J decayed for the next step.  GACC: the arrivals
+// are loaded from a.acc (and cleared there for direct delivery); else they
+// are `din`.  Every load of the neuron precedes its first store: the compiler
+// makes a load that follows a store to another array wait for that store.
 // PSET: the constants shared by groups of neurons (reversal potentials,
 // synaptic decays, leak, threshold, reset, I_ext) come from a small table in
 // LDS indexed by a 1-byte set id instead of 56 B of per-neuron arrays.
+template <bool PAIR, bool PSET, bool GACC>
+__device__ __forceinline__ void neuron_update(const NeuronArgs& a, int64_t i, int64_t step, double t, int32_t k,
+                                              float4 din, const float* s_tab, bool& act, bool& spike, float& vout)
+{
+    // the neuron kernel streams the state (read once per step from arrays far
+    // larger than L2); the persistent launch re-reads its small block's state
+    // every step, so it keeps it cached
+    const float V = GACC ? ld_stream(a.V + i) : a.V[i];
+    float4 J = GACC ? ld_stream(a.Jp + i) : a.Jp[i];
+    float4 d = din; // this step's arrivals per receptor (compact: AMPA, NMDA share one sum, GABAs the other)
+    if (GACC) {
+        if (PAIR) {
+            d = ld_stream(reinterpret_cast<const float4*>(a.acc) + i);
+        } else {
+            const float2 d2 = ld_stream(reinterpret_cast<const float2*>(a.acc) + i);
+            d = make_float4(d2.x, d2.x, d2.y, d2.y);
+        }
+    }
+    const float4 g = GACC ? ld_stream(a.g + i) : a.g[i];
+    const int32_t rend = a.ref_end[i];
+    const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
+    const float un = a.noise_mode == DTB_NOISE_UNIFORMS ? a.uniforms[(int64_t)k * a.n + i] : 0.0f;
+    const char2 ee = a.ext.n ? *reinterpret_cast<const char2*>(a.jexp + 4 * i) : make_char2(0, 0);
+    const uint8_t ss = a.src ? a.src[i] : (uint8_t)0;
+    J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
+    float4 vu, dc;
+    float gL, vL, dtC, vth, vrst, iext;
+    if (PSET) {
+        const float* T = s_tab + 16 * a.pset[i];
+        vu = make_float4(T[0], T[1], T[2], T[3]);
+        dc = make_float4(T[4], T[5], T[6], T[7]);
+        gL = T[8]; vL = T[9]; dtC = T[10]; vth = T[11]; vrst = T[12]; iext = T[13];
+    } else {
+        vu = a.vu[i];
+        dc = a.decay[i];
+        gL = a.gL[i]; vL = a.vL[i]; dtC = a.dtC[i]; vth = a.vth[i]; vrst = a.vrst[i]; iext = a.iext[i];
+    }
+    float isyn = g.x * (vu.x - V) * J.x;
+    isyn += g.y * (vu.y - V) * J.y;
+    isyn += g.z * (vu.z - V) * J.z;
+    isyn += g.w * (vu.w - V) * J.w;
+    const float main_part = -gL * (V - vL);
+    const float vn = V + dtC * (main_part + isyn + iext);
+    const bool not_ref = step >= (int64_t)rend;
+    const float vi = not_ref ? vn : vrst;
+    act = vi >= vth;
+    vout = vi > vth ? vth : vi;
+    if (ss & 1) {
+        // a source's threshold spike comes from the schedule (models/block.py:104-107);
+        // a frozen one (an input block's neuron) also keeps its V
+        act = (ss & 4) != 0;
+        if (ss & 2) vout = V;
+    }
+    if (GACC && a.clear_acc) {
+        if (PAIR) reinterpret_cast<float4*>(a.acc)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        else reinterpret_cast<float2*>(a.acc)[i] = make_float2(0.f, 0.f);
+    }
+    a.V[i] = vout;
+    if (ss & 4) a.src[i] = ss & 3;
+    if (act) {
+        a.ref_end[i] = refractory_end(step, t, a.tref[i], a.dt);
+        a.last_step[i] = (int32_t)step;
+        a.t_last[i] = t;
+    }
+    const float u = a.noise_mode == DTB_NOISE_UNIFORMS ? un : noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
+    spike = ((u < pr) && !(ss & 2)) || (act && !a.isolated);
+    float4 jn = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
+    if (a.ext.n) {
+        // external arrivals of this step: part of J at the next step, like this step's deliveries
+        const float x = (float)ext_count(a.ext, a.seed, (uint64_t)step, (uint64_t)i) * a.ext.w;
+        jn.x += ldexpf(x, ee.x);
+        jn.y += ldexpf(x, ee.y);
+    }
+    a.Jp[i] = jn;
+    a.flags[i] = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
+}
+
+// Per sub-population statistics of a step (threshold spikes, V sum) for the
+// 64 neurons of a wavefront: one atomic per wavefront when they share a
+// population.
+__device__ __forceinline__ void step_stats(const NeuronArgs& a, int32_t k, int32_t p, bool act, float vout)
+{
+    const uint32_t lane = __lane_id();
+    const int32_t p0 = __shfl(p, 0);
+    if (__all(p == p0)) {
+        if (p0 >= 0) {
+            if (a.rec_freq) {
+                const uint32_t c = (uint32_t)__popcll(__ballot(act));
+                if (lane == 0 && c) atomicAdd(&a.rec_freq[(int64_t)k * a.S + p0], c);
+            }
+            if (a.rec_vsum) {
+                const float v = wave_sum(vout);
+                if (lane == 0) unsafeAtomicAdd(&a.rec_vsum[(int64_t)k * a.S + p0], v);
+            }
+        }
+    } else if (p >= 0) {
+        if (a.rec_freq && act) atomicAdd(&a.rec_freq[(int64_t)k * a.S + p], 1u);
+        if (a.rec_vsum) unsafeAtomicAdd(&a.rec_vsum[(int64_t)k * a.S + p], vout);
+    }
+}
+
 template <bool PAIR, bool PSET>
 __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t k)
 {
@@ -207,76 +314,8 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     float vout = 0.0f;
     int32_t p = -1;
     if (i < a.n) {
-        // every load of the neuron before its first store: the compiler makes a
-        // load that follows a store to another array wait for that store
-        const float V = ld_stream(a.V + i);
-        float4 J = ld_stream(a.Jp + i);
-        float4 d; // this step's arrivals per receptor (compact: AMPA, NMDA share one sum, GABAs the other)
-        if (PAIR) {
-            d = ld_stream(reinterpret_cast<const float4*>(a.acc) + i);
-        } else {
-            const float2 d2 = ld_stream(reinterpret_cast<const float2*>(a.acc) + i);
-            d = make_float4(d2.x, d2.x, d2.y, d2.y);
-        }
-        const float4 g = ld_stream(a.g + i);
-        const int32_t rend = a.ref_end[i];
-        const int32_t pp = a.pop[i];
-        const float pr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[i] : a.noise_rate;
-        const float un = a.noise_mode == DTB_NOISE_UNIFORMS ? a.uniforms[(int64_t)k * a.n + i] : 0.0f;
-        const char2 ee = a.ext.n ? *reinterpret_cast<const char2*>(a.jexp + 4 * i) : make_char2(0, 0);
-        const uint8_t ss = a.src ? a.src[i] : (uint8_t)0;
-        J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
-        float4 vu, dc;
-        float gL, vL, dtC, vth, vrst, iext;
-        if (PSET) {
-            const float* T = s_tab + 16 * a.pset[i];
-            vu = make_float4(T[0], T[1], T[2], T[3]);
-            dc = make_float4(T[4], T[5], T[6], T[7]);
-            gL = T[8]; vL = T[9]; dtC = T[10]; vth = T[11]; vrst = T[12]; iext = T[13];
-        } else {
-            vu = a.vu[i];
-            dc = a.decay[i];
-            gL = a.gL[i]; vL = a.vL[i]; dtC = a.dtC[i]; vth = a.vth[i]; vrst = a.vrst[i]; iext = a.iext[i];
-        }
-        float isyn = g.x * (vu.x - V) * J.x;
-        isyn += g.y * (vu.y - V) * J.y;
-        isyn += g.z * (vu.z - V) * J.z;
-        isyn += g.w * (vu.w - V) * J.w;
-        const float main_part = -gL * (V - vL);
-        const float vn = V + dtC * (main_part + isyn + iext);
-        const bool not_ref = step >= (int64_t)rend;
-        const float vi = not_ref ? vn : vrst;
-        act = vi >= vth;
-        vout = vi > vth ? vth : vi;
-        if (ss & 1) {
-            // a source's threshold spike comes from the schedule (models/block.py:104-107);
-            // a frozen one (an input block's neuron) also keeps its V
-            act = (ss & 4) != 0;
-            if (ss & 2) vout = V;
-        }
-        if (a.clear_acc) {
-            if (PAIR) reinterpret_cast<float4*>(a.acc)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            else reinterpret_cast<float2*>(a.acc)[i] = make_float2(0.f, 0.f);
-        }
-        a.V[i] = vout;
-        if (ss & 4) a.src[i] = ss & 3;
-        if (act) {
-            a.ref_end[i] = refractory_end(step, t, a.tref[i], a.dt);
-            a.last_step[i] = (int32_t)step;
-            a.t_last[i] = t;
-        }
-        const float u = a.noise_mode == DTB_NOISE_UNIFORMS ? un : noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
-        spike = ((u < pr) && !(ss & 2)) || (act && !a.isolated);
-        float4 jn = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
-        if (a.ext.n) {
-            // external arrivals of this step: part of J at the next step, like this step's deliveries
-            const float x = (float)ext_count(a.ext, a.seed, (uint64_t)step, (uint64_t)i) * a.ext.w;
-            jn.x += ldexpf(x, ee.x);
-            jn.y += ldexpf(x, ee.y);
-        }
-        a.Jp[i] = jn;
-        a.flags[i] = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
-        p = pp;
+        p = a.pop[i];
+        neuron_update<PAIR, PSET, true>(a, i, step, t, k, make_float4(0.f, 0.f, 0.f, 0.f), s_tab, act, spike, vout);
     }
 
     // spike compaction: ballot + per-wave prefix, one global atomic per workgroup
@@ -299,24 +338,7 @@ __global__ __launch_bounds__(256) void neuron_step_kernel(NeuronArgs a, int32_t 
     }
 
     // per sub-population statistics (threshold spikes, V sum)
-    if (a.rec_freq || a.rec_vsum) {
-        const int32_t p0 = __shfl(p, 0);
-        if (__all(p == p0)) {
-            if (p0 >= 0) {
-                if (a.rec_freq) {
-                    const uint32_t c = (uint32_t)__popcll(__ballot(act));
-                    if (lane == 0 && c) atomicAdd(&a.rec_freq[(int64_t)k * a.S + p0], c);
-                }
-                if (a.rec_vsum) {
-                    const float v = wave_sum(vout);
-                    if (lane == 0) unsafeAtomicAdd(&a.rec_vsum[(int64_t)k * a.S + p0], v);
-                }
-            }
-        } else if (p >= 0) {
-            if (a.rec_freq && act) atomicAdd(&a.rec_freq[(int64_t)k * a.S + p], 1u);
-            if (a.rec_vsum) unsafeAtomicAdd(&a.rec_vsum[(int64_t)k * a.S + p], vout);
-        }
-    }
+    if (a.rec_freq || a.rec_vsum) step_stats(a, k, p, act, vout);
 }
 
 struct DeliverArgs {
@@ -340,6 +362,25 @@ struct DeliverArgs {
 // Inclusive scan over the 256 lanes of a workgroup (shuffles within each
 // wavefront, then the four wavefront totals through LDS).
 __device__ inline uint32_t block_inclusive_scan_256(uint32_t v, uint32_t* wtot)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += y;
+    }
+    if (lane == kWave - 1) wtot[wid] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wid; ++w) add += wtot[w];
+    __syncthreads();
+    return v + add;
+}
+
+// Inclusive scan over the W wavefronts of a workgroup (W * 64 lanes).
+template <int W>
+__device__ inline uint32_t block_inclusive_scan_n(uint32_t v, uint32_t* wtot)
 {
     const uint32_t lane = __lane_id();
     const uint32_t wid = threadIdx.x >> 6;
@@ -1189,6 +1230,8 @@ static int update_jexp(Block* b, bool with_state)
     if (st) return st;
     DTB_TRY_HIP(e);
     if (!b->fixed_ok) b->accum_fixed = false;
+    b->persist_dirty = true; // the grouped copy holds the rounded scaled weights
+    if (!b->fixed_ok && b->deliver_mode == 2) b->deliver_mode = 1;
     return DTB_OK;
 }
 
@@ -1324,6 +1367,280 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     if (t == 0) s.ovf[(size_t)b * kCursorStride] = 0u;
 }
 
+// ---------------------------------------------------------------------------
+// Persistent delivery (deliver_mode 2) for blocks small enough that every
+// workgroup of one launch is resident: one launch runs all K steps of a
+// batch, so a step costs one grid barrier instead of two or three kernel
+// launches.  Workgroup p (p < G <= kSubLists) owns the neuron-kernel blocks
+// p, p + 32, p + 64, ... (256 neurons each): it updates them, writes their
+// spikes into spike sub-list p (its only writer, so in the standard list
+// format without reservation atomics), and after the barrier accumulates
+// every arrival for its own neurons into an int32 LDS slice.  The synapses
+// are a copy of the CSR grouped by owning workgroup inside every (row, delay)
+// segment, with the weights pre-rounded to the same int32 fixed point as
+// bucketed delivery, so the results equal bucketed delivery bit for bit.
+constexpr int kPersistMaxLds = 64 * 1024; // LDS slice bytes per workgroup
+constexpr uint32_t kBarrierSpinLimit = 1u << 22;
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima), gfx9 encoding
+constexpr unsigned kWaitVmcnt0 = 0x0F70;
+constexpr int kPubStride = 16;                 // words between publication words (one 64-byte line each)
+constexpr int kPubAbort = kSubLists * kPubStride; // word index of the abort flag
+constexpr int kPubWords = kPubAbort + 64;
+
+struct PersistArgs {
+    NeuronArgs na;
+    const int32_t* __restrict__ bptr;  // [n*D][G+1]: start of each workgroup's synapses in (row, delay) segments
+    const uint32_t* __restrict__ pcol; // grouped synapses: local index | kind << 31
+    const int32_t* __restrict__ pval;  // [nnz] (compact) or [nnz][2] (pair): round(scaled weight)
+    int32_t G;                         // workgroups
+    int32_t m;                         // blocks per workgroup (the LDS slice holds m*256 neurons)
+    int64_t nblk;                      // neuron-kernel blocks: ceil(n / 256)
+    int32_t K;                         // steps of the batch
+    uint32_t* pub;                     // [kSubLists] 64-bit words (step << 32 | spikes), kPubStride apart; abort flag
+    uint64_t* __restrict__ rec_events;
+    uint32_t* __restrict__ rec_nspk;
+};
+
+// Spike-list entries written by other workgroups of the same launch are read
+// with agent-coherent loads (they bypass a stale copy in this XCD's L2);
+// everything else the delivery reads is read-only during the launch or the
+// workgroup's own, and stays cacheable.
+__device__ inline int32_t ld_coherent(const int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// MB blocks of 256 neurons per pass (workgroups of 256*MB threads).
+template <bool PAIR, bool PSET, int MB>
+__global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
+{
+    constexpr int C = PAIR ? 4 : 2;
+    constexpr int NT = 256 * MB;
+    extern __shared__ int s_acc[]; // [m*256][C] arrivals of the workgroup's neurons, int32 fixed point
+    __shared__ uint32_t s_wcnt[4 * MB];
+    __shared__ float s_tab[kMaxPsets * 16];
+    __shared__ uint32_t cum[kMaxEntries + 1];
+    __shared__ uint32_t wtot[4 * MB];
+    __shared__ uint32_t s_cnt[(DTB_MAX_DELAY + 1) * kSubLists]; // spike counts of every sub-list, per slot
+    __shared__ int32_t s_lo[NT];       // delivery: start of each segment of the current chunk
+    __shared__ uint32_t s_end[NT + 1]; // delivery: inclusive prefix of the chunk's segment lengths
+    __shared__ int s_ok;
+    const NeuronArgs& a = P.na;
+    const int32_t p = blockIdx.x;
+    const int32_t G = P.G;
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = threadIdx.x >> 6;
+    if (PSET)
+        for (int x = threadIdx.x; x < a.npsets * 16; x += NT) s_tab[x] = a.ptab[x];
+    for (int x = threadIdx.x; x < P.m * 256 * C; x += NT) s_acc[x] = 0;
+    // counts of the earlier steps' lists (written before this launch)
+    for (int x = threadIdx.x; x < a.L * kSubLists; x += NT) s_cnt[x] = a.spk_cnt[(int64_t)x * kCntStride];
+    if (threadIdx.x == 0) s_ok = 1;
+    __syncthreads();
+    const int32_t E = a.D * kSubLists;
+    for (int32_t k = 0; k < P.K; ++k) {
+        const int64_t step = a.ctl->nbase + k + 1;
+        const double t = a.ttab[k];
+        const int32_t slot = (int32_t)(step % a.L);
+        const int64_t sl = (int64_t)slot * kSubLists + p;
+        uint32_t run = 0; // spikes written to sub-list p this step (uniform)
+        for (int32_t jb = 0; jb < P.m; jb += MB) {
+            const int32_t sb = jb + (int32_t)(threadIdx.x >> 8);
+            const int64_t blk = p + (int64_t)kSubLists * sb;
+            const int64_t i = blk * 256 + (threadIdx.x & 255);
+            const int li = sb * 256 + (threadIdx.x & 255);
+            bool act = false, spike = false;
+            float vout = 0.0f;
+            int32_t pp = -1;
+            if (sb < P.m && blk < P.nblk && i < a.n) {
+                pp = a.pop[i];
+                float4 d;
+                if (k == 0) {
+                    // the pending arrivals of the batch's first step, as the neuron kernel reads them
+                    if (PAIR) {
+                        d = reinterpret_cast<const float4*>(a.acc)[i];
+                    } else {
+                        const float2 d2 = reinterpret_cast<const float2*>(a.acc)[i];
+                        d = make_float4(d2.x, d2.x, d2.y, d2.y);
+                    }
+                } else if (PAIR) {
+                    d = make_float4((float)s_acc[4 * li], (float)s_acc[4 * li + 1], (float)s_acc[4 * li + 2],
+                                    (float)s_acc[4 * li + 3]);
+                } else {
+                    const float e = (float)s_acc[2 * li], f = (float)s_acc[2 * li + 1];
+                    d = make_float4(e, e, f, f);
+                }
+#pragma unroll
+                for (int c = 0; c < C; ++c) s_acc[C * li + c] = 0;
+                neuron_update<PAIR, PSET, false>(a, i, step, t, k, d, s_tab, act, spike, vout);
+            }
+            // compaction into sub-list p: the blocks in order, each after the earlier ones
+            const uint64_t msk = __ballot(spike);
+            const uint32_t pre = (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+            if (lane == 0) s_wcnt[wid] = (uint32_t)__popcll(msk);
+            __syncthreads();
+            uint32_t off = run, tot = 0;
+            for (uint32_t w = 0; w < 4 * MB; ++w) {
+                if (w < wid) off += s_wcnt[w];
+                tot += s_wcnt[w];
+            }
+            if (spike) a.spk[sl * a.spk_cap + off + pre] = (int32_t)i;
+            run += tot;
+            if (a.rec_freq || a.rec_vsum) step_stats(a, k, pp, act, vout);
+            __syncthreads();
+        }
+        // publish: every wave waits until its stores (list entries, state)
+        // have reached L2, then one agent-scope release writes L2 back before
+        // the (step, count) word that other workgroups poll
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            a.spk_cnt[sl * kCntStride] = run; // the standard counter, for launches after this one
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(reinterpret_cast<uint64_t*>(P.pub + kPubStride * p),
+                               ((uint64_t)step << 32) | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // wait for every workgroup's word of this step (wave 0, one lane per
+        // workgroup), then wave 0 builds the (delay, sub-list) entry prefix:
+        // cum[e + 1] = spikes in entries <= e, e = (d - 1) * kSubLists + q
+        if (wid == 0) {
+            // a lane stops loading once its word shows this step, so the
+            // waiting workgroups do not keep the few polled lines busy
+            uint32_t spins = 0;
+            bool ok = true;
+            bool mine = (int32_t)lane >= G;
+            uint64_t v = 0;
+            for (;;) {
+                if (!mine) {
+                    v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(P.pub + kPubStride * lane),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    mine = (v >> 32) == (uint64_t)step;
+                }
+                if (__all(mine)) break;
+                const bool ab = lane == 0 &&
+                                __hip_atomic_load(P.pub + kPubAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                if (++spins > kBarrierSpinLimit || __any(ab)) {
+                    if (lane == 0) __hip_atomic_store(P.pub + kPubAbort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (ok && (int32_t)lane < kSubLists) s_cnt[slot * kSubLists + lane] = (int32_t)lane < G ? (uint32_t)v : 0u;
+            if (lane == 0) {
+                s_ok = ok ? 1 : 0;
+                cum[0] = 0;
+            }
+            uint32_t base = 0;
+            for (int32_t e0 = 0; e0 < E; e0 += kWave) {
+                const int32_t e = e0 + (int32_t)lane;
+                uint32_t c = 0;
+                if (e < E)
+                    c = min(s_cnt[delay_slot(slot, e / kSubLists + 1, a.L) * kSubLists + e % kSubLists],
+                            (uint32_t)a.spk_cap);
+#pragma unroll
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const uint32_t y = __shfl_up(c, off);
+                    if (lane >= (uint32_t)off) c += y;
+                }
+                if (e < E) cum[e + 1] = base + c;
+                base += __shfl(c, kWave - 1);
+            }
+        }
+        __syncthreads();
+        if (!s_ok) break;
+
+        // delivery, NT spiking (row, delay) segments at a time: each thread
+        // reads one segment's bounds for this workgroup, a scan spreads the
+        // segments' synapses evenly over the threads, and every thread issues
+        // its synapse loads together before its LDS atomics
+        const uint32_t total = cum[E];
+        uint64_t ev = 0;
+        for (uint32_t c0 = 0; c0 < total; c0 += NT) {
+            const uint32_t r = c0 + threadIdx.x;
+            int32_t lo = 0, hi = 0;
+            if (r < total) {
+                const int32_t e = entry_of(cum, E, r);
+                const int32_t d = e / kSubLists + 1;
+                const int32_t j = ld_coherent(a.spk + ((int64_t)delay_slot(slot, d, a.L) * kSubLists + e % kSubLists) *
+                                                          a.spk_cap + (r - cum[e]));
+                if (j >= 0 && (int64_t)j < a.n) {
+                    const int32_t* bp = P.bptr + ((int64_t)j * a.D + (d - 1)) * (G + 1);
+                    lo = bp[p];
+                    hi = bp[p + 1];
+                }
+            }
+            const uint32_t len = (uint32_t)(hi - lo);
+            ev += len; // every workgroup counts the events it receives
+            const uint32_t incl = block_inclusive_scan_n<4 * MB>(len, wtot);
+            s_lo[threadIdx.x] = lo;
+            s_end[threadIdx.x + 1] = incl;
+            if (threadIdx.x == 0) s_end[0] = 0;
+            __syncthreads();
+            const uint32_t tot = s_end[NT];
+            for (uint32_t x0 = 0; x0 < tot; x0 += 4 * NT) {
+                constexpr int U = 4;
+                int32_t pos[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t x = x0 + (uint32_t)u * NT + threadIdx.x;
+                    pos[u] = -1;
+                    if (x < tot) {
+                        int32_t l = 0, h = NT - 1; // the segment holding x: last t with s_end[t] <= x
+                        while (l < h) {
+                            const int32_t mid = (l + h + 1) >> 1;
+                            if (s_end[mid] <= x) l = mid;
+                            else h = mid - 1;
+                        }
+                        pos[u] = s_lo[l] + (int32_t)(x - s_end[l]);
+                    }
+                }
+                uint32_t cc[U];
+                int2 vv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (pos[u] < 0) continue;
+                    cc[u] = P.pcol[pos[u]];
+                    vv[u] = PAIR ? reinterpret_cast<const int2*>(P.pval)[pos[u]] : make_int2(P.pval[pos[u]], 0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (pos[u] < 0) continue;
+                    const int32_t o = (int32_t)(cc[u] & 0x7fffffffu) * C + (int32_t)(cc[u] >> 31) * (C / 2);
+                    atomicAdd(&s_acc[o], vv[u].x);
+                    if (PAIR) atomicAdd(&s_acc[o + 1], vv[u].y);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) ev += __shfl_xor(ev, off);
+        if (lane == 0 && ev) atomicAdd((unsigned long long*)&P.rec_events[k], (unsigned long long)ev);
+        if (p == 0 && threadIdx.x == 0) P.rec_nspk[k] = cum[kSubLists];
+        __syncthreads();
+    }
+    // the arrivals pending for the next step go to acc, as bucketed delivery leaves them
+    for (int32_t li = threadIdx.x; li < P.m * 256; li += NT) {
+        const int64_t blk = p + (int64_t)kSubLists * (li >> 8);
+        const int64_t i = blk * 256 + (li & 255);
+        if (blk >= P.nblk || i >= a.n) continue;
+        if (PAIR)
+            reinterpret_cast<float4*>(a.acc)[i] = make_float4((float)s_acc[4 * li], (float)s_acc[4 * li + 1],
+                                                              (float)s_acc[4 * li + 2], (float)s_acc[4 * li + 3]);
+        else
+            reinterpret_cast<float2*>(a.acc)[i] = make_float2((float)s_acc[2 * li], (float)s_acc[2 * li + 1]);
+    }
+}
+
+// After a persistent batch: the spike-list counters of the slot the next step
+// writes start at zero (the neuron kernel appends with atomics), as the
+// neuron kernel of the batch's last step would have left them.
+__global__ void persist_epilogue_kernel(const StepCtl* __restrict__ ctl, int32_t L, uint32_t* spk_cnt)
+{
+    const int64_t next = ctl->nsteps + 1;
+    if (threadIdx.x < kSubLists) spk_cnt[((int64_t)(next % L) * kSubLists + threadIdx.x) * kCntStride] = 0;
+}
+
 // Forced spikes of batch step k: the sources listed for the step's 0-based
 // index (nbase + k) get their fire mark, which the neuron kernel consumes.
 __global__ void src_mark_kernel(const StepCtl* __restrict__ ctl, int32_t k, int64_t nts,
@@ -1346,11 +1663,13 @@ __global__ void src_mark_kernel(const StepCtl* __restrict__ ctl, int32_t k, int6
 // Batch prologue: times of the K steps, step counters, record clearing.
 __global__ void prologue_kernel(StepCtl* ctl, double* ttab, int32_t K, double dt, uint32_t* rec_freq,
                                 float* rec_vsum, float* rec_isum, int64_t nrec, uint64_t* rec_events,
-                                uint32_t* rec_nspk)
+                                uint32_t* rec_nspk, uint32_t* bar)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     if (tid == 0) {
+        if (bar)
+            for (int q = 0; q < kSubLists; ++q) *reinterpret_cast<uint64_t*>(bar + kPubStride * q) = 0ull;
         const int64_t n0 = ctl->nsteps;
         double t = ctl->t;
         for (int32_t k = 0; k < K; ++k) {
@@ -2157,6 +2476,9 @@ static int create_from_csr(Block* b, const dtb_block_desc* d)
 }
 
 // ------------------------------------------------------------- the run loop
+static int ensure_persist(Block* b);
+static bool persist_fits(const Block* b, int32_t* G = nullptr, int32_t* m = nullptr);
+
 static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* uniforms, bool freq, bool vsum)
 {
     NeuronArgs x;
@@ -2193,6 +2515,39 @@ struct Want {
     bool freq, vsum, isum, sample, bold, timing;
 };
 
+template <bool PAIR, bool PSET>
+static void launch_persist_t(const Block* b, const PersistArgs& pa, int mb, size_t lds)
+{
+    if (mb == 4)
+        hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 4>), dim3(b->pG), dim3(1024), lds, b->stream, pa);
+    else if (mb == 2)
+        hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 2>), dim3(b->pG), dim3(512), lds, b->stream, pa);
+    else
+        hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 1>), dim3(b->pG), dim3(256), lds, b->stream, pa);
+}
+
+static void launch_persist(const Block* b, const PersistArgs& pa, int mb, size_t lds)
+{
+    if (b->pair) {
+        if (b->npsets) launch_persist_t<true, true>(b, pa, mb, lds);
+        else launch_persist_t<true, false>(b, pa, mb, lds);
+    } else {
+        if (b->npsets) launch_persist_t<false, true>(b, pa, mb, lds);
+        else launch_persist_t<false, false>(b, pa, mb, lds);
+    }
+}
+
+// A batch runs as one persistent launch when the block is set to deliver_mode
+// 2 and the batch needs nothing that launch does not produce (injected
+// uniforms, sources, the external drive, I_syn means, samples, per-kernel
+// times); otherwise it runs on the bucketed kernels, whose fixed-point sums
+// give the same results.
+static bool persist_batch(const Block* b, const dtb_run_args* a, const Want& w)
+{
+    return b->deliver_mode == 2 && b->pcol && !b->persist_dirty && a->noise_mode != DTB_NOISE_UNIFORMS &&
+           b->src_nts == 0 && !b->src && b->ext.n == 0 && !w.isum && !w.sample && !w.timing;
+}
+
 static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float* uniforms, Want w)
 {
     if (w.timing && (int64_t)b->events.size() < 3ll * K) {
@@ -2206,10 +2561,32 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
     hipLaunchKernelGGL(prologue_kernel, dim3(grid_for(std::max<int64_t>(nrec, K), 256 * 4)), dim3(256), 0,
                        b->stream, b->ctl, b->ttab, K, b->dt, w.freq ? b->rec_freq : nullptr,
                        w.vsum ? b->rec_vsum : nullptr, w.isum ? b->rec_isum : nullptr, nrec, b->rec_events,
-                       b->rec_nspk);
+                       b->rec_nspk, b->pbar);
     const NeuronArgs na = neuron_args(b, a, uniforms, w.freq, w.vsum);
     const DeliverArgs da = deliver_args(b);
     const unsigned ng = grid_for(b->n, 256);
+    if (persist_batch(b, a, w)) {
+        PersistArgs pa;
+        pa.na = na;
+        pa.bptr = b->pbptr; pa.pcol = b->pcol; pa.pval = b->pval;
+        pa.G = b->pG; pa.m = b->pm; pa.nblk = (b->n + 255) / 256; pa.K = K;
+        pa.pub = b->pbar;
+        pa.rec_events = b->rec_events; pa.rec_nspk = b->rec_nspk;
+        const size_t lds = sizeof(int) * (size_t)b->pm * 256 * (b->pair ? 4 : 2);
+        const int mb = b->pm >= 4 ? 4 : b->pm >= 2 ? 2 : 1;
+        launch_persist(b, pa, mb, lds);
+        hipLaunchKernelGGL(persist_epilogue_kernel, dim3(1), dim3(64), 0, b->stream, b->ctl, b->D + 1, b->spk_cnt);
+        if (w.bold)
+            for (int32_t k = b->bold_sps - 1; k < K; k += b->bold_sps)
+                hipLaunchKernelGGL(bold_counts_kernel, dim3(grid_for(b->nvox, 256)), dim3(256), 0, b->stream, b->S,
+                                   b->nvox, k, b->bold_sps, b->rec_freq, b->vox_ptr, b->vox_pops, b->vox_size,
+                                   b->bold_state + 6 * (int64_t)b->nvox, b->bold_state, b->rec_bold);
+        if (w.vsum)
+            hipLaunchKernelGGL(scale_kernel, dim3(grid_for(nrec, 256)), dim3(256), 0, b->stream, b->rec_vsum,
+                               b->pop_inv, (int64_t)K, b->S);
+        DTB_TRY_HIP(hipGetLastError());
+        return DTB_OK;
+    }
     for (int32_t k = 0; k < K; ++k) {
         if (b->src_nts > 0)
             hipLaunchKernelGGL(src_mark_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (b->src_max + 255) / 256))),
@@ -2228,7 +2605,7 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
                 hipLaunchKernelGGL((neuron_step_kernel<false, false>), dim3(ng), dim3(256), 0, b->stream, na, k);
         }
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
-        if (b->deliver_mode == 1) {
+        if (b->deliver_mode >= 1) {
             StageArgs sa{da, b->nbk, b->bsize, b->bmagic, b->bshift, b->stage_cap, b->cursor, b->stage, b->acc2,
                          b->ovf};
             if (b->pair) {
@@ -2303,6 +2680,11 @@ static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
         return DTB_UNSET_SAMPLE_FAILURE;
     }
     DTB_TRY_HIP(hipSetDevice(b->device));
+    if (b->deliver_mode == 2 && b->persist_dirty) {
+        // weights or layout changed since the grouped copy was built
+        if (persist_fits(b)) DTB_TRY(ensure_persist(b));
+        else b->deliver_mode = 1;
+    }
     float* ustage = nullptr;
     const bool uhost = a->noise_mode == DTB_NOISE_UNIFORMS && !is_device_ptr(a->uniforms);
     if (uhost) DTB_TRY_HIP(hipMalloc(&ustage, sizeof(float) * (size_t)b->n * (size_t)std::min<int64_t>(nsteps, b->kmax)));
@@ -2410,6 +2792,16 @@ static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
     if (e != hipSuccess) {
         set_error("step batch failed: %s", hipGetErrorString(e));
         return DTB_RUNTIME_ERROR;
+    }
+    if (b->pbar) {
+        uint32_t ab = 0;
+        DTB_TRY_HIP(hipMemcpy(&ab, b->pbar + kPubAbort, 4, hipMemcpyDeviceToHost));
+        if (ab) {
+            DTB_TRY_HIP(hipMemset(b->pbar + kPubAbort, 0, 4));
+            set_error("persistent step launch: a workgroup waited past the barrier limit (not all workgroups "
+                      "resident); the block state is undefined");
+            return DTB_RUNTIME_ERROR;
+        }
     }
     return DTB_OK;
 }
@@ -2834,6 +3226,7 @@ static int make_pair(Block* b)
     b->acc2 = nullptr;
     b->pair = true;
     b->stage_ept = 16;
+    b->persist_dirty = true;
     for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
     b->graphs.clear();
     if (had || b->deliver_mode == 1) {
@@ -2844,6 +3237,107 @@ static int make_pair(Block* b)
     }
     return DTB_OK;
 }
+
+namespace dtb {
+
+// Persistent delivery fits when at most kSubLists workgroups of 256 threads
+// cover the neurons with an LDS slice of at most kPersistMaxLds bytes, the
+// synapse positions fit int32 and the fixed-point sums cannot overflow.
+static bool persist_fits(const Block* b, int32_t* G, int32_t* m)
+{
+    const int64_t nblk = (b->n + 255) / 256;
+    const int64_t mm = (nblk + kSubLists - 1) / kSubLists;
+    const int64_t lds = mm * 256 * (b->pair ? 4 : 2) * 4;
+    if (G) *G = (int32_t)std::min<int64_t>(kSubLists, nblk);
+    if (m) *m = (int32_t)mm;
+    return b->n > 0 && b->fixed_ok && lds <= kPersistMaxLds && b->nnz < (1ll << 31) &&
+           b->n * b->D * (std::min<int64_t>(kSubLists, nblk) + 1) < (1ll << 28);
+}
+
+// (Re)build the grouped copy of the CSR: inside every (row, delay) segment
+// the synapses are ordered by owning workgroup ((post >> 8) mod 32, stable),
+// each stored as its index in that workgroup's LDS slice (the workgroup's
+// blocks back to back) with the kind bit, and its weight rounded to the
+// int32 the bucketed accumulation adds (__float2int_rn of the scaled weight).
+static int ensure_persist(Block* b)
+{
+    if (!b->persist_dirty && b->pcol) return DTB_OK;
+    int32_t G = 0, m = 0;
+    DTB_REQUIRE(persist_fits(b, &G, &m),
+                "persistent delivery needs at most %d x 256 x %d neurons, fixed-point sums and fewer than 2^31 "
+                "synapses", kSubLists, kPersistMaxLds / (256 * 4 * (b->pair ? 4 : 2)), kSubLists);
+    DTB_TRY_HIP(hipSetDevice(b->device));
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    const int64_t n = b->n, nnz = b->nnz, D = b->D;
+    std::vector<int64_t> rp((size_t)n + 1);
+    std::vector<uint32_t> col((size_t)nnz);
+    std::vector<float> w((size_t)nnz * (b->pair ? 2 : 1));
+    DTB_TRY_HIP(hipMemcpy(rp.data(), b->rowptr, 8 * (size_t)(n + 1), hipMemcpyDeviceToHost));
+    if (nnz > 0) {
+        DTB_TRY_HIP(hipMemcpy(col.data(), b->col, 4 * (size_t)nnz, hipMemcpyDeviceToHost));
+        DTB_TRY_HIP(hipMemcpy(w.data(), b->pair ? (const void*)b->w2 : (const void*)b->w, 4 * w.size(),
+                              hipMemcpyDeviceToHost));
+    }
+    std::vector<int32_t> bptr((size_t)(n * D) * (G + 1));
+    std::vector<uint32_t> pcol((size_t)nnz);
+    std::vector<int32_t> pval(w.size());
+    const int W = b->pair ? 2 : 1;
+    std::vector<int32_t> cnt(G + 1);
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t e = rp[(size_t)j];
+        for (int64_t d = 1; d <= D; ++d) {
+            // the row is sorted by delay: the segment of delay d is the run whose delay field is d - 1
+            int64_t end = e;
+            while (end < rp[(size_t)j + 1] && (int64_t)(((col[(size_t)end] & 0x7fffffffu) >> b->pk.idx_bits) &
+                                                         b->pk.dmask) == d - 1)
+                ++end;
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int64_t x = e; x < end; ++x) ++cnt[(((col[(size_t)x] & b->pk.idx_mask) >> 8) & (kSubLists - 1)) + 1];
+            int32_t* bp = &bptr[(size_t)((j * D + d - 1) * (G + 1))];
+            bp[0] = (int32_t)e;
+            for (int g = 0; g < G; ++g) bp[g + 1] = bp[g] + cnt[g + 1];
+            std::vector<int32_t> pos(bp, bp + G);
+            for (int64_t x = e; x < end; ++x) {
+                const uint32_t c = col[(size_t)x];
+                const uint32_t post = c & b->pk.idx_mask;
+                const int g = (int)((post >> 8) & (kSubLists - 1));
+                const int32_t to = pos[g]++;
+                pcol[(size_t)to] = (((post >> 13) << 8) | (post & 255u)) | (c & 0x80000000u);
+                for (int y = 0; y < W; ++y)
+                    pval[(size_t)to * W + y] = (int32_t)std::nearbyint(w[(size_t)x * W + y]);
+            }
+            e = end;
+        }
+        DTB_REQUIRE(e == rp[(size_t)j + 1], "row %lld is not sorted by delay", (long long)j);
+    }
+    block_free(b, b->pbptr);
+    block_free(b, b->pcol);
+    block_free(b, b->pval);
+    b->pbptr = nullptr;
+    b->pcol = nullptr;
+    b->pval = nullptr;
+    DTB_TRY(alloc_n(b, &b->pbptr, (int64_t)bptr.size()));
+    DTB_TRY(alloc_n(b, &b->pcol, std::max<int64_t>(nnz, 1)));
+    DTB_TRY(alloc_n(b, &b->pval, std::max<int64_t>((int64_t)pval.size(), 1)));
+    if (!b->pbar) {
+        DTB_TRY(alloc_n(b, &b->pbar, kPubWords));
+        DTB_TRY_HIP(hipMemsetAsync(b->pbar, 0, 4 * kPubWords, b->stream));
+    }
+    DTB_TRY(upload(b, b->pbptr, bptr.data(), 4 * bptr.size()));
+    if (nnz > 0) {
+        DTB_TRY(upload(b, b->pcol, pcol.data(), 4 * (size_t)nnz));
+        DTB_TRY(upload(b, b->pval, pval.data(), 4 * pval.size()));
+    }
+    DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    b->pG = G;
+    b->pm = m;
+    b->persist_dirty = false;
+    for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    return DTB_OK;
+}
+
+} // namespace dtb
 
 static int set_weights_impl(Block* b, int64_t m, const uint32_t* pre, const uint32_t* post, const uint8_t* chan,
                             const float* val)
@@ -3070,15 +3564,20 @@ int dtb_block_create_fixed_indegree(dtb_block** out, const dtb_fixed_indegree_sp
 static int create_members_impl(dtb_block** out, const dtb_member_spec* members, int32_t n_members,
                                const uint64_t* source_cdf, int32_t max_delay, float w_lo, float w_hi,
                                int32_t pair_weights, uint64_t seed, const float* props, double dt,
-                               uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device, bool regions)
+                               uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device, bool regions,
+                               int32_t replicas = 1)
 {
     DTB_REQUIRE(out != nullptr, "out handle pointer is NULL");
     DTB_REQUIRE(members != nullptr && props != nullptr && n_members >= 1, "members and props are required");
     DTB_REQUIRE(max_delay >= 1 && max_delay <= DTB_MAX_DELAY, "max_delay must be in 1..64");
     DTB_REQUIRE(dt > 0.0, "dt must be > 0");
     DTB_REQUIRE(!regions || source_cdf != nullptr, "source_cdf is required");
+    DTB_REQUIRE(replicas >= 1 && (int64_t)n_members * replicas < (1ll << 31), "replicas must be >= 1");
+    const int32_t R0 = n_members;
     std::vector<dtb_member_spec> hm((size_t)n_members);
     DTB_TRY_HIP(hipMemcpy(hm.data(), members, sizeof(dtb_member_spec) * (size_t)n_members, hipMemcpyDefault));
+    for (int32_t r = 1; r < replicas; ++r) hm.insert(hm.end(), hm.begin(), hm.begin() + R0);
+    n_members *= replicas;
     int64_t n = 0, nnz = 0;
     for (const auto& m : hm) {
         DTB_REQUIRE(m.n >= 1 && m.n_exc >= 0 && m.n_exc <= m.n && m.in_degree >= 0 && m.inter_degree >= 0,
@@ -3100,7 +3599,8 @@ static int create_members_impl(dtb_block** out, const dtb_member_spec* members, 
         if ((st = make_pack(b, max_delay))) break;
         b->max_row = 8192;
         if ((st = alloc_state(b))) break;
-        if ((st = gen_members(b, hm.data(), n_members, max_delay, w_lo, w_hi, seed, regions ? source_cdf : nullptr)))
+        if ((st = gen_members(b, hm.data(), n_members, max_delay, w_lo, w_hi, seed, regions ? source_cdf : nullptr,
+                              R0)))
             break;
         st = finish_create(b, props);
     } while (0);
@@ -3128,6 +3628,15 @@ int dtb_block_create_regions(dtb_block** out, const dtb_member_spec* members, in
 {
     return create_members_impl(out, members, n_members, source_cdf, max_delay, w_lo, w_hi, pair_weights, seed, props,
                                dt, noise_seed, stats_exclude_blocked, device, true);
+}
+
+int dtb_block_create_region_replicas(dtb_block** out, const dtb_member_spec* regions, int32_t n_regions,
+                                     int32_t replicas, const uint64_t* source_cdf, int32_t max_delay, float w_lo,
+                                     float w_hi, int32_t pair_weights, uint64_t seed, const float* props, double dt,
+                                     uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device)
+{
+    return create_members_impl(out, regions, n_regions, source_cdf, max_delay, w_lo, w_hi, pair_weights, seed, props,
+                               dt, noise_seed, stats_exclude_blocked, device, true, replicas);
 }
 
 int dtb_block_destroy(dtb_block* b)
@@ -3423,6 +3932,7 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
         DTB_TRY(ensure_bucketed(b));
         DTB_REQUIRE(value == 0 || b->fixed_ok,
                     "fixed-point accumulation needs 2^S resolution with S >= 16 (largest per-step input sum < 16384)");
+        DTB_REQUIRE(value == 1 || b->deliver_mode != 2, "persistent delivery sums in fixed point");
         b->accum_fixed = value != 0;
     } else if (what == DTB_LAUNCH_ACCUM_UNROLL) {
         DTB_REQUIRE(value == 8 || value == 16, "accumulation unroll must be 8 or 16");
@@ -3441,8 +3951,13 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
         }
         b->stage_variant = value;
     } else if (what == DTB_LAUNCH_DELIVER_MODE) {
-        DTB_REQUIRE(value == 0 || value == 1, "delivery mode must be 0 (atomics) or 1 (bucketed)");
-        if (value == 1) DTB_TRY(ensure_bucketed(b));
+        DTB_REQUIRE(value >= 0 && value <= 2,
+                    "delivery mode must be 0 (atomics), 1 (bucketed) or 2 (persistent, small blocks)");
+        if (value >= 1) DTB_TRY(ensure_bucketed(b));
+        if (value == 2) {
+            DTB_TRY(ensure_persist(b));
+            b->accum_fixed = true; // batches that fall back to bucketed kernels sum as the persistent launch does
+        }
         b->deliver_mode = value;
     } else {
         set_error("unknown launch parameter %d", what);

@@ -139,6 +139,15 @@ struct Block {
     void* stage = nullptr;      // [nbk][stage_cap]
     void* acc2 = nullptr;       // [n] overflow accumulator
 
+    // persistent delivery (deliver_mode 2): the CSR grouped by owning
+    // workgroup inside every (row, delay) segment, rebuilt when the weights change
+    int32_t* pbptr = nullptr;   // [n*D][pG+1] start of each workgroup's synapses
+    uint32_t* pcol = nullptr;   // [nnz] local index | kind << 31
+    int32_t* pval = nullptr;    // [nnz] or [nnz][2]: round(scaled weight)
+    uint32_t* pbar = nullptr;   // [0] barrier counter, [64] abort flag
+    int32_t pG = 0, pm = 0;     // workgroups, neuron blocks per workgroup
+    bool persist_dirty = true;
+
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
     std::vector<void*> allocs;
 };
@@ -147,6 +156,6 @@ int block_alloc(Block* b, void** p, size_t bytes);
 int derive_props(Block* b, bool psets_dirty = true);
 int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec);
 int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_delay, float w_lo, float w_hi,
-                uint64_t seed, const uint64_t* cdf);
+                uint64_t seed, const uint64_t* cdf, int32_t R0);
 
 } // namespace dtb

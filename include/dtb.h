@@ -121,6 +121,19 @@ int dtb_block_create_regions(dtb_block** out, const dtb_member_spec* members, in
                              int32_t pair_weights, uint64_t seed, const float* props, double dt,
                              uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device);
 
+/* `replicas` copies of one region network back to back (replica r holds
+ * regions r*n_regions .. (r+1)*n_regions - 1): every replica draws its
+ * inter-region sources through the same source_cdf [n_regions][n_regions],
+ * inside itself.  The same network as dtb_block_create_regions with the
+ * block-diagonal source table of all replicas, without that table's
+ * (replicas*n_regions)^2 entries.  For ensembles of whole-brain members
+ * (data_assimilation/DataAssimilation.py:97 replicates equal members); props
+ * holds the rows of all replicas. */
+int dtb_block_create_region_replicas(dtb_block** out, const dtb_member_spec* regions, int32_t n_regions,
+                                     int32_t replicas, const uint64_t* source_cdf, int32_t max_delay, float w_lo,
+                                     float w_hi, int32_t pair_weights, uint64_t seed, const float* props, double dt,
+                                     uint64_t noise_seed, int32_t stats_exclude_blocked, int32_t device);
+
 int dtb_block_destroy(dtb_block* b);
 
 /* Noise modes of dtb_block_run. */
@@ -286,7 +299,11 @@ int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* t
  * beyond floating-point summation order. */
 #define DTB_LAUNCH_DELIVER_GRID 0  /* 1..65536 */
 #define DTB_LAUNCH_DELIVER_LANES 1 /* 16, 32 or 64 */
-#define DTB_LAUNCH_DELIVER_MODE 2  /* 0: global atomics, 1: bucketed LDS accumulation */
+#define DTB_LAUNCH_DELIVER_MODE 2  /* 0: global atomics, 1: bucketed LDS accumulation, 2: persistent (small
+                                      blocks: one launch per step batch with a grid barrier per step, at most
+                                      32 workgroups; fixed-point sums, results equal to mode 1; batches that
+                                      need injected uniforms, sources, the external drive, I_syn means, samples
+                                      or kernel times run on the mode-1 kernels) */
 #define DTB_LAUNCH_NEURON_PSETS 3  /* set: 0 per-neuron constants, 1 shared parameter sets when <= 64
                                       distinct; get: number of sets in use (0 = per-neuron) */
 #define DTB_LAUNCH_STAGE_SEGS 4    /* bucketed delivery: at most 64, 128 or 256 (row, delay) segments per chunk (the

@@ -36,10 +36,13 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-5
 
 
-def test_plain_block_8000_steps_match_oracle():
+@pytest.mark.parametrize("mode", [None, 2])
+def test_plain_block_8000_steps_match_oracle(mode):
     from digital_twin_brain_amd import networks
 
     nb, props = networks.build_plain()
+    if mode is not None:
+        nb.set_launch(deliver_mode=mode)  # 2: the persistent launch of small blocks
     n = nb.n
     assert n == 10_000 and nb.nnz == 1_000_000
     csr = dict(rowptr=nb.get("rowptr"), post=nb.get("post"), kind=nb.get("kind"), delay=nb.get("delay"),
@@ -107,7 +110,7 @@ def test_plain_block_8000_steps_match_oracle():
         ob.set("V", V_g)
         ob.set("J", J_g)
         ob.set("t_last", nb.get("t_last"))
-    print(f"plain block: {steps} steps, {n_thr} threshold and {n_spk} delivered spikes compared; free-running "
+    print(f"plain block (launch {nb.get_launch()}): {steps} steps, {n_thr} threshold and {n_spk} delivered spikes compared; free-running "
           f"first threshold divergence {'none' if first_div is None else f'at step {first_div}'} (worst V "
           f"{free_worst_v:.2e} relative before it); per step: {len(ties)} ties {ties[:10]}, worst V {worst_v:.2e}, "
           f"worst J {worst_j:.2e} relative")
