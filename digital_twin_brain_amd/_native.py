@@ -82,6 +82,8 @@ def lib():
         "dtb_block_create_members": [_vp, _vp, _i32, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32, _i32],
         "dtb_block_create_regions": [_vp, _vp, _i32, _vp, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32,
                                      _i32],
+        "dtb_block_create_region_replicas": [_vp, _vp, _i32, _i32, _vp, _i32, _f32, _f32, _i32, _u64, _vp, _f64,
+                                             _u64, _i32, _i32],
         "dtb_block_run": [_vp, _i64, _vp],
         "dtb_block_get": [_vp, _i32, _vp, _i64, _vp],
         "dtb_block_info": [_vp, _vp],
@@ -127,7 +129,7 @@ def lib():
 def exported_symbols():
     """Names the header declares, for the load test."""
     return ["dtb_last_error", "dtb_abi_version", "dtb_block_create", "dtb_block_create_fixed_indegree",
-            "dtb_block_create_members", "dtb_block_create_regions",
+            "dtb_block_create_members", "dtb_block_create_regions", "dtb_block_create_region_replicas",
             "dtb_block_destroy", "dtb_block_run", "dtb_block_get", "dtb_block_info", "dtb_block_time",
             "dtb_block_state_bytes", "dtb_block_save_state", "dtb_block_load_state",
             "dtb_block_set_props", "dtb_block_mul_props_by_pop", "dtb_block_gamma_props_by_pop",

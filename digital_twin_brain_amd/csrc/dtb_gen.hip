@@ -41,7 +41,8 @@ struct GenSpec {
     float w_lo, span;
     uint32_t k0, k1;
     const GenMember* __restrict__ mem;
-    const uint64_t* __restrict__ cdf; // [M][M] source tables (scaled to 2^32), or null
+    const uint64_t* __restrict__ cdf; // [R0][R0] source tables (scaled to 2^32), or null
+    int32_t R0; // members per replica: member m = rep * R0 + r draws inter sources from row r, inside replica rep
 };
 
 constexpr uint32_t kStreamRegion = 0x52454749u; // ctr = (k, post_lo, post_hi, REGION)
@@ -98,14 +99,15 @@ __device__ inline uint32_t syn_pre(const GenSpec& s, int32_t m, int64_t i, int32
     if (k < g.Ki) return conn_pre(g, i, r0);
     const u32x4 q =
         philox4x32_10(u32x4{(uint32_t)k, (uint32_t)i, (uint32_t)((uint64_t)i >> 32), kStreamRegion}, s.k0, s.k1);
-    const uint64_t* row = s.cdf + (int64_t)m * s.M;
-    int32_t lo = 0, hi = s.M - 1; // smallest source with q0 < row[source]; row[M-1] = 2^32
+    const int32_t rep = m / s.R0;
+    const uint64_t* row = s.cdf + (int64_t)(m - rep * s.R0) * s.R0;
+    int32_t lo = 0, hi = s.R0 - 1; // smallest source with q0 < row[source]; row[R0-1] = 2^32
     while (lo < hi) {
         const int32_t mid = (lo + hi) >> 1;
         if ((uint64_t)q.x < row[mid]) hi = mid;
         else lo = mid + 1;
     }
-    const GenMember& h = s.mem[lo];
+    const GenMember& h = s.mem[rep * s.R0 + lo];
     return (uint32_t)(h.off + (int64_t)(((uint64_t)r0 * (uint64_t)h.n) >> 32));
 }
 
@@ -371,8 +373,10 @@ static int gen_run(Block* b, const GenSpec& s, int32_t max_delay)
 }
 
 int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_delay, float w_lo, float w_hi,
-                uint64_t seed, const uint64_t* cdf)
+                uint64_t seed, const uint64_t* cdf, int32_t R0)
 {
+    if (R0 <= 0) R0 = M;
+    DTB_REQUIRE(M % R0 == 0, "members must be whole replicas of the source table's %d members", R0);
     const int64_t n = b->n;
     const int64_t nnz = b->nnz;
     DTB_REQUIRE(n <= (1ll << 26), "generated networks are limited to 2^26 neurons");
@@ -393,13 +397,13 @@ int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_del
     DTB_TRY_HIP(hipMalloc(&dmem, sizeof(GenMember) * (size_t)M));
     DTB_TRY_HIP(hipMemcpy(dmem, hm.data(), sizeof(GenMember) * (size_t)M, hipMemcpyHostToDevice));
     if (inter) {
-        std::vector<uint64_t> hc((size_t)M * (size_t)M);
+        std::vector<uint64_t> hc((size_t)R0 * (size_t)R0);
         hipError_t e = hipMemcpy(hc.data(), cdf, sizeof(uint64_t) * hc.size(), hipMemcpyDefault);
         int bad = e != hipSuccess;
-        for (int32_t m = 0; m < M && !bad; ++m) {
-            const uint64_t* row = hc.data() + (size_t)m * M;
-            for (int32_t j = 1; j < M; ++j) bad |= row[j] < row[j - 1];
-            bad |= row[M - 1] != (1ull << 32);
+        for (int32_t m = 0; m < R0 && !bad; ++m) {
+            const uint64_t* row = hc.data() + (size_t)m * R0;
+            for (int32_t j = 1; j < R0; ++j) bad |= row[j] < row[j - 1];
+            bad |= row[R0 - 1] != (1ull << 32);
         }
         if (!bad) e = hipMalloc(&dcdf, sizeof(uint64_t) * hc.size());
         if (!bad && e == hipSuccess) e = hipMemcpy(dcdf, hc.data(), sizeof(uint64_t) * hc.size(), hipMemcpyHostToDevice);
@@ -414,7 +418,7 @@ int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_del
             return DTB_RUNTIME_ERROR;
         }
     }
-    GenSpec s{n, nnz, M, max_delay, w_lo, w_hi - w_lo, (uint32_t)seed, (uint32_t)(seed >> 32), dmem, dcdf};
+    GenSpec s{n, nnz, M, max_delay, w_lo, w_hi - w_lo, (uint32_t)seed, (uint32_t)(seed >> 32), dmem, dcdf, R0};
     int st = gen_run(b, s, max_delay);
     hipFree(dmem);
     if (dcdf) hipFree(dcdf);
@@ -424,7 +428,7 @@ int gen_members(Block* b, const dtb_member_spec* mem, int32_t M, int32_t max_del
 int gen_fixed_indegree(Block* b, const dtb_fixed_indegree_spec* spec)
 {
     dtb_member_spec m{spec->n, spec->n_exc, spec->in_degree, 0};
-    return gen_members(b, &m, 1, spec->max_delay, spec->w_lo, spec->w_hi, spec->seed, nullptr);
+    return gen_members(b, &m, 1, spec->max_delay, spec->w_lo, spec->w_hi, spec->seed, nullptr, 1);
 }
 
 } // namespace dtb

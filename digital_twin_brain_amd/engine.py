@@ -128,6 +128,27 @@ class NativeBlock:
                                                      int(device)))
         return cls(h.value)
 
+    @classmethod
+    def region_replicas(cls, props, regions, replicas, source_cdf, max_delay=1, w_lo=0.0, w_hi=1.0,
+                        pair_weights=False, conn_seed=1, dt=0.1, noise_seed=3, stats_exclude_blocked=False, device=0):
+        """`replicas` copies of one region network back to back, each drawing its
+        inter-region sources through the same source_cdf [R, R] inside itself;
+        props holds the rows of all replicas."""
+        nat.require_gpu()
+        props = np.ascontiguousarray(_as_np(props, np.float32)[:, :nat.NPROP])
+        cdf = np.ascontiguousarray(source_cdf, dtype=np.uint64)
+        if cdf.shape != (len(regions), len(regions)):
+            raise ValueError("source_cdf must be [regions, regions]")
+        arr = (nat.MemberSpec * len(regions))(*[nat.MemberSpec(int(n), int(ne), int(k), int(kx))
+                                                for n, ne, k, kx in regions])
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().dtb_block_create_region_replicas(ctypes.byref(h), arr, len(regions), int(replicas),
+                                                             nat.ptr(cdf), int(max_delay), float(w_lo), float(w_hi),
+                                                             int(bool(pair_weights)), int(conn_seed),
+                                                             nat.ptr(props), float(dt), int(noise_seed),
+                                                             int(bool(stats_exclude_blocked)), int(device)))
+        return cls(h.value)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             nat.lib().dtb_block_destroy(self._h)

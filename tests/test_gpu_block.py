@@ -658,6 +658,36 @@ def test_gpu_regions_generator_csr_bit_exact():
         assert np.array_equal(nb.get("w1"), ref["w1"] if pair else ref["w0"])
 
 
+def test_region_replicas_equal_the_block_diagonal_region_network():
+    """dtb_block_create_region_replicas (one source table shared by every
+    replica, sources drawn inside the replica) builds the same CSR, bit for
+    bit, as dtb_block_create_regions given the block-diagonal table of all
+    replicas -- which the test above pins to the oracle generator."""
+    from digital_twin_brain_amd import NativeBlock
+
+    props, spec, cdf = _region_case()
+    reps = 3
+    R = len(spec)
+    big = np.zeros((reps * R, reps * R), np.uint64)
+    for r in range(reps):
+        big[r * R:(r + 1) * R, r * R:(r + 1) * R] = cdf
+        big[r * R:(r + 1) * R, (r + 1) * R:] = np.uint64(1 << 32)
+    allp = np.concatenate([props] * reps)
+    for pair, D in ((False, 12), (True, 3)):
+        a = NativeBlock.region_replicas(allp, spec, reps, cdf, max_delay=D, pair_weights=pair, conn_seed=23)
+        b = NativeBlock.regions(allp, spec * reps, big, max_delay=D, pair_weights=pair, conn_seed=23)
+        assert a.n == b.n == reps * int(sum(x[0] for x in spec)) and a.nnz == b.nnz
+        for k in ("rowptr", "post", "kind", "delay", "w0", "w1"):
+            assert np.array_equal(a.get(k), b.get(k)), k
+        # no synapse leaves its replica
+        rp = a.get("rowptr")
+        pre = np.repeat(np.arange(a.n), np.diff(rp))
+        m = int(sum(x[0] for x in spec))
+        assert np.array_equal(pre // m, a.get("post") // m)
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_region_network_with_external_drive_matches_oracle(mode):
     """Region network, delays 1..40, external Poisson drive: the GPU against
