@@ -131,7 +131,7 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
 
     def reset_hp():
         hp = da.hp_random_initialize(low, high, rng=np.random.default_rng(seed + 1))
-        da.mul_property_by_subblk(da._hp_index_updating, hp.float().reshape(-1))
+        da._push(hp)
         da.timings = []
 
     reset_hp()

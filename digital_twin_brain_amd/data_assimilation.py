@@ -94,9 +94,11 @@ class DataAssimilation(simulation):
         self._hp = self.sigmoid_torch(self._hp_log, self._hp_low, self._hp_high).float()
         return self._hp.reshape(-1)
 
-    def hp_index2hidden_state(self, cortical_or_not=None, index_da_voxel_pblk=None):
+    def hp_index2hidden_state(self, cortical_or_not=None, index_da_voxel_pblk=None, hp_rows_per_voxel=2):
         """data_assimilation/DataAssimilation.py:254-293 (voxel version: 2 populations
-        per voxel; micro-columns: 8 cortical + 2 sub-cortical layouts)."""
+        per voxel; micro-columns: 8 cortical + 2 sub-cortical layouts).  With
+        hp_rows_per_voxel=1 one row of hyper-parameters serves both
+        populations of a voxel (see da_property_initialize's pops_per_hp)."""
         Vb = self._num_voxel_pblk
         self._index_da_voxel_pblk = torch.arange(Vb) if index_da_voxel_pblk is None else \
             torch.tensor(index_da_voxel_pblk).reshape(-1)
@@ -105,8 +107,8 @@ class DataAssimilation(simulation):
         nv = cort.shape[0]
         H = self._hp_num
         if cort.sum() == 0:
-            self.for_hidden_state = torch.arange(nv * 2 * H)
-            self.from_hidden_state = torch.arange(nv * 2 * H)
+            self.for_hidden_state = torch.arange(nv * hp_rows_per_voxel * H)
+            self.from_hidden_state = torch.arange(nv * hp_rows_per_voxel * H)
         else:
             fhs = 10 * torch.arange(nv).reshape(-1, 1) + torch.tensor([6, 7]).repeat(4)
             fhs[torch.from_numpy(cort)] += torch.tensor([-4, -4, -2, -2, 0, 0, 2, 2])
@@ -116,8 +118,10 @@ class DataAssimilation(simulation):
             frm, _ = torch.cat((vox, cor.reshape(-1))).reshape(-1, 1).sort(dim=0)
             self.from_hidden_state = (H * frm + torch.arange(H)).reshape(-1)
 
-    def da_property_initialize(self, property_index, alpha, gui, index_da_population=None):
-        """data_assimilation/DataAssimilation.py:295-325."""
+    def da_property_initialize(self, property_index, alpha, gui, index_da_population=None, pops_per_hp=1):
+        """data_assimilation/DataAssimilation.py:295-325.  pops_per_hp > 1: one
+        row of hyper-parameters (gui) drives that many consecutive populations
+        (for example a region's E and I populations sharing its 4 g_ui)."""
         pops = self.population_id.cpu()[: self._da_populations] if index_da_population is None \
             else torch.as_tensor(index_da_population)
         if alpha is not None:  # alpha=None keeps the existing values and only scales them
@@ -126,7 +130,24 @@ class DataAssimilation(simulation):
         self._property_index = torch.tensor(property_index, dtype=torch.int64).reshape(-1)
         self._hp_index_updating = torch.stack(torch.meshgrid(pops, self._property_index, indexing="ij"),
                                               dim=-1).reshape(-1, 2).to(torch.int64)
-        self.mul_property_by_subblk(self._hp_index_updating, gui.float().reshape(-1))
+        H = self._property_index.numel()
+        if pops_per_hp > 1:
+            row = torch.arange(pops.numel()) // int(pops_per_hp)
+            self._hp_push_gather = (row.reshape(-1, 1) * H + torch.arange(H)).reshape(-1)
+        else:
+            self._hp_push_gather = None
+        self._push(gui)
+
+    def _push(self, hp):
+        vals = hp.float().reshape(-1)
+        if getattr(self, "_hp_push_gather", None) is not None:
+            vals = vals[self._hp_push_gather]
+        self.mul_property_by_subblk(self._hp_index_updating, vals)
+
+    def push_hp(self):
+        """Push the current hyper-parameters into the network (relative to the
+        original properties, cuda/src/stage/property.cpp:145-284)."""
+        self._push(self._hp)
 
     def get_hidden_state(self, steps=800, show_info=False):
         """data_assimilation/DataAssimilation.py:327-351."""
@@ -141,7 +162,7 @@ class DataAssimilation(simulation):
         """data_assimilation/DataAssimilation.py:353-370."""
         self._hp_log += torch.normal(0, hp_sigma ** 0.5, size=self._hp_log.shape, generator=generator).float()
         self._hp = self.sigmoid_torch(self._hp_log, self._hp_low, self._hp_high)
-        self.mul_property_by_subblk(self._hp_index_updating, self._hp.float().reshape(-1))
+        self.push_hp()
         self.get_hidden_state(steps)
 
     def da_filter(self, bold_real_t, bold_sigma=1e-8, solo_rate=0.8, debug=False, bound=None, xi=None):

@@ -9,6 +9,8 @@ large block     N=5e6, in-degree 1,000 (5e9 synapses, compact int32 index +
                 per ms.
 ensemble        E members of log-uniform size and uniform in-degree packed in
                 one block, plus a hidden "truth" member.
+region ensemble E whole-brain members (replicas of one 90-region network with
+                a shared connectome) plus a hidden truth replica.
 Property columns follow generation/make_block.py:394-434.
 """
 from __future__ import annotations
@@ -187,3 +189,35 @@ def build_regions(R=90, total=20_000_000, k_intra=400, k_inter=100, max_delay=40
     internal = (k_intra + k_inter) * target_hz * dt * 1e-3  # input events per neuron per step
     nb.set_external(ext_fraction / (1.0 - ext_fraction) * internal, 0.5)
     return nb, p, spec, cdf
+
+
+def build_region_ensemble(E=256, R=90, member_n=20_000, k_intra=80, k_inter=20, max_delay=40, ext_fraction=0.2,
+                          target_hz=8.0, truth=True, dt=0.1, device=0, conn_seed=1, noise_seed=3, min_size=40,
+                          size_seed=11, conn_matrix_seed=12, props_seed=13):
+    """Ensemble of whole-brain members for data assimilation: E replicas (plus
+    a hidden truth replica) of one R-region network -- lognormal(sigma 0.8)
+    region sizes summing to member_n, k_intra inputs from the own region and
+    k_inter from the others through one Exp(1) connectome shared by all
+    replicas, delays 1..max_delay, the external Poisson drive of the region
+    share.  Replicas differ in their synapse and noise draws and in the +-20%
+    per-neuron conductance jitter.  Sub-population of replica m, region r:
+    2 (m R + r) + {0: E, 1: I}; one BOLD voxel per region.  Returns
+    (block, props, spec, cdf) with spec the (n, n_exc, k_intra, k_inter) of
+    one replica's regions."""
+    sizes = region_sizes(R, member_n, min_size=min_size, seed=size_seed)
+    cdf = source_cdf(connectome(R, seed=conn_matrix_seed))
+    reps = E + (1 if truth else 0)
+    parts = []
+    spec = None
+    for m in range(reps):
+        p, regs = region_props(sizes, k_intra + k_inter, seed=props_seed + m, dt=dt)
+        p[:, 3] += 2 * R * m
+        parts.append(p)
+        spec = [(n, ne, k_intra, k_inter) for n, ne in regs]
+    props = np.concatenate(parts)
+    nb = NativeBlock.region_replicas(props, spec, reps, cdf, max_delay=max_delay, w_lo=0.0, w_hi=1.0,
+                                     conn_seed=conn_seed, dt=dt, noise_seed=noise_seed, stats_exclude_blocked=True,
+                                     device=device)
+    internal = (k_intra + k_inter) * target_hz * dt * 1e-3
+    nb.set_external(ext_fraction / (1.0 - ext_fraction) * internal, 0.5)
+    return nb, props, spec, cdf

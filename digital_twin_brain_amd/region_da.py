@@ -1,0 +1,289 @@
+"""Surrogate-assisted data assimilation on an ensemble of whole-brain members
+(no counterpart in the reference: the surrogate, the judge and the fallback
+policy are new in this library; the assimilation loop they accelerate is
+data_assimilation/DataAssimilation.py:402-453).
+
+Members.  Every member is a replica of one R-region network (lognormal region
+sizes, intra- and inter-region inputs through one shared connectome, delays
+up to 40 steps, external Poisson drive), packed with a hidden truth replica
+into one block (networks.build_region_ensemble).  Its hyper-parameters are the
+4 receptor conductances g_ui (property columns 10-13) of each region, shared
+by the region's E and I populations: the hidden state per voxel (= region) is
+[4 hp logits, act, s, q, v, f_in, bold].
+
+Surrogate.  The per-member window map (WindowSurrogate, per_voxel=False): the
+4 hp logits of every region, the BOLD state (s, q, v, f_in) at the window
+start, the previous window's rate per region and a one-hot of the member ->
+the window's (act, s, q, v, f_in, bold) per region, a 3 x 512 bf16 MLP
+trained on the simulator's own windows.  The member one-hot matters: members
+are replicas with their own synapse and cell draws, and two replicas under the
+same hyper-parameters differ in BOLD by more than two noise realizations of
+one replica do.
+
+Fallback (member-subset judge).  A forced full-ensemble simulation every
+`every`-th window caps the gain below `every` (the window cost is then at
+least T_sim / every).  Here the periodic check simulates only the first
+`subset` members, on a separate block holding copies of exactly those
+members (the same synapses and properties, another noise stream:
+SubsetJudge): it pushes their current hyper-parameters, lets the network
+relax under them for `pre_ms`, sets their window-start BOLD state and runs
+the window.  Its cost is T_sim * subset / E.  The subset's simulated windows
+replace the surrogate's forecasts for those members, are added to the
+training set and fine-tune the surrogate.  When the subset's NRMSE
+(surrogate against simulation, over the subset's BOLD) exceeds `nrmse_max`,
+the next window simulates the full ensemble, and so on until a full window
+passes.
+
+The simulator is stochastic: two simulations of the same members from the
+same state and hyper-parameters differ.  run_region_surrogate_da measures
+that floor (the judge's simulation against the full simulation of the same
+members in one window) and, with nrmse_max=None, sets the threshold to
+max(0.02, 1.25 x floor): the surrogate counts as wrong when it departs from
+the simulation by more than another simulation would.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from . import networks
+from .data_assimilation import DataAssimilation
+from .surrogate_da import WindowSurrogate, collect_windows, nrmse, window_inputs
+
+HP_COLUMNS = (10, 11, 12, 13)  # g_AMPA, g_NMDA, g_GABAa, g_GABAb of every region
+
+
+def make_region_da(E, R, member_n, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, truth=True, tune=True, **kw):
+    """Region ensemble (+ truth replica) and its DataAssimilation; the initial
+    hyper-parameters are the reference's logit grid, pushed; the truth's
+    factors are drawn once and pushed.  Returns (nb, da, truth_hp, (low, high))."""
+    rng = np.random.default_rng(seed)
+    nb, props, spec, cdf = networks.build_region_ensemble(E=E, R=R, member_n=member_n, truth=truth, dt=dt, **kw)
+    sps = int(round(1.0 / dt))  # BOLD once per ms
+    da = DataAssimilation(network=nb, column=False, ensemble=E, dt=dt, bold_steps=sps, da_populations=E * 2 * R,
+                          da_voxels=E * R, write_path="/tmp")
+    H = len(HP_COLUMNS)
+    low = torch.full((R, H), hp_lo)
+    high = torch.full((R, H), hp_hi)
+    hp = da.hp_random_initialize(low, high, num_da_populations_pblk=R, rng=rng)
+    da.hp_index2hidden_state(hp_rows_per_voxel=1)
+    da.da_property_initialize(list(HP_COLUMNS), None, hp, pops_per_hp=2)
+    truth_hp = None
+    if truth:
+        truth_hp = rng.uniform(hp_lo, hp_hi, size=(R, H)).astype(np.float32)
+        pops = np.arange(E * 2 * R, (E + 1) * 2 * R)
+        nb.mul_props_by_pop(np.repeat(pops, H).astype(np.uint32), np.tile(HP_COLUMNS, 2 * R).astype(np.uint32),
+                            np.repeat(truth_hp, 2, axis=0).reshape(-1))
+    if tune:
+        from .autotune import tune_delivery
+
+        nb.run(10 * sps, noise_mode=1, graph=False)  # leave the initial transient before measuring
+        tune_delivery(nb, dict(noise_mode=1), steps=2 * sps)
+    return nb, da, truth_hp, (low, high)
+
+
+class SubsetJudge:
+    """Copies of the first `subset` members on a block of their own (the
+    generator draws a member's synapses from its own neuron indices, so the
+    same seeds give the same members), with another noise stream; simulates
+    their windows from given hyper-parameters and window-start BOLD state."""
+
+    def __init__(self, subset, R, member_n, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=8, pre_ms=100.0, noise_seed=103,
+                 **kw):
+        self.S = int(subset)
+        self.nb, self.da, _, _ = make_region_da(self.S, R, member_n, dt=dt, hp_lo=hp_lo, hp_hi=hp_hi, seed=seed,
+                                                truth=False, noise_seed=noise_seed, **kw)
+        self.pre_steps = int(round(pre_ms / dt))
+        self.pre_steps -= self.pre_steps % self.da.bold_steps
+
+    def simulate(self, hp_log, bold_start, steps):
+        """hp_log [S, R*4] logits; bold_start [S, R, 4] (s, q, v, f_in) ->
+        the window's hidden-state tail [S, R, 6] (float64, CPU)."""
+        da = self.da
+        da._hp_log = torch.as_tensor(hp_log, dtype=torch.float32).reshape(self.S, -1).clone()
+        da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
+        da.push_hp()
+        if self.pre_steps:
+            da.evolve(self.pre_steps, bold_detail=False)  # the network settles under the new conductances
+        da.bold.state_update(torch.as_tensor(bold_start, dtype=torch.float64))
+        da.get_hidden_state(steps)
+        return da._hidden_state[:, :, -6:].double().cpu()
+
+
+class SubsetSurrogateDA:
+    """The window loop with the surrogate forecasting every member, a
+    member subset simulated every `every`-th window as the judge,
+    and full-ensemble simulation after a judged (or full) window whose NRMSE
+    exceeds nrmse_max.  Every simulated window fine-tunes the surrogate on
+    its rows plus an equal replay of earlier rows."""
+
+    def __init__(self, da, surrogate, judge, every=5, nrmse_max=0.02, finetune_epochs=20, replay=1.0):
+        self.da = da
+        self.sur = surrogate
+        self.judge = judge
+        self.every = int(every)
+        self.nrmse_max = float(nrmse_max)
+        self.finetune_epochs = int(finetune_epochs)
+        self.replay = float(replay)
+
+    def run(self, observe, windows, steps, bold_sigma=1e-8, solo_rate=0.8, hp_sigma=0.25):
+        da = self.da
+        E = da.ensemble_number
+        S = self.judge.S
+        forecasts, kinds, errs, times = [], [], [], []
+        full_next = False
+        for t in range(windows):
+            t0 = time.perf_counter()
+            da._hp_log += torch.normal(0, hp_sigma ** 0.5, size=da._hp_log.shape).float()
+            da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
+            X = window_inputs(da)
+            pred = self.sur.predict(X)  # [E, V, 6] on the device
+            hpl = da._hp_log.reshape(E, -1)[:, da.for_hidden_state].reshape(E, pred.shape[1], -1)
+            if full_next:
+                kind = "full"
+                da.push_hp()
+                da.get_hidden_state(steps)
+                tail = da._hidden_state[:, :, -6:].double().cpu()
+                e = nrmse(pred[:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
+                self.sur.add(X, tail.reshape(E, -1))
+            elif t % self.every == 0:
+                kind = "judge"
+                J = np.arange(S)
+                bold0 = X.reshape(E, -1, 9)[J][:, :, 4:8]
+                tail = self.judge.simulate(da._hp_log.reshape(E, -1)[J], bold0, steps)
+                e = nrmse(pred[J][:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
+                self.sur.add(X[J], tail.reshape(S, -1), idx=J)
+                full = torch.cat((hpl.double().to(pred.device), pred), dim=2)
+                full[torch.from_numpy(J).to(pred.device), :, -6:] = tail.to(pred.device)
+                da._hidden_state = full
+            else:
+                kind = "surrogate"
+                e = None
+                da._hidden_state = torch.cat((hpl.double().to(pred.device), pred), dim=2)
+            if e is not None:
+                errs.append((t, kind, e))
+                full_next = e > self.nrmse_max
+                if self.finetune_epochs:
+                    self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t)
+            forecasts.append(da._hidden_state.cpu().numpy())
+            kinds.append(kind)
+            y = observe(t)
+            da.da_filter(torch.as_tensor(y).reshape(1, -1), bold_sigma=bold_sigma, solo_rate=solo_rate)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        return dict(forecasts=np.stack(forecasts), kinds=kinds, nrmse=errs, times=np.array(times))
+
+
+def _snapshot(da):
+    return dict(block=da.block_model.state_dict(), hp_log=da._hp_log.clone(), hp=da._hp.clone(),
+                hidden=None if da._hidden_state is None else da._hidden_state.clone(),
+                rng_cpu=torch.get_rng_state(), rng_dev=torch.cuda.get_rng_state(da.block_model._dev))
+
+
+def _restore(da, s):
+    da.block_model.load_state_dict(s["block"])
+    da._hp_log = s["hp_log"].clone()
+    da._hp = s["hp"].clone()
+    da._hidden_state = None if s["hidden"] is None else s["hidden"].clone()
+    torch.set_rng_state(s["rng_cpu"])
+    torch.cuda.set_rng_state(s["rng_dev"], da.block_model._dev)
+    da.timings = []
+
+
+def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, windows=20, window_ms=800.0, dt=0.1,
+                            subset=16, every=5, nrmse_max=None, hidden=(512, 512, 512), epochs=30,
+                            finetune_epochs=20, seed=7, pre_ms=100.0, member_ids=True, log=None, tune=True, **kw):
+    """Simulator-only DA against surrogate-assisted DA (subset judge) on E
+    whole-brain members, from the same state and hyper-parameters, against
+    the same truth observations.  Returns a dict of timings, gain, the
+    fallback pattern and NRMSE values, and the run records."""
+    say = log or (lambda *a: None)
+    t0 = time.perf_counter()
+    nb, da, truth_hp, (low, high) = make_region_da(E, R, member_n, dt=dt, seed=seed, tune=tune, **kw)
+    judge = SubsetJudge(subset, R, member_n, dt=dt, seed=seed + 1, pre_ms=pre_ms, **kw)
+    t_build = time.perf_counter() - t0
+    steps = int(round(window_ms / dt))
+    truth_vox = slice(E * R, (E + 1) * R)
+    da.evolve(steps, bold_detail=True)
+    say(f"region DA: {E} members + truth, {nb.n} neurons, {nb.nnz} synapses, judge {subset} members, built and "
+        f"warmed in {time.perf_counter() - t0:.1f} s")
+    # training windows: random walks of the logits from the assimilation's starting spread
+    # (twice its per-window step), restarted every 8 windows
+    t0 = time.perf_counter()
+    Xs, Ys = [], []
+    rng = np.random.default_rng(seed + 3)
+    for w0 in range(0, train_windows, 8):
+        da.hp_random_initialize(low, high, num_da_populations_pblk=R, rng=np.random.default_rng(seed + 100 + w0))
+        Xw, Yw = collect_windows(da, min(8, train_windows - w0), steps, rng=rng, walk_sigma=2 * 0.25 ** 0.5)
+        Xs.append(Xw)
+        Ys.append(Yw)
+    X, Y = torch.cat(Xs), torch.cat(Ys)
+    t_collect = time.perf_counter() - t0
+    say(f"region DA: {train_windows} training windows ({X.shape[0]} samples) in {t_collect:.1f} s")
+    ws = WindowSurrogate(R, hidden=hidden, seed=seed, per_voxel=False, max_batch=256, members=E if member_ids else 0)
+    ws.add(X, Y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = ws.fit(epochs=epochs)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    say(f"region DA: surrogate fitted in {t_fit:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    # the simulator's own floor: the judge and the full ensemble simulate the first members' window
+    # from the same state and hyper-parameters (two noise streams)
+    Xc = window_inputs(da)
+    J = np.arange(subset)
+    tail_j = judge.simulate(da._hp_log.reshape(E, -1)[J], Xc.reshape(E, -1, 9)[J][:, :, 4:8], steps)
+    da.push_hp()
+    da.get_hidden_state(steps)
+    tail_m = da._hidden_state[:, :, -6:].double().cpu()
+    pred_c = ws.predict(Xc)[:, :, -1].cpu().numpy()
+    floor = nrmse(tail_j[:, :, -1].numpy(), tail_m[J][:, :, -1].numpy())
+    calib = dict(sim_vs_sim=floor, sur_vs_full=nrmse(pred_c[J], tail_m[J][:, :, -1].numpy()),
+                 sur_vs_judge=nrmse(pred_c[J], tail_j[:, :, -1].numpy()),
+                 sur_vs_full_all_members=nrmse(pred_c, tail_m[:, :, -1].numpy()))
+    if nrmse_max is None:
+        nrmse_max = max(0.02, 1.25 * floor)
+    say(f"region DA: NRMSE floor (simulation against simulation) {floor:.4f}, surrogate {calib}; threshold "
+        f"{nrmse_max:.4f}")
+    # both runs start from this state and these hyper-parameters
+    hp = da.hp_random_initialize(low, high, num_da_populations_pblk=R, rng=np.random.default_rng(seed + 1))
+    da._push(hp)
+    da._hidden_state = None
+    snap = _snapshot(da)
+    obs = []
+
+    def observe(t):
+        y = nb.bold_state("bold")[truth_vox].copy()
+        obs.append(y)
+        return torch.from_numpy(y)
+
+    da.timings = []
+    w_sim, _, t_sim = da.da_rest_run(observe, "/tmp", observation_times=windows, steps=steps, save=False)
+    obs = np.stack(obs)
+    say(f"region DA: {windows} simulator-only windows, {t_sim.mean():.3f} s each")
+    _restore(da, snap)
+    del snap
+    sda = SubsetSurrogateDA(da, ws, judge, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs)
+    res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
+    kinds = res["kinds"]
+    say(f"region DA: {windows} surrogate-assisted windows, {res['times'].mean():.3f} s each, "
+        f"{kinds.count('judge')} judged, {kinds.count('full')} full")
+    rmse = lambda w: np.sqrt(((w[:, :, :, -1].mean(1) - obs) ** 2).mean(1))
+    errs = [e for _, _, e in res["nrmse"]]
+    out = dict(members=E, regions=R, member_neurons=member_n, windows=windows, steps_per_window=steps,
+               nrmse_threshold=nrmse_max, calibration={k: round(v, 5) for k, v in calib.items()},
+               neurons=int(nb.n), synapses=int(nb.nnz), judge_members=subset, judge_every=every,
+               build_s=t_build, train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect,
+               fit_s=t_fit, fit_epochs=epochs, fit_loss_first=hist[0], fit_loss_last=hist[-1],
+               surrogate=f"per-member MLP {R * 9}{f' + {E} member one-hot' if member_ids else ''} -> " +
+               "x".join(str(h) for h in hidden) + f" -> {R * 6}",
+               sim_window_s_mean=float(t_sim.mean()), sur_window_s_mean=float(res["times"].mean()),
+               gain=float(t_sim.sum() / res["times"].sum()), kinds=kinds,
+               nrmse=[(t, k, round(e, 5)) for t, k, e in res["nrmse"]],
+               nrmse_median=float(np.median(errs)) if errs else None, nrmse_max=float(np.max(errs)) if errs else None,
+               bold_rmse_sim_last=float(rmse(w_sim)[-1]), bold_rmse_sur_last=float(rmse(res["forecasts"])[-1]),
+               bold_rmse_sim_mean=float(rmse(w_sim).mean()), bold_rmse_sur_mean=float(rmse(res["forecasts"]).mean()))
+    say(f"region DA: {out}")
+    return out, dict(w_sim=w_sim, sur=res, obs=obs, hist=hist, truth_hp=truth_hp, da=da, judge=judge)

@@ -79,28 +79,43 @@ class WindowSurrogate:
     member), and each window gives E*V rows.  per_voxel=False: one row per
     member, all V voxels at once (V*NF inputs, V*6 outputs)."""
 
-    def __init__(self, voxels, hidden=(512, 512, 512), max_batch=None, lr=1e-3, seed=0, device=0, per_voxel=True):
+    def __init__(self, voxels, hidden=(512, 512, 512), max_batch=None, lr=1e-3, seed=0, device=0, per_voxel=True,
+                 members=0, voxel_ids=False):
         self.V = int(voxels)
         self.per_voxel = bool(per_voxel)
-        n_in, n_out = (2 * NF, 6) if self.per_voxel else (self.V * NF, self.V * 6)
+        # members > 0 (per-member form): a one-hot of the member is part of the input, so the
+        # network can learn what is particular to each member's own synapses and cells
+        self.members = int(members) if not self.per_voxel else 0
+        # voxel_ids (per-voxel form): a one-hot of the voxel, for voxels that are not
+        # exchangeable (regions of different size and connectivity)
+        self.voxel_ids = bool(voxel_ids) and self.per_voxel
+        n_in, n_out = ((2 * NF + (self.V if self.voxel_ids else 0), 6) if self.per_voxel
+                       else (self.V * NF + self.members, self.V * 6))
         mb = max_batch or (1024 if self.per_voxel else 256)
         self.mlp = MLPSurrogate(n_in, n_out, hidden=hidden, max_batch=mb, lr=lr, seed=seed, device=device)
         self.X = []
         self.Y = []
 
-    def _rows(self, X):
+    def _rows(self, X, idx=None):
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
         if not self.per_voxel:
+            if self.members:
+                idx = torch.arange(X.shape[0]) % self.members if idx is None else torch.as_tensor(idx)
+                X = torch.cat((X, torch.nn.functional.one_hot(idx.long(), self.members).double()), dim=1)
             return X
         x = X.reshape(X.shape[0], self.V, NF)
         m = x.mean(dim=1, keepdim=True).expand(-1, self.V, -1)
-        return torch.cat((x, m), dim=2).reshape(-1, 2 * NF)
+        parts = [x, m]
+        if self.voxel_ids:
+            parts.append(torch.eye(self.V, dtype=torch.float64).expand(X.shape[0], -1, -1))
+        return torch.cat(parts, dim=2).reshape(X.shape[0] * self.V, -1)
 
-    def add(self, X, Y):
+    def add(self, X, Y, idx=None):
+        """Rows of X belong to members idx (default: row r is member r mod members)."""
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
         Y = torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], -1).cpu()
         d = Y - window_base(X, self.V)
-        self.X.append(self._rows(X))
+        self.X.append(self._rows(X, idx))
         self.Y.append(d.reshape(-1, 6) if self.per_voxel else d)
 
     def fit(self, epochs=None, batch=None, renormalize=True):
@@ -126,9 +141,9 @@ class WindowSurrogate:
             Yn = torch.cat((Yn, Yo.index_select(0, idx)))
         return self.mlp.fit(Xn, Yn, epochs=epochs, seed=seed)
 
-    def predict(self, X):
+    def predict(self, X, idx=None):
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
-        d = self.mlp.predict(self._rows(X)).double().cpu().reshape(X.shape[0], -1)
+        d = self.mlp.predict(self._rows(X, idx)).double().cpu().reshape(X.shape[0], -1)
         return (d + window_base(X, self.V)).reshape(X.shape[0], self.V, 6).to(self.mlp.dev)
 
 
@@ -172,7 +187,7 @@ class SurrogateDA:
             simulate = force or (t % self.every == 0)
             pred = self.sur.predict(X)
             if simulate:
-                da.mul_property_by_subblk(da._hp_index_updating, da._hp.float().reshape(-1))
+                da.push_hp()
                 da.get_hidden_state(steps)
                 tail = da._hidden_state[:, :, -6:].double().cpu()
                 e = nrmse(pred[:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
@@ -211,7 +226,7 @@ def collect_windows(da, windows, steps, logit_range=10.0, rng=None, walk_sigma=N
         da._hp_log = torch.from_numpy(hl).float()
         da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
         X = window_inputs(da)
-        da.mul_property_by_subblk(da._hp_index_updating, da._hp.float().reshape(-1))
+        da.push_hp()
         da.get_hidden_state(steps)
         Xs.append(X)
         Ys.append(da._hidden_state[:, :, -6:].double().cpu().reshape(E, -1))
