@@ -19,12 +19,14 @@ block's own stream; they give the per-kernel times of `roofline` and
 
 Beside the main line's fields: `plain` (the 10,000-neuron plain block,
 in-degree 100, 8,000 steps after 100 warm-up steps and delivery tuning,
-which picks the persistent launch for this block -- the workload the CPU
-comparison is quoted on), `cpu_baseline` (the oracle's FP32 restatement of
+which also tries the persistent launch -- the workload the CPU comparison
+is quoted on), `cpu_baseline` (the oracle's FP32 restatement of
 the reference algorithm -- a full pass over every synapse per step, as
 torch.sparse.mm in models/block.py:76-81 -- with OpenMP on the host cores,
-on those same plain-block inputs), `da` (surrogate-assisted against
-simulator-only DA windows, with the surrogate's matrix-core GEMM throughput)
+on those same plain-block inputs), `da` (DA window seconds: the
+64-member batched ensemble with the simulator; surrogate-assisted against
+simulator-only windows on 256 whole-brain members; the surrogate's
+matrix-core GEMM throughput)
 and `regions` (the 90-region, 2e7-neuron share of a whole-brain model, timed
 like the main line).  --da-windows 0, --region-steps 0, --plain-steps 0 and
 --no-cpu skip them.
@@ -275,7 +277,9 @@ def main():
                     help="steps of the region workload reported beside the main one (0: skip)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--da-windows", type=int, default=20)
-    ap.add_argument("--train-windows", type=int, default=24, help="surrogate training windows (x64 samples)")
+    ap.add_argument("--train-windows", type=int, default=196,
+                    help="surrogate training windows (x256 members: 196 gives 50,176 training windows)")
+    ap.add_argument("--member-neurons", type=int, default=10_000, help="neurons per whole-brain DA member")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps directly (for counter runs)")
     ap.add_argument("--launch", default="", metavar="MODE,GRID,SEGS[,EPT[,VARIANT]]",
                     help="fixed delivery launch shape instead of the autotuner (e.g. 1,1024,128,32,0)")
@@ -399,18 +403,33 @@ def main():
 
     da = None
     if a.da_windows > 0 and a.workload == "large":
-        from digital_twin_brain_amd.da_runner import run_surrogate_da
+        from digital_twin_brain_amd.da_runner import run_ensemble_da
+        from digital_twin_brain_amd.region_da import run_region_surrogate_da
 
+        # the batched ensemble: 64 uneven members, simulator-only windows
         t0 = time.perf_counter()
-        c5, rec = run_surrogate_da(E=64, voxels=90, train_windows=a.train_windows, windows=a.da_windows,
-                                   window_ms=800.0, dt=0.1, log=log)
+        c4, da4, _ = run_ensemble_da(E=64, voxels=90, windows=a.da_windows, window_ms=800.0, dt=0.1, log=log)
+        da4.block_model.nb.close()
+        del da4
+        torch.cuda.empty_cache()
+        c4["total_s"] = time.perf_counter() - t0
+        # surrogate-assisted against simulator-only windows on 256 whole-brain members
+        t0 = time.perf_counter()
+        c5, rec = run_region_surrogate_da(E=256, R=90, member_n=a.member_neurons, train_windows=a.train_windows,
+                                          windows=a.da_windows, window_ms=800.0, dt=0.1, subset=16, every=5,
+                                          log=log)
         rec["da"].block_model.nb.close()
+        rec["judge"].nb.close()
         da = {"metric": "DA window seconds (surrogate vs full sim)", "unit": "s", "higher_is_better": False,
               "full_sim_window_s": c5["sim_window_s_mean"], "surrogate_window_s": c5["sur_window_s_mean"],
-              "gain": c5["gain"],
-              "config": "64 uneven members (2k-200k neurons, in-degree 100-500) + truth, 90 voxels, 800 ms windows, "
-                        "EnKF FP64",
-              "detail": c5, "total_s": time.perf_counter() - t0}
+              "gain": c5["gain"], "nrmse_median": c5["nrmse_median"], "nrmse_max": c5["nrmse_max"],
+              "config": (f"256 whole-brain members (90 regions, {a.member_neurons} neurons, in-degree 100, delays "
+                         f"1-40) + truth, 4 g_ui per region, 800 ms windows, EnKF FP64, surrogate trained on "
+                         f"{c5['train_samples']} simulated windows, judge: 16 members every 5th window"),
+              "detail": c5, "total_s": time.perf_counter() - t0,
+              "batched_ensemble": {"config": "64 uneven members (2k-200k neurons, in-degree 100-500) + truth, 90 "
+                                             "voxels, 800 ms windows, EnKF FP64, simulator only",
+                                   "window_s": c4["window_s_mean"], "detail": c4}}
         del rec
         # the surrogate's matrix-core product, live: one layer of the MLP and a large square product
         from digital_twin_brain_amd.surrogate import gemm_kernel_name, gemm_throughput

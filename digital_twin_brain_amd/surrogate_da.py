@@ -51,12 +51,15 @@ def window_inputs(da):
 _V0, _E0 = 0.02, 0.8
 
 
-def window_base(X, V):
+def window_base(X, V, use_act=True):
     """What a window would end at if nothing changed: per voxel (the previous
-    act, the start s, q, v, f_in, and the BOLD signal of the start q, v as
-    models/bold_model_pytorch.py:76-77 computes it).  X: [E, V*NF]."""
+    act (0 without use_act), the start s, q, v, f_in, and the BOLD signal of
+    the start q, v as models/bold_model_pytorch.py:76-77 computes it).
+    X: [E, V*NF]."""
     x = torch.as_tensor(X, dtype=torch.float64).reshape(X.shape[0], V, NF)
     s, q, v, f, act = x[..., 4], x[..., 5], x[..., 6], x[..., 7], x[..., 8]
+    if not use_act:
+        act = torch.zeros_like(act)
     bold = _V0 * (7 * _E0 * (1 - q) + 2 * (1 - q / v) + (2 * _E0 - 0.2) * (1 - v))
     return torch.stack((act, s, q, v, f, bold), dim=-1).reshape(X.shape[0], V * 6)
 
@@ -80,17 +83,22 @@ class WindowSurrogate:
     member, all V voxels at once (V*NF inputs, V*6 outputs)."""
 
     def __init__(self, voxels, hidden=(512, 512, 512), max_batch=None, lr=1e-3, seed=0, device=0, per_voxel=True,
-                 members=0, voxel_ids=False):
+                 members=0, voxel_ids=False, use_act=True):
         self.V = int(voxels)
         self.per_voxel = bool(per_voxel)
+        # use_act=False: the previous window's rate is left out of the input (the
+        # DA's act is the last step's rate, mostly noise; after a surrogate window
+        # it would be the surrogate's own smooth prediction instead)
+        self.use_act = bool(use_act)
+        self.nf = NF if self.use_act else NF - 1
         # members > 0 (per-member form): a one-hot of the member is part of the input, so the
         # network can learn what is particular to each member's own synapses and cells
         self.members = int(members) if not self.per_voxel else 0
         # voxel_ids (per-voxel form): a one-hot of the voxel, for voxels that are not
         # exchangeable (regions of different size and connectivity)
         self.voxel_ids = bool(voxel_ids) and self.per_voxel
-        n_in, n_out = ((2 * NF + (self.V if self.voxel_ids else 0), 6) if self.per_voxel
-                       else (self.V * NF + self.members, self.V * 6))
+        n_in, n_out = ((2 * self.nf + (self.V if self.voxel_ids else 0), 6) if self.per_voxel
+                       else (self.V * self.nf + self.members, self.V * 6))
         mb = max_batch or (1024 if self.per_voxel else 256)
         self.mlp = MLPSurrogate(n_in, n_out, hidden=hidden, max_batch=mb, lr=lr, seed=seed, device=device)
         self.X = []
@@ -98,12 +106,14 @@ class WindowSurrogate:
 
     def _rows(self, X, idx=None):
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
+        if not self.use_act:
+            X = X.reshape(X.shape[0], self.V, NF)[:, :, :NF - 1].reshape(X.shape[0], -1)
         if not self.per_voxel:
             if self.members:
                 idx = torch.arange(X.shape[0]) % self.members if idx is None else torch.as_tensor(idx)
                 X = torch.cat((X, torch.nn.functional.one_hot(idx.long(), self.members).double()), dim=1)
             return X
-        x = X.reshape(X.shape[0], self.V, NF)
+        x = X.reshape(X.shape[0], self.V, self.nf)
         m = x.mean(dim=1, keepdim=True).expand(-1, self.V, -1)
         parts = [x, m]
         if self.voxel_ids:
@@ -114,7 +124,7 @@ class WindowSurrogate:
         """Rows of X belong to members idx (default: row r is member r mod members)."""
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
         Y = torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], -1).cpu()
-        d = Y - window_base(X, self.V)
+        d = Y - window_base(X, self.V, self.use_act)
         self.X.append(self._rows(X, idx))
         self.Y.append(d.reshape(-1, 6) if self.per_voxel else d)
 
@@ -141,10 +151,19 @@ class WindowSurrogate:
             Yn = torch.cat((Yn, Yo.index_select(0, idx)))
         return self.mlp.fit(Xn, Yn, epochs=epochs, seed=seed)
 
+    def residual_std(self, X, Y):
+        """Per voxel and output the standard deviation of Y - predict(X) [V, 6]
+        over the rows given: the spread of the simulator's windows around the
+        surrogate's mean."""
+        X = torch.as_tensor(X, dtype=torch.float64)
+        Y = torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], self.V, 6)
+        P = self.predict(X).double().cpu()
+        return (Y - P).std(dim=0)
+
     def predict(self, X, idx=None):
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
         d = self.mlp.predict(self._rows(X, idx)).double().cpu().reshape(X.shape[0], -1)
-        return (d + window_base(X, self.V)).reshape(X.shape[0], self.V, 6).to(self.mlp.dev)
+        return (d + window_base(X, self.V, self.use_act)).reshape(X.shape[0], self.V, 6).to(self.mlp.dev)
 
 
 class SurrogateDA:

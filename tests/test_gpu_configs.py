@@ -221,3 +221,74 @@ def test_ensemble_closed_window_arrivals_per_member():
     assert bool((err <= tol).all()), f"member arrivals differ: worst {float((err / tol).max()):.3g} x tolerance"
     assert bool((want[:, 0] > 0).all()), "some member had no excitatory noise spike"
     nb.close()
+
+
+def test_ensemble_window_enkf_at_size():
+    """One real DA window of the 64-member ensemble (3.7e6 neurons, 90 voxels
+    per member, 800 ms): the hidden state [64, 90, 10] and the FP64 EnKF
+    analysis against the oracle's diffusion_enkf on the same hidden state and
+    the same observation noise."""
+    from digital_twin_brain_amd import networks
+    from digital_twin_brain_amd.data_assimilation import DataAssimilation
+
+    E, V = 64, 90
+    nb, props, members = networks.build_ensemble(E=E, voxels=V)
+    da = DataAssimilation(network=nb, column=False, ensemble=E, dt=0.1, bold_steps=10, da_populations=E * 2 * V,
+                          da_voxels=E * V, write_path="/tmp")
+    low, high = torch.full((2 * V, 2), 0.5), torch.full((2 * V, 2), 2.0)
+    hp = da.hp_random_initialize(low, high, rng=np.random.default_rng(0))
+    da.hp_index2hidden_state()
+    da.da_property_initialize([10, 12], None, hp)
+    da.da_evolve(8000)
+    hs = da._hidden_state.cpu().numpy()
+    assert hs.shape == (E, V, 2 * 2 + 6)
+    assert np.isfinite(hs).all() and hs[:, :, -1].std() > 0
+    truth = nb.bold_state("bold")[E * V:(E + 1) * V]
+    xi = np.random.default_rng(4).normal(size=(E, V))
+    da.da_filter(torch.from_numpy(truth.reshape(1, -1)), bold_sigma=1e-8, solo_rate=0.8, xi=torch.from_numpy(xi))
+    want = orc.diffusion_enkf(hs, 1e-8, truth.reshape(1, -1), 0.8, xi)
+    np.testing.assert_allclose(da._hidden_state.cpu().numpy(), want, rtol=1e-9, atol=1e-12)
+    nb.close()
+
+
+def test_whole_brain_ensemble_structure_at_size():
+    """256 whole-brain members of 10,000 neurons (90 regions, lognormal
+    sizes, 80 intra-region + 20 inter-region inputs through one connectome,
+    delays 1-40) + truth: every neuron's in-degree, no synapse leaves its
+    replica, intra-region inputs come from the own region and inter-region
+    inputs from others, members are replicas of one region layout; one
+    800 ms window of the assimilation with the 4 g_ui per region gives the
+    hidden state [256, 90, 10]."""
+    from digital_twin_brain_amd import networks
+    from digital_twin_brain_amd.region_da import make_region_da
+
+    E, R, mn = 256, 90, 10_000
+    nb, da, truth_hp, _ = make_region_da(E, R, mn, tune=False)
+    dev = torch.device("cuda", 0)
+    n, nnz = nb.n, nb.nnz
+    assert n == (E + 1) * mn and nnz == n * 100
+    sizes = networks.region_sizes(R, mn, min_size=40)
+    reg_off = np.concatenate([[0], np.cumsum(sizes)])
+    rowptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    post = torch.empty(nnz, dtype=torch.int32, device=dev)
+    nb.get("rowptr", out=rowptr)
+    nb.get("post", out=post)
+    pre = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int64), rowptr.diff())
+    post = post.to(torch.int64)
+    assert bool((pre // mn == post // mn).all()), "a synapse leaves its replica"
+    ro = torch.from_numpy(reg_off).to(dev)
+    rpre = torch.searchsorted(ro, pre % mn, right=True) - 1
+    rpost = torch.searchsorted(ro, post % mn, right=True) - 1
+    del pre
+    same = (rpre == rpost).to(torch.int64)
+    intra = torch.zeros(n, dtype=torch.int64, device=dev).index_add_(0, post, same)
+    indeg = torch.bincount(post, minlength=n)
+    assert bool((indeg == 100).all())
+    # 80 drawn inside the own region; the connectome has no self-loops, so the other 20 come from elsewhere
+    assert bool((intra == 80).all()), "intra-region inputs differ from 80"
+    del post, rpre, rpost, same
+    da.da_evolve(8000)
+    hs = da._hidden_state
+    assert tuple(hs.shape) == (E, R, 4 + 6)
+    assert bool(torch.isfinite(hs).all())
+    nb.close()

@@ -234,3 +234,38 @@ def test_surrogate_da_small(adapt):
     if (~sim).any():
         assert times[~sim].mean() < times[sim].mean()
     assert out["fit_loss_last"] < out["fit_loss_first"]
+
+
+def test_region_surrogate_da_small():
+    """Surrogate-assisted DA on whole-brain members (4 g_ui per region, a
+    judge simulating copies of the first members every 5th window, full
+    simulation after a judged window over the NRMSE threshold) against
+    simulator-only DA from the same state, on a small configuration: the
+    judge's floor (simulation against simulation) is measured; the
+    surrogate's error on that window is a few percent; surrogate windows cost
+    a fraction of simulated ones; the fallback pattern follows the policy.
+    The full configuration (256 members of 10,000 neurons, 50,176 training
+    windows, 800 ms windows) runs in bench.py."""
+    from digital_twin_brain_amd.region_da import run_region_surrogate_da
+
+    out, rec = run_region_surrogate_da(E=32, R=20, member_n=4000, train_windows=32, windows=10, window_ms=200.0,
+                                       subset=4, every=5, seed=3)
+    cal = out["calibration"]
+    # 200 ms windows: the BOLD state at the window end is mostly the start state's, so two
+    # simulations agree closely; the surrogate (1,024 training windows) is off by a few percent
+    assert 0 < cal["sim_vs_sim"] < 0.2
+    assert cal["sur_vs_full"] < 0.05, cal
+    assert out["nrmse_threshold"] >= 0.02
+    kinds = out["kinds"]
+    assert kinds[0] == "judge"
+    for t, k in enumerate(kinds):
+        prev = [e for (tt, kk, e) in out["nrmse"] if tt == t - 1]
+        if prev and prev[0] > out["nrmse_threshold"]:
+            assert k == "full", (t, kinds)
+        elif k != "full":
+            assert k == ("judge" if t % 5 == 0 else "surrogate"), (t, kinds)
+    if "surrogate" in out["window_s_by_kind"]:
+        assert out["window_s_by_kind"]["surrogate"] < 0.5 * out["sim_window_s_mean"]
+    assert out["train_samples"] == 32 * 32
+    rec["da"].block_model.nb.close()
+    rec["judge"].nb.close()
