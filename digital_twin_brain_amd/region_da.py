@@ -95,7 +95,7 @@ class SubsetJudge:
     same seeds give the same members), with another noise stream; simulates
     their windows from given hyper-parameters and window-start BOLD state."""
 
-    def __init__(self, subset, R, member_n, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=8, pre_ms=100.0, noise_seed=103,
+    def __init__(self, subset, R, member_n, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=8, pre_ms=50.0, noise_seed=103,
                  **kw):
         self.S = int(subset)
         self.nb, self.da, _, _ = make_region_da(self.S, R, member_n, dt=dt, hp_lo=hp_lo, hp_hi=hp_hi, seed=seed,
@@ -186,7 +186,8 @@ class SubsetSurrogateDA:
                 errs.append((t, kind, e))
                 full_next = e > self.nrmse_max
                 if self.finetune_epochs:
-                    self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t)
+                    # a full window adds E x R rows: a sample of them keeps the update cheap
+                    self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t, max_rows=4 * 1024)
             forecasts.append(da._hidden_state.cpu().numpy())
             kinds.append(kind)
             y = observe(t)
@@ -201,17 +202,23 @@ def collect_da_windows(da, windows, steps, rng, low, high, R, segment=8, hp_sigm
     """Training windows drawn the way the assimilation moves: segments of
     `segment` windows that start from the reference's logit grid, take the
     assimilation's random-walk step and then an EnKF analysis each window,
-    against a pseudo-observation -- the simulated BOLD of one member picked at
-    random per segment (never the truth), so that the ensemble's
-    hyper-parameters and BOLD states wander as they do under assimilation.
+    against a pseudo-observation: the simulated BOLD of one member picked at
+    random per segment, whose hyper-parameters are drawn uniformly from the
+    prior range and held fixed (a stand-in truth, never the real one), so
+    that the ensemble's hyper-parameters and BOLD states travel as they do
+    when assimilating an observation from outside the ensemble.
     Returns (X [windows*E, 9R], Y [windows*E, 6R])."""
     E = da.ensemble_number
     Xs, Ys = [], []
     for w0 in range(0, windows, segment):
         da.hp_random_initialize(low, high, num_da_populations_pblk=R, rng=np.random.default_rng(seed + w0))
         target = int(rng.integers(E))
+        lo, hi = da._hp_low.numpy(), da._hp_high.numpy()
+        u = rng.uniform(0.02, 0.98, size=lo.shape)
+        fixed = torch.from_numpy(10.0 * np.log(u / (1.0 - u))).float()  # logit of lo + (hi - lo) u
         for _ in range(min(segment, windows - w0)):
             da._hp_log += torch.from_numpy(rng.normal(0.0, hp_sigma ** 0.5, size=tuple(da._hp_log.shape))).float()
+            da._hp_log[target] = fixed
             da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
             X = window_inputs(da)
             da.push_hp()
@@ -250,7 +257,7 @@ def _restore(da, s):
 
 def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, windows=20, window_ms=800.0, dt=0.1,
                             subset=16, every=5, nrmse_max=None, hidden=(512, 512, 512), epochs=None,
-                            finetune_epochs=None, seed=7, pre_ms=100.0, per_voxel=True, sample_residual=False, log=None,
+                            finetune_epochs=None, seed=7, pre_ms=50.0, per_voxel=True, sample_residual=False, log=None,
                             tune=True, **kw):
     """Simulator-only DA against surrogate-assisted DA (subset judge) on E
     whole-brain members, from the same state and hyper-parameters, against

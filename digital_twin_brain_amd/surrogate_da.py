@@ -137,11 +137,15 @@ class WindowSurrogate:
             epochs = 20 if self.per_voxel else 400
         return self.mlp.fit(X, Y, epochs=epochs, batch=batch)
 
-    def finetune(self, epochs, replay=1.0, seed=0):
-        """Train on the last added window plus `replay` times as many rows drawn
-        at random from the earlier ones (so the fit does not drift towards the
-        latest window, at a small fraction of a pass over everything)."""
+    def finetune(self, epochs, replay=1.0, seed=0, max_rows=None):
+        """Train on the last added window (at most max_rows of its rows, drawn
+        at random) plus `replay` times as many rows drawn at random from the
+        earlier ones (so the fit does not drift towards the latest window, at
+        a small fraction of a pass over everything)."""
         Xn, Yn = self.X[-1], self.Y[-1]
+        if max_rows is not None and Xn.shape[0] > max_rows:
+            pick = torch.randperm(Xn.shape[0], generator=torch.Generator().manual_seed(seed + 1))[:max_rows]
+            Xn, Yn = Xn.index_select(0, pick), Yn.index_select(0, pick)
         if len(self.X) > 1 and replay > 0:
             Xo = torch.cat(self.X[:-1])
             Yo = torch.cat(self.Y[:-1])
