@@ -262,8 +262,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=("large", "regions"), default="large",
-                    help="large: one 5e6-neuron block; regions: 90 regions, 2e7 neurons, connectome, delays 1-40")
+    ap.add_argument("--workload", choices=("large", "regions", "plain"), default="large",
+                    help="large: one 5e6-neuron block; regions: 90 regions, 2e7 neurons, connectome, delays 1-40; "
+                         "plain: only the 10,000-neuron plain block line (for profiling)")
     ap.add_argument("--n", type=int, default=5_000_000)
     ap.add_argument("--in-degree", type=int, default=1000)
     ap.add_argument("--kernel-window", type=int, default=200,
@@ -312,6 +313,12 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
                           "launches": 5}), flush=True)
+        return
+    if a.workload == "plain":
+        out = plain_line(a.plain_steps, 100, log)
+        out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+                    "data": "synthetic"})
+        print(json.dumps(out), flush=True)
         return
     graph = not a.no_graph
     t0 = time.perf_counter()
