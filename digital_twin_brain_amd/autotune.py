@@ -91,6 +91,8 @@ def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
                       deliver_grid=best["deliver_grid"], stage_segs=best.get("stage_segs", 256),
                       stage_ept=best.get("stage_ept", 32), stage_variant=best.get("stage_variant", 0),
                       accum_occ=best.get("accum_occ", 1))
+        if "segment_lists" in best:
+            nb.set_launch(segment_lists=best["segment_lists"])
         if best["deliver_mode"] == 2:
             nb.set_launch(deliver_mode=2)
         return best, {"cached": ckey}
@@ -145,6 +147,21 @@ def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
             "stage_variant": var, "accum_occ": occ, "us_per_1e6_events": refined[bkey]}
     nb.set_launch(deliver_mode=mode, deliver_lanes=lanes, deliver_grid=grid, stage_segs=sg, stage_ept=ept,
                   accum_occ=occ, stage_variant=var)
+    # bucketed delivery with delays: segment lists on or off, by whole-step
+    # wall time (the emission kernel's cost shows only there)
+    if mode == 1 and nb.max_delay > 1:
+        cur = nb.get_launch()["segment_lists"]
+        walls = {}
+        for on in (cur > 0, not cur > 0):
+            try:
+                nb.set_launch(segment_lists=int(on))
+            except nat_error():
+                continue
+            walls[int(on)] = _step_us(nb, run_kwargs, 10 * steps)
+        seg = min(walls, key=walls.get)
+        nb.set_launch(segment_lists=seg)
+        best["segment_lists"] = seg
+        table.update({f"segment lists {k} step us": v for k, v in walls.items()})
     # small blocks: the persistent launch (one launch per batch, a grid barrier
     # per step) has no per-kernel times; it is compared with the best shape
     # above by whole-step wall time over step batches

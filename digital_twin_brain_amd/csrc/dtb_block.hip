@@ -82,6 +82,34 @@ __device__ inline float wave_sum(float v)
 constexpr int kSubLists = 32;
 constexpr int kCntStride = 32;
 constexpr int kMaxEntries = DTB_MAX_DELAY * kSubLists; // (delay, sub-list) entries of a delivery
+constexpr int kCursorStride = 32;    // one 128-byte line per bucket cursor / overflow flag / list count
+
+// Segment lists (bucketed delivery with delays, D > 1).  A spike of step s
+// is delivered over D steps, one (row, delay) segment per step, and reading a
+// segment's bounds from rowptr and dend at each of those steps costs two
+// random cache lines per segment: for short segments (in-degree / D of a
+// dozen synapses or less) several times the synapse data itself.  Instead,
+// right after the neuron kernel, segment_emit_kernel reads each spike's row
+// start and segment ends once and writes its non-empty segments, packed as
+// (row start + offset) << 24 | length, into the list of the step that
+// delivers them: segment d of a spike of step s goes to step s + d - 1, list
+// (s + d - 1) mod D, reserved by one returning atomic per wavefront and
+// delay.  The stage kernel of step s then reads list s mod D with coalesced
+// 8-byte loads, and the accumulation kernel clears its count.  A list over
+// its capacity raises its overflow flag, and that step reads the spike lists
+// as before; so do the first D - 1 steps after the lists were (re)started,
+// whose lists lack the segments of spikes before the restart.
+struct SegArgs {
+    uint64_t* seg; // [D][cap] (NULL: segment lists off)
+    uint32_t* cnt; // [D + 1][kCursorStride]: entries, overflow flag; entry D: first complete step (int64)
+    int64_t cap;   // entries per list
+};
+
+__device__ inline int64_t seg_from(const SegArgs& g, int32_t D)
+{
+    return *reinterpret_cast<const int64_t*>(g.cnt + (int64_t)D * kCursorStride);
+}
+
 
 __host__ __device__ inline int64_t spk_cap_of(int64_t n)
 {
@@ -559,7 +587,6 @@ struct BucketCfg {
     static constexpr int kChan = PAIR ? 4 : 2; // floats per neuron in acc
 };
 constexpr int kMaxBucketBits = 15;   // neurons per bucket < 2^15 (local index in staged words)
-constexpr int kCursorStride = 32;    // one 128-byte line per bucket cursor / overflow flag
 constexpr int kStageParts = 8;       // staging partitions per bucket: workgroup w reserves in part
                                      // w % kStageParts, so each cursor takes 1/8 of the (serialising)
                                      // same-address reservation atomics
@@ -580,6 +607,8 @@ struct StageArgs {
     void* stage;      // [nbk][kStageParts][cap] uint2 (compact) / uint4 (pair)
     void* acc2;       // overflow accumulator [n]
     uint32_t* ovf;    // [nbk] overflow flags
+    SegArgs sg;       // segment lists
+    int32_t k;        // batch step (the accumulation kernel clears the step's segment list)
 };
 
 
@@ -698,9 +727,32 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
     const int t = threadIdx.x;
     const int32_t part = (int32_t)(blockIdx.x % kStageParts);
     for (int b = t; b < s.nbk; b += 256) hist[b] = 0;
-    const int32_t NE = delivery_entries(a, k, cum, wtot);
-    const uint32_t total = cum[NE];
-    if (blockIdx.x == 0 && t == 0 && a.rec_nspk) a.rec_nspk[k] = cum[kSubLists];
+    // this step's segments: its segment list when complete, else the spike
+    // lists of the last D steps (the same for every workgroup)
+    bool use_seg = false;
+    uint32_t total = 0;
+    int64_t seg0 = 0;
+    if (s.sg.seg) {
+        const int64_t step = a.ctl->nbase + k + 1;
+        const int32_t ls = (int32_t)(step % a.D);
+        const uint32_t* c = s.sg.cnt + (int64_t)ls * kCursorStride;
+        use_seg = step >= seg_from(s.sg, a.D) && c[1] == 0u;
+        total = use_seg ? c[0] : 0u; // no overflow flag: every entry is in the list
+        seg0 = (int64_t)ls * s.sg.cap;
+    }
+    int32_t NE = 0;
+    if (!use_seg) {
+        NE = delivery_entries(a, k, cum, wtot);
+        total = cum[NE];
+        if (blockIdx.x == 0 && t == 0 && a.rec_nspk) a.rec_nspk[k] = cum[kSubLists];
+    } else if (blockIdx.x == 0 && t < 64 && a.rec_nspk) {
+        // the step's spikes: the sum of its sub-list counts
+        const int32_t slot = (int32_t)((a.ctl->nbase + k + 1) % a.L);
+        uint32_t c = t < kSubLists ? min(a.spk_cnt[((int64_t)slot * kSubLists + t) * kCntStride], (uint32_t)a.spk_cap)
+                                   : 0u;
+        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+        if (t == 0) a.rec_nspk[k] = c;
+    }
     // The step's segments in kper * gridDim chunks of equal size (at most
     // SEG), dealt round-robin: every workgroup gets kper chunks, and each
     // gets chunks from all delays (segment lengths vary with the delay).
@@ -716,7 +768,11 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
         const uint32_t g = c0 + t;
         uint32_t len = 0;
         int64_t beg = 0;
-        if (t < SEG && g < g_end) {
+        if (t < SEG && g < g_end && use_seg) {
+            const uint64_t v = s.sg.seg[seg0 + g];
+            beg = (int64_t)(v >> 24);
+            len = (uint32_t)(v & 0xffffffu);
+        } else if (t < SEG && g < g_end) {
             const int32_t e = entry_of(cum, NE, g);
             const int32_t d = e / kSubLists + 1;
             const int32_t j = a.spk[entry_base(a, s0, e) + (g - cum[e])];
@@ -931,6 +987,106 @@ __global__ __launch_bounds__(256, EPT <= 16 ? 4 : 2) void deliver_stage_kernel(S
         }
     }
     if (t == 0 && ev_wg && a.rec_events) atomicAdd((unsigned long long*)&a.rec_events[k], (unsigned long long)ev_wg);
+}
+
+// Appends every spike of the step's non-empty segments to the segment lists
+// of the steps that deliver them (D > 1), right after the neuron kernel.
+// Wavefront-uniform loop: lane l of a wavefront takes spike w0 + l of the
+// step's sub-lists in order.  Lane d - 1 keeps delay d's ballot of non-empty
+// segments and reserves that delay's entries, so the D reservation atomics
+// of a wavefront are in flight together; then every lane writes its entries.
+__global__ __launch_bounds__(256) void segment_emit_kernel(DeliverArgs a, SegArgs sg, int32_t k)
+{
+    __shared__ uint32_t cum[kSubLists + 1];
+    const int64_t step = a.ctl->nbase + k + 1;
+    const int32_t slot = (int32_t)(step % a.L);
+    const int t = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    if (t < 64) {
+        uint32_t c = t < kSubLists
+                         ? min(a.spk_cnt[((int64_t)slot * kSubLists + t) * kCntStride], (uint32_t)a.spk_cap)
+                         : 0u;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(c, o);
+            if ((int)lane >= o) c += y;
+        }
+        if (t < kSubLists) cum[t + 1] = c;
+        if (t == 0) cum[0] = 0u;
+    }
+    __syncthreads();
+    const uint32_t total = cum[kSubLists];
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t w0 = blockIdx.x * 256u + ((uint32_t)t & ~63u); w0 < total; w0 += gridDim.x * 256u) {
+        const uint32_t g = w0 + lane;
+        const bool ok = g < total;
+        int64_t row = 0;
+        const uint16_t* de = a.dend;
+        if (ok) {
+            int q = 0; // the last sub-list starting at or before g
+#pragma unroll
+            for (int st = kSubLists / 2; st >= 1; st >>= 1) q += cum[q + st] <= g ? st : 0;
+            const int32_t j = a.spk[((int64_t)slot * kSubLists + q) * a.spk_cap + (g - cum[q])];
+            row = a.rowptr[j];
+            de += (int64_t)j * a.D;
+        }
+        // pass 1: lane d - 1 keeps the ballot of delay d (segment ends read
+        // eight at a time, the eight loads in flight together)
+        constexpr int kU = 8;
+        uint64_t mine = 0ull;
+        uint32_t lo = 0u;
+        for (int32_t d0 = 0; d0 < a.D; d0 += kU) {
+            uint32_t hv[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) hv[u] = ok && d0 + u < a.D ? (uint32_t)de[d0 + u] : 0u;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (d0 + u < a.D) { // uniform
+                    const uint64_t m = __ballot(hv[u] > lo);
+                    if ((int32_t)lane == d0 + u) mine = m;
+                    lo = hv[u];
+                }
+            }
+        }
+        uint32_t base = 0u;
+        if ((int32_t)lane < a.D && mine)
+            base = atomicAdd(sg.cnt + ((step + lane) % a.D) * kCursorStride, (uint32_t)__popcll(mine));
+        // pass 2: the entries (the segment ends again, from the cache)
+        lo = 0u;
+        for (int32_t d0 = 0; d0 < a.D; d0 += kU) {
+            uint32_t hv[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) hv[u] = ok && d0 + u < a.D ? (uint32_t)de[d0 + u] : 0u;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int32_t d = d0 + u + 1;
+                if (d > a.D) continue; // uniform
+                const uint32_t hi = hv[u];
+                const uint32_t mlo = __shfl((uint32_t)mine, d - 1);
+                const uint32_t mhi = __shfl((uint32_t)(mine >> 32), d - 1);
+                const uint32_t bd = __shfl(base, d - 1);
+                if (hi > lo) {
+                    const uint64_t m = ((uint64_t)mhi << 32) | mlo;
+                    const uint64_t pos = (uint64_t)bd + (uint64_t)__popcll(m & below);
+                    const int64_t ls = (step + d - 1) % a.D;
+                    if (pos < (uint64_t)sg.cap)
+                        sg.seg[ls * sg.cap + (int64_t)pos] = ((uint64_t)(row + lo) << 24) | (uint64_t)(hi - lo);
+                    else
+                        sg.cnt[ls * kCursorStride + 1] = 1u;
+                }
+                lo = hi;
+            }
+        }
+    }
+}
+
+// (Re)start the segment lists: empty, complete from step `from` on.
+__global__ void segment_reset_kernel(uint32_t* cnt, int32_t D, int64_t from)
+{
+    for (int x = threadIdx.x; x < D; x += blockDim.x) {
+        cnt[(int64_t)x * kCursorStride] = 0u;
+        cnt[(int64_t)x * kCursorStride + 1] = 0u;
+    }
+    if (threadIdx.x == 0) *reinterpret_cast<int64_t*>(cnt + (int64_t)D * kCursorStride) = from;
 }
 
 // dynamic LDS of the stage kernel: the delivery entries and the bucket
@@ -1263,6 +1419,12 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
     // the partition counts and the overflow flag, one lane each (loads in
     // parallel, overlapped with clearing the slice)
     __shared__ uint32_t s_part[kStageParts + 1];
+    if (s.sg.seg && b == 0 && t == 64) {
+        // the stage kernel of this step has read its segment list
+        uint32_t* c = s.sg.cnt + ((a.ctl->nbase + s.k + 1) % a.D) * kCursorStride;
+        c[0] = 0u;
+        c[1] = 0u;
+    }
     if (t < kStageParts)
         s_part[t] = (uint32_t)min((uint64_t)s.cursor[((size_t)b * kStageParts + t) * kCursorStride], (uint64_t)s.cap);
     else if (t == kStageParts)
@@ -2225,6 +2387,25 @@ static int use_bucket_geom(Block* b, int occ)
 // Staging buffers of the bucketed delivery, allocated on first use for the
 // largest geometry (1 or 2 accumulation workgroups per CU) so that the
 // launch option can switch between them.
+// Segment lists (allocated on first use, switched on): room for the segments
+// of a mean rate of 100 Hz at 0.1 ms (1% of the neurons spiking in each of
+// the last D steps); a fuller list sends its step back to the spike lists.
+static int ensure_segments(Block* b)
+{
+    DTB_REQUIRE(b->D > 1 && b->nnz < (1ll << 40),
+                "segment lists need delays (max delay > 1) and fewer than 2^40 synapses");
+    if (!b->seg) {
+        b->seg_alloc = std::max<int64_t>(65536, (int64_t)b->D * (b->n / 100 + 1));
+        DTB_TRY(alloc_n(b, &b->seg, (int64_t)b->D * b->seg_alloc));
+        DTB_TRY(alloc_n(b, &b->seg_cnt, (int64_t)(b->D + 1) * kCursorStride));
+        DTB_TRY_HIP(hipMemsetAsync(b->seg_cnt, 0, 4 * (size_t)(b->D + 1) * kCursorStride, b->stream));
+        DTB_TRY_HIP(hipStreamSynchronize(b->stream));
+    }
+    b->seg_cap = b->seg_alloc;
+    b->seg_dirty = true;
+    return DTB_OK;
+}
+
 static int ensure_bucketed(Block* b)
 {
     if (b->stage) return DTB_OK;
@@ -2271,6 +2452,11 @@ static int ensure_bucketed(Block* b)
     DTB_TRY_HIP(hipMemsetAsync(b->cursor, 0, 4 * (size_t)nbk_max * kStageParts * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->ovf, 0, 4 * (size_t)nbk_max * kCursorStride, b->stream));
     DTB_TRY_HIP(hipMemsetAsync(b->acc2, 0, (b->pair ? 16 : 8) * (size_t)b->n, b->stream));
+    // segment lists by default where their stage-kernel saving (two random
+    // cache lines per delivered segment) outweighs the emission kernel's
+    // fixed cost: many (neuron, delay) pairs and short segments
+    const double pairs = (double)b->n * (double)b->D;
+    if (b->D > 1 && pairs >= 4e8 && (double)b->nnz <= 16.0 * pairs) DTB_TRY(ensure_segments(b));
     b->accum_fixed = b->fixed_ok;
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     return DTB_OK;
@@ -2479,6 +2665,16 @@ static int create_from_csr(Block* b, const dtb_block_desc* d)
 static int ensure_persist(Block* b);
 static bool persist_fits(const Block* b, int32_t* G = nullptr, int32_t* m = nullptr);
 
+static SegArgs seg_args(const Block* b)
+{
+    SegArgs g;
+    const bool on = b->seg && b->seg_cap > 0 && b->D > 1;
+    g.seg = on ? b->seg : nullptr;
+    g.cnt = b->seg_cnt;
+    g.cap = b->seg_cap;
+    return g;
+}
+
 static NeuronArgs neuron_args(Block* b, const dtb_run_args* a, const float* uniforms, bool freq, bool vsum)
 {
     NeuronArgs x;
@@ -2514,6 +2710,7 @@ static DeliverArgs deliver_args(Block* b)
 struct Want {
     bool freq, vsum, isum, sample, bold, timing;
 };
+
 
 template <bool PAIR, bool PSET>
 static void launch_persist_t(const Block* b, const PersistArgs& pa, int mb, size_t lds)
@@ -2606,8 +2803,12 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         }
         if (w.timing) DTB_TRY_HIP(hipEventRecord(b->events[(size_t)(3 * k + 1)], b->stream));
         if (b->deliver_mode >= 1) {
+            const SegArgs sg = seg_args(b);
             StageArgs sa{da, b->nbk, b->bsize, b->bmagic, b->bshift, b->stage_cap, b->cursor, b->stage, b->acc2,
-                         b->ovf};
+                         b->ovf, sg, k};
+            if (sg.seg)
+                hipLaunchKernelGGL(segment_emit_kernel, dim3((unsigned)std::min<int64_t>(1024, (b->n + 255) / 256)),
+                                   dim3(256), 0, b->stream, da, sg, k);
             if (b->pair) {
                 launch_stage<true>(b, sa, k);
                 launch_accum<true>(b, sa);
@@ -2685,6 +2886,16 @@ static int run_impl(Block* b, int64_t nsteps, const dtb_run_args* a)
         if (persist_fits(b)) DTB_TRY(ensure_persist(b));
         else b->deliver_mode = 1;
     }
+    // segment lists: written only by bucketed batches; restarted after any
+    // batch that did not write them (or a state load)
+    const bool seg_batches = b->deliver_mode >= 1 && seg_args(b).seg != nullptr && !persist_batch(b, a, w);
+    if (seg_batches && b->seg_dirty && nsteps > 0) {
+        hipLaunchKernelGGL(segment_reset_kernel, dim3(1), dim3(64), 0, b->stream, b->seg_cnt, b->D,
+                           (int64_t)(b->steps_done + b->D));
+        DTB_TRY_HIP(hipGetLastError());
+        b->seg_dirty = false;
+    }
+    if (!seg_batches && nsteps > 0) b->seg_dirty = true;
     float* ustage = nullptr;
     const bool uhost = a->noise_mode == DTB_NOISE_UNIFORMS && !is_device_ptr(a->uniforms);
     if (uhost) DTB_TRY_HIP(hipMalloc(&ustage, sizeof(float) * (size_t)b->n * (size_t)std::min<int64_t>(nsteps, b->kmax)));
@@ -3054,6 +3265,7 @@ static int load_state_impl(Block* b, const void* src, int64_t bytes)
     const char* in = static_cast<const char*>(src);
     int64_t off = 256;
     int st = DTB_OK;
+    b->seg_dirty = true; // the segment lists are not part of the snapshot: rebuilt from the spike lists
     int8_t* jsnap = nullptr;
     DTB_TRY_HIP(hipMalloc(&jsnap, 4 * (size_t)b->n));
     const std::vector<SnapSection> secs = snap_sections(b);
@@ -3950,6 +4162,12 @@ int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value)
             DTB_REQUIRE(b->sorted_ok, "too many delivery buckets for the sorted staging variant");
         }
         b->stage_variant = value;
+    } else if (what == DTB_LAUNCH_SEGMENT_LISTS) {
+        DTB_REQUIRE(value >= 0, "segment list capacity must be >= 0");
+        DTB_TRY(ensure_bucketed(b));
+        if (value > 0) DTB_TRY(ensure_segments(b));
+        b->seg_cap = value == 0 ? 0 : value == 1 ? b->seg_alloc : std::min<int64_t>(value, b->seg_alloc);
+        b->seg_dirty = true;
     } else if (what == DTB_LAUNCH_DELIVER_MODE) {
         DTB_REQUIRE(value >= 0 && value <= 2,
                     "delivery mode must be 0 (atomics), 1 (bucketed) or 2 (persistent, small blocks)");
@@ -3981,6 +4199,7 @@ int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value)
     else if (what == DTB_LAUNCH_ACCUM_OCC) *value = b->acc_occ;
     else if (what == DTB_LAUNCH_ACCUM_UNROLL) *value = b->accum_unroll;
     else if (what == DTB_LAUNCH_ACCUM_FIXED) *value = b->accum_fixed ? b->acc_shift : 0;
+    else if (what == DTB_LAUNCH_SEGMENT_LISTS) *value = (int32_t)std::min<int64_t>(b->seg ? b->seg_cap : 0, INT32_MAX);
     else {
         set_error("unknown launch parameter %d", what);
         return DTB_INVALID_PARAMETER;

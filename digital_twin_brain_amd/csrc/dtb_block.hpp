@@ -139,6 +139,15 @@ struct Block {
     void* stage = nullptr;      // [nbk][stage_cap]
     void* acc2 = nullptr;       // [n] overflow accumulator
 
+    // segment lists (bucketed delivery, D > 1): every spike's non-empty
+    // (row, delay) segments, written once after the neuron kernel into the
+    // list of the step that delivers them
+    uint64_t* seg = nullptr;      // [D][seg_alloc]: (row start + offset) << 24 | length
+    uint32_t* seg_cnt = nullptr;  // [D + 1][32 words]: entries, overflow flag; entry D: first complete step (int64)
+    int64_t seg_alloc = 0;        // entries allocated per list
+    int64_t seg_cap = 0;          // entries used per list (0: segment lists off)
+    bool seg_dirty = true;        // the lists must be restarted before the next bucketed batch
+
     // persistent delivery (deliver_mode 2): the CSR grouped by owning
     // workgroup inside every (row, delay) segment, rebuilt when the weights change
     int32_t* pbptr = nullptr;   // [n*D][pG+1] start of each workgroup's synapses

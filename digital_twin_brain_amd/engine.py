@@ -253,7 +253,7 @@ class NativeBlock:
 
     # -------------------------------------------------------- launch shape
     LAUNCH = dict(deliver_grid=0, deliver_lanes=1, deliver_mode=2, neuron_psets=3, stage_segs=4, stage_ept=5,
-                  stage_variant=6, accum_occ=7, accum_unroll=8, accum_fixed=9)
+                  stage_variant=6, accum_occ=7, accum_unroll=8, accum_fixed=9, segment_lists=10)
 
     def set_launch(self, **kw):
         for k, v in kw.items():

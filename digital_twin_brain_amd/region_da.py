@@ -125,8 +125,12 @@ class SubsetSurrogateDA:
     its rows plus an equal replay of earlier rows."""
 
     def __init__(self, da, surrogate, judge, every=5, nrmse_max=0.02, finetune_epochs=4, replay=1.0,
-                 residual=None, seed=0):
+                 residual=None, seed=0, rejudge=0):
         self.da = da
+        # a judged window over the threshold is followed by up to `rejudge`
+        # further judged windows (the surrogate fine-tuned on each) before the
+        # full ensemble is simulated
+        self.rejudge = int(rejudge)
         # residual [V, 6]: the simulator's spread around the surrogate's mean; surrogate
         # forecasts that enter the analysis carry draws of it, so that the ensemble's
         # forecast spread, and with it the Kalman gain, is that of simulated windows
@@ -144,7 +148,8 @@ class SubsetSurrogateDA:
         E = da.ensemble_number
         S = self.judge.S
         forecasts, kinds, errs, times = [], [], [], []
-        full_next = False
+        full_next = judge_next = False
+        fails = 0  # consecutive judged windows over the threshold
         for t in range(windows):
             t0 = time.perf_counter()
             da._hp_log += torch.normal(0, hp_sigma ** 0.5, size=da._hp_log.shape).float()
@@ -168,7 +173,7 @@ class SubsetSurrogateDA:
                 tail = da._hidden_state[:, :, -6:].double().cpu()
                 e = nrmse(pred[:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
                 self.sur.add(X, tail.reshape(E, -1))
-            elif t % self.every == 0:
+            elif t % self.every == 0 or judge_next:
                 kind = "judge"
                 J = np.arange(S)
                 bold0 = X.reshape(E, -1, 9)[J][:, :, 4:8]
@@ -184,7 +189,10 @@ class SubsetSurrogateDA:
                 da._hidden_state = torch.cat((hpl.double().to(pred.device), fc), dim=2)
             if e is not None:
                 errs.append((t, kind, e))
-                full_next = e > self.nrmse_max
+                bad = e > self.nrmse_max
+                fails = fails + 1 if (bad and kind == "judge") else 0
+                judge_next = bad and kind == "judge" and fails <= self.rejudge
+                full_next = bad and not judge_next
                 if self.finetune_epochs:
                     # a full window adds E x R rows: a sample of them keeps the update cheap
                     self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t, max_rows=4 * 1024)
@@ -198,7 +206,7 @@ class SubsetSurrogateDA:
 
 
 def collect_da_windows(da, windows, steps, rng, low, high, R, segment=8, hp_sigma=0.25, bold_sigma=1e-8,
-                       solo_rate=0.8, seed=0):
+                       solo_rate=0.8, seed=0, log=None):
     """Training windows drawn the way the assimilation moves: segments of
     `segment` windows that start from the reference's logit grid, take the
     assimilation's random-walk step and then an EnKF analysis each window,
@@ -227,6 +235,8 @@ def collect_da_windows(da, windows, steps, rng, low, high, R, segment=8, hp_sigm
             Ys.append(da._hidden_state[:, :, -6:].double().cpu().reshape(E, -1))
             y = da._hidden_state[target, :, -1].reshape(1, -1)
             da.da_filter(y, bold_sigma=bold_sigma, solo_rate=solo_rate)
+        if log:
+            log(f"region DA: {len(Xs)} of {windows} training windows")
     return torch.cat(Xs), torch.cat(Ys)
 
 
@@ -258,7 +268,7 @@ def _restore(da, s):
 def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, windows=20, window_ms=800.0, dt=0.1,
                             subset=16, every=5, nrmse_max=None, hidden=(512, 512, 512), epochs=None,
                             finetune_epochs=None, seed=7, pre_ms=50.0, per_voxel=True, sample_residual=False, log=None,
-                            tune=True, **kw):
+                            tune=True, rejudge=0, **kw):
     """Simulator-only DA against surrogate-assisted DA (subset judge) on E
     whole-brain members, from the same state and hyper-parameters, against
     the same truth observations.  Returns a dict of timings, gain, the
@@ -276,7 +286,7 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     # training windows: assimilation-like segments against pseudo-observations
     t0 = time.perf_counter()
     X, Y = collect_da_windows(da, train_windows, steps, np.random.default_rng(seed + 3), low, high, R,
-                              seed=seed + 100)
+                              seed=seed + 100, log=log)
     t_collect = time.perf_counter() - t0
     say(f"region DA: {train_windows} training windows ({X.shape[0]} samples) in {t_collect:.1f} s")
     ws = WindowSurrogate(R, hidden=hidden, seed=seed, per_voxel=per_voxel, max_batch=2048 if per_voxel else 256,
@@ -330,7 +340,7 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     _restore(da, snap)
     del snap
     sda = SubsetSurrogateDA(da, ws, judge, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs,
-                            residual=residual, seed=seed)
+                            residual=residual, seed=seed, rejudge=rejudge)
     res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
     kinds = res["kinds"]
     say(f"region DA: {windows} surrogate-assisted windows, {res['times'].mean():.3f} s each, "
@@ -339,7 +349,7 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     errs = [e for _, _, e in res["nrmse"]]
     out = dict(members=E, regions=R, member_neurons=member_n, windows=windows, steps_per_window=steps,
                nrmse_threshold=nrmse_max, calibration={k: round(v, 5) for k, v in calib.items()},
-               neurons=int(nb.n), synapses=int(nb.nnz), judge_members=subset, judge_every=every,
+               neurons=int(nb.n), synapses=int(nb.nnz), judge_members=subset, judge_every=every, rejudge=rejudge,
                build_s=t_build, train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect,
                fit_s=t_fit, fit_epochs=epochs, fit_loss_first=hist[0], fit_loss_last=hist[-1],
                surrogate=(f"per-region MLP {2 * 9} -> " if per_voxel else f"per-member MLP {R * 9} -> ") +

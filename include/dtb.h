@@ -319,6 +319,12 @@ int dtb_block_set_external(dtb_block* b, int32_t n_thresholds, const uint32_t* t
                                       atomics; S is the largest shift keeping every neuron's largest possible
                                       per-step sum (sum of |w| over its inputs) below 2^30, allowed when S >= 16
                                       (the default then); get returns S when on, else 0 */
+#define DTB_LAUNCH_SEGMENT_LISTS 10 /* bucketed delivery with delays (max delay > 1): 1 (the default) writes each
+                                       spike's (row, delay) segments once, after the neuron kernel, into the list
+                                       of the step that delivers them, so the stage kernel reads them coalesced
+                                       instead of the CSR's row pointer and segment ends at every delay; 0 turns
+                                       this off; a value > 1 caps the entries per list (a step whose list is full
+                                       reads the spike lists instead); get returns the cap in use, 0 when off */
 int dtb_block_set_launch(dtb_block* b, int32_t what, int32_t value);
 int dtb_block_get_launch(dtb_block* b, int32_t what, int32_t* value);
 
