@@ -125,8 +125,13 @@ class SubsetSurrogateDA:
     its rows plus an equal replay of earlier rows."""
 
     def __init__(self, da, surrogate, judge, every=5, nrmse_max=0.02, finetune_epochs=4, replay=1.0,
-                 residual=None, seed=0, rejudge=0):
+                 residual=None, seed=0, rejudge=0, nrmse_scale="ensemble"):
         self.da = da
+        # a judged window's NRMSE divides the RMSE over the judged members by the
+        # range of the window's BOLD over the whole ensemble ("ensemble": the
+        # judged members' simulated values, the others' forecasts), the scale a
+        # full window's NRMSE has; "members": by the judged members' own range
+        self.nrmse_scale = nrmse_scale
         # a judged window over the threshold is followed by up to `rejudge`
         # further judged windows (the surrogate fine-tuned on each) before the
         # full ensemble is simulated
@@ -178,11 +183,15 @@ class SubsetSurrogateDA:
                 J = np.arange(S)
                 bold0 = X.reshape(E, -1, 9)[J][:, :, 4:8]
                 tail = self.judge.simulate(da._hp_log.reshape(E, -1)[J], bold0, steps)
-                e = nrmse(pred[J][:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
                 self.sur.add(X[J], tail.reshape(S, -1), idx=J)
                 full = torch.cat((hpl.double().to(pred.device), fc), dim=2)
                 full[torch.from_numpy(J).to(pred.device), :, -6:] = tail.to(pred.device)
                 da._hidden_state = full
+                scale = None
+                if self.nrmse_scale == "ensemble":
+                    wb = full[:, :, -1]
+                    scale = float((wb.max() - wb.min()).item())
+                e = nrmse(pred[J][:, :, -1].cpu().numpy(), tail[:, :, -1].numpy(), scale)
             else:
                 kind = "surrogate"
                 e = None
@@ -268,7 +277,7 @@ def _restore(da, s):
 def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, windows=20, window_ms=800.0, dt=0.1,
                             subset=16, every=5, nrmse_max=None, hidden=(512, 512, 512), epochs=None,
                             finetune_epochs=None, seed=7, pre_ms=50.0, per_voxel=True, sample_residual=False, log=None,
-                            tune=True, rejudge=0, **kw):
+                            tune=True, rejudge=0, nrmse_scale="ensemble", **kw):
     """Simulator-only DA against surrogate-assisted DA (subset judge) on E
     whole-brain members, from the same state and hyper-parameters, against
     the same truth observations.  Returns a dict of timings, gain, the
@@ -313,9 +322,11 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     da.get_hidden_state(steps)
     tail_m = da._hidden_state[:, :, -6:].double().cpu()
     pred_c = ws.predict(Xc)[:, :, -1].cpu().numpy()
-    floor = nrmse(tail_j[:, :, -1].numpy(), tail_m[J][:, :, -1].numpy())
-    calib = dict(sim_vs_sim=floor, sur_vs_full=nrmse(pred_c[J], tail_m[J][:, :, -1].numpy()),
-                 sur_vs_judge=nrmse(pred_c[J], tail_j[:, :, -1].numpy()),
+    # judged windows' scale: the range over the whole ensemble's window
+    sc = float(tail_m[:, :, -1].max() - tail_m[:, :, -1].min()) if nrmse_scale == "ensemble" else None
+    floor = nrmse(tail_j[:, :, -1].numpy(), tail_m[J][:, :, -1].numpy(), sc)
+    calib = dict(sim_vs_sim=floor, sur_vs_full=nrmse(pred_c[J], tail_m[J][:, :, -1].numpy(), sc),
+                 sur_vs_judge=nrmse(pred_c[J], tail_j[:, :, -1].numpy(), sc),
                  sur_vs_full_all_members=nrmse(pred_c, tail_m[:, :, -1].numpy()))
     if nrmse_max is None:
         nrmse_max = max(0.02, 1.25 * floor)
@@ -340,7 +351,7 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     _restore(da, snap)
     del snap
     sda = SubsetSurrogateDA(da, ws, judge, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs,
-                            residual=residual, seed=seed, rejudge=rejudge)
+                            residual=residual, seed=seed, rejudge=rejudge, nrmse_scale=nrmse_scale)
     res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
     kinds = res["kinds"]
     say(f"region DA: {windows} surrogate-assisted windows, {res['times'].mean():.3f} s each, "
@@ -350,6 +361,7 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     out = dict(members=E, regions=R, member_neurons=member_n, windows=windows, steps_per_window=steps,
                nrmse_threshold=nrmse_max, calibration={k: round(v, 5) for k, v in calib.items()},
                neurons=int(nb.n), synapses=int(nb.nnz), judge_members=subset, judge_every=every, rejudge=rejudge,
+               nrmse_scale=nrmse_scale,
                build_s=t_build, train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect,
                fit_s=t_fit, fit_epochs=epochs, fit_loss_first=hist[0], fit_loss_last=hist[-1],
                surrogate=(f"per-region MLP {2 * 9} -> " if per_voxel else f"per-member MLP {R * 9} -> ") +

@@ -64,10 +64,12 @@ def window_base(X, V, use_act=True):
     return torch.stack((act, s, q, v, f, bold), dim=-1).reshape(X.shape[0], V * 6)
 
 
-def nrmse(sur, sim):
+def nrmse(sur, sim, scale=None):
+    """RMSE of sur against sim over all entries, divided by `scale` (default:
+    the range of sim)."""
     sur = np.asarray(sur, np.float64)
     sim = np.asarray(sim, np.float64)
-    rng = float(sim.max() - sim.min())
+    rng = float(sim.max() - sim.min()) if scale is None else float(scale)
     return float(np.linalg.norm(sur - sim) / np.sqrt(sim.size) / max(rng, 1e-300))
 
 
