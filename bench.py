@@ -190,7 +190,7 @@ def plain_line(steps, warmup, log):
 
     from digital_twin_brain_amd.autotune import tune_delivery
 
-    nb, _ = networks.build_plain()
+    nb, props = networks.build_plain()
     nb.run(warmup, noise_mode=1, graph=True)
     s0 = nb.steps
     best, table = tune_delivery(nb, dict(noise_mode=1), steps=20, use_cache=False)
@@ -208,7 +208,35 @@ def plain_line(steps, warmup, log):
            "launch": nb.get_launch()}
     log(f"plain block: {out['value']:.3e} events/s, {out['ms_per_step'] * 1e3:.2f} us/step, "
         f"{out['events_per_step']:.0f} events/step")
+    # the same network behind the drop-in `block` interface (models/block.py), untuned:
+    # run() hands back each step's statistics (a host round trip per step); run_steps()
+    # replays a captured graph and returns them all at the end
+    import torch
+
+    from digital_twin_brain_amd.block import block
+
+    rp, post, kind, w = nb.get("rowptr"), nb.get("post"), nb.get("kind"), nb.get("w0")
+    pre = np.repeat(np.arange(nb.n), np.diff(rp))
+    ch = kind.astype(np.int64) * 2
+    idx = np.concatenate([np.stack([ch, post, pre]), np.stack([ch + 1, post, pre])], axis=1)
+    W = torch.sparse_coo_tensor(torch.from_numpy(idx), torch.from_numpy(np.concatenate([w, w])), (4, nb.n, nb.n))
     nb.close()
+    B = block(torch.from_numpy(props), W, delta_t=0.1, seed=3)
+    for _ in range(20):
+        B.run(noise_rate=1e-3)
+    t0 = time.perf_counter()
+    for _ in range(300):
+        B.run(noise_rate=1e-3)
+    per_run = (time.perf_counter() - t0) / 300 * 1e6
+    B.run_steps(steps, noise_rate=1e-3)  # capture
+    t0 = time.perf_counter()
+    B.run_steps(steps, noise_rate=1e-3)
+    per_batched = (time.perf_counter() - t0) / steps * 1e6
+    out["dropin_block"] = {"run_us_per_step": per_run, "run_steps_us_per_step": per_batched,
+                           "note": "models/block.py interface, default launch, noise 10 Hz as a scalar rate"}
+    log(f"plain block behind the drop-in interface: run() {per_run:.1f} us/step, run_steps() "
+        f"{per_batched:.1f} us/step")
+    B._nb.close()
     return out
 
 

@@ -249,6 +249,28 @@ class block:
             self.iter += 1
         return self._t(self._freq.astype(np.float32)), self._t(self._vmean.copy())
 
+    def run_steps(self, steps, noise_rate=0.01, isolated=False):
+        """`steps` calls of run() in one library call (replayed as a captured
+        graph, no host work between steps): returns (sum_activate[steps, S],
+        mean_Vi[steps, S]), row t being what the t-th run() would return.
+        run() pays a host round trip per step, since it hands back that step's
+        statistics; this form is for callers that only need them afterwards."""
+        steps = int(steps)
+        if steps < 0:
+            raise ValueError("steps must be >= 0")
+        if self.src_neuron is not None and self.iter + steps > self._sched.shape[1]:
+            raise IndexError(f"src has {self._sched.shape[1]} columns: no forced spikes past step "
+                             f"{self._sched.shape[1]}")
+        S = self._nb.S
+        freq = np.zeros((steps, S), np.uint32)
+        vmean = np.zeros((steps, S), np.float32)
+        if steps:
+            self._nb.run(steps, noise_mode=nat.NOISE_SCALAR, noise_rate=float(noise_rate), isolated=isolated,
+                         graph=True, freqs=freq, vmeans=vmean)
+        if self.src_neuron is not None:
+            self.iter += steps
+        return self._t(freq.astype(np.float32)), self._t(vmean)
+
     # state attributes, read from the device on access
     @property
     def active(self):

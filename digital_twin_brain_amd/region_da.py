@@ -125,7 +125,7 @@ class SubsetSurrogateDA:
     its rows plus an equal replay of earlier rows."""
 
     def __init__(self, da, surrogate, judge, every=5, nrmse_max=0.02, finetune_epochs=4, replay=1.0,
-                 residual=None, seed=0, rejudge=0, nrmse_scale="ensemble"):
+                 residual=None, seed=0, rejudge=0, nrmse_scale="members"):
         self.da = da
         # a judged window's NRMSE divides the RMSE over the judged members by the
         # range of the window's BOLD over the whole ensemble ("ensemble": the
@@ -277,7 +277,7 @@ def _restore(da, s):
 def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, windows=20, window_ms=800.0, dt=0.1,
                             subset=16, every=5, nrmse_max=None, hidden=(512, 512, 512), epochs=None,
                             finetune_epochs=None, seed=7, pre_ms=50.0, per_voxel=True, sample_residual=False, log=None,
-                            tune=True, rejudge=0, nrmse_scale="ensemble", **kw):
+                            tune=True, rejudge=0, nrmse_scale="members", **kw):
     """Simulator-only DA against surrogate-assisted DA (subset judge) on E
     whole-brain members, from the same state and hyper-parameters, against
     the same truth observations.  Returns a dict of timings, gain, the

@@ -367,3 +367,40 @@ def test_simulation_run_end_to_end(tmp_path):
         out = sim2.evolve(40)
         assert np.array_equal(out[0].cpu().numpy(), fa[i])
         np.testing.assert_array_equal(out[-1].cpu().numpy().astype(np.float32), bolds[i])
+
+
+def test_dropin_run_steps_equals_repeated_run():
+    """block.run_steps(K) (one graph-replayed library call) returns row by row
+    what K calls of block.run() return, and leaves the same state (dyadic
+    weights: every delivery order gives the same sums)."""
+    from digital_twin_brain_amd.block import block
+
+    rng = np.random.default_rng(41)
+    n, ne = 3000, 2400
+    csr = orc.gen_fixed_indegree(n, ne, 60, 1, 0.0, 1.0, True, seed=42)
+    for k in ("w0", "w1"):
+        csr[k] = (np.floor(csr[k] * 256) / 256).astype(np.float32)
+    pre = np.repeat(np.arange(n), np.diff(csr["rowptr"]))
+    _, first = np.unique(pre * n + csr["post"], return_index=True)
+    keep = np.zeros(pre.shape[0], bool)
+    keep[first] = True
+    csr = _keep_synapses(n, csr, keep)
+    pre = np.repeat(np.arange(n), np.diff(csr["rowptr"]))
+    p = _lif_props(n, rng, g_scale=1.5)
+    idx = torch.from_numpy(np.stack([csr["kind"].astype(np.int64) * 2, csr["post"], pre]))
+    idx = torch.cat([idx, idx + torch.tensor([[1], [0], [0]])], dim=1)
+    W = torch.sparse_coo_tensor(idx, torch.from_numpy(np.concatenate([csr["w0"], csr["w1"]])), (4, n, n))
+    a = block(torch.from_numpy(p), W, delta_t=0.1, seed=9)
+    b = block(torch.from_numpy(p), W, delta_t=0.1, seed=9)
+    K = 300
+    rows = [a.run(noise_rate=2.0 ** -6) for _ in range(K)]
+    fa = torch.stack([r[0] for r in rows]).numpy()
+    va = torch.stack([r[1] for r in rows]).numpy()
+    fb, vb = (x.numpy() for x in b.run_steps(K, noise_rate=2.0 ** -6))
+    assert fa.shape == fb.shape == (K, a.sub_idx.max().item() + 1)
+    assert fa.sum() > 0
+    assert np.array_equal(fa, fb)
+    np.testing.assert_allclose(va, vb, rtol=1e-6, atol=1e-6)  # float atomic sums per population
+    assert np.array_equal(a.V_i.numpy(), b.V_i.numpy())
+    assert np.array_equal(a.J_ui.numpy(), b.J_ui.numpy())
+    assert float(a.t) == float(b.t)
