@@ -109,6 +109,135 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int32_t M, int32_t N, int3
     }
 }
 
+// Large products: LDS-staged tiles.  A workgroup of eight wavefronts (4 x 2)
+// computes a 256x256 tile of C; each wavefront a 64x128 sub-tile (2 x 4
+// MFMA tiles of 32x32, eight v_mfma_f32_32x32x16_bf16 per 16-wide K step
+// from six LDS fragments).  K advances 64 at a time: the next 256x64 slices
+// of A and B are loaded from global memory into registers (16-byte loads, 8
+// lanes per 128-byte row) while the current ones are multiplied out of LDS,
+// then stored to the other half of a double buffer; one barrier per slice.
+// LDS rows are 64 + 8 bf16 long, so the 32 rows a fragment read touches
+// fall on distinct banks.  Workgroups are dealt to tiles XCD by XCD
+// (workgroup w runs on XCD w mod 8): each XCD takes a contiguous run of
+// tiles, whose A and B slices then repeat in its own L2.
+// Requires M, N multiples of 256 and K of 64.
+constexpr int kLdsBM = 256, kLdsBN = 256, kLdsKT = 64, kLdsRow = kLdsKT + 8;
+
+__global__ __launch_bounds__(512, 1) void gemm_nt_lds_kernel(int32_t M, int32_t N, int32_t K,
+                                                             const uint16_t* __restrict__ A, int64_t lda,
+                                                             const uint16_t* __restrict__ B, int64_t ldb,
+                                                             const float* __restrict__ bias, int32_t act,
+                                                             float* __restrict__ C, int64_t ldc,
+                                                             uint16_t* __restrict__ Cb, int64_t ldcb)
+{
+    extern __shared__ __align__(16) uint16_t lds[]; // [2][A 256 x 72 | B 256 x 72]
+    constexpr int kStage = (kLdsBM + kLdsBN) * kLdsRow;
+    const int t = threadIdx.x;
+    const int wid = t >> 6;
+    const int lane = t & 63;
+    const int wm = wid >> 1, wn = wid & 1;
+    // tile of this workgroup: XCD-contiguous runs of tiles, row-major within
+    const int tn = N / kLdsBN;
+    const int tiles = (M / kLdsBM) * tn;
+    const int nx = 8;
+    const int w = blockIdx.x;
+    const int per = (tiles + nx - 1) / nx;
+    const int xcd = w % nx, slot = w / nx;
+    const int tile = xcd * per + slot;
+    if (tile >= tiles) return; // workgroup-uniform
+    const int m_blk = (tile / tn) * kLdsBM, n_blk = (tile % tn) * kLdsBN;
+    // global -> register copies: chunk c = t + 512 u (u < 4) of each operand,
+    // row c / 8, 8 bf16 at column 8 (c % 8) of the 64-wide slice
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const uint16_t* ga[4];
+    const uint16_t* gb[4];
+    int so[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int c = t + 512 * u;
+        const int row = c >> 3, kc = (c & 7) * 8;
+        ga[u] = A + (int64_t)(m_blk + row) * lda + kc;
+        gb[u] = B + (int64_t)(n_blk + row) * ldb + kc;
+        so[u] = row * kLdsRow + kc;
+    }
+    u32x4 ra[4], rb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        ra[u] = *reinterpret_cast<const u32x4*>(ga[u]);
+        rb[u] = *reinterpret_cast<const u32x4*>(gb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        *reinterpret_cast<u32x4*>(lds + so[u]) = ra[u];
+        *reinterpret_cast<u32x4*>(lds + kLdsBM * kLdsRow + so[u]) = rb[u];
+    }
+    __syncthreads();
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int x = 0; x < 16; ++x) acc[i][j][x] = 0.0f;
+    const int r = lane & 31;
+    const int kh = (lane >> 5) * 8;
+    const int nk = K / kLdsKT;
+    for (int kt = 0; kt < nk; ++kt) {
+        const bool more = kt + 1 < nk;
+        if (more) {
+            const int64_t k1 = (int64_t)(kt + 1) * kLdsKT;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                ra[u] = *reinterpret_cast<const u32x4*>(ga[u] + k1);
+                rb[u] = *reinterpret_cast<const u32x4*>(gb[u] + k1);
+            }
+        }
+        const uint16_t* sa = lds + (kt & 1) * kStage;
+        const uint16_t* sb = sa + kLdsBM * kLdsRow;
+#pragma unroll
+        for (int ks = 0; ks < kLdsKT / 16; ++ks) {
+            bf16x8 a[2], b[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(sa + (wm * 64 + 32 * i + r) * kLdsRow + ks * 16 + kh);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                b[j] = *reinterpret_cast<const bf16x8*>(sb + (wn * 128 + 32 * j + r) * kLdsRow + ks * 16 + kh);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            uint16_t* da = lds + ((kt + 1) & 1) * kStage;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                *reinterpret_cast<u32x4*>(da + so[u]) = ra[u];
+                *reinterpret_cast<u32x4*>(da + kLdsBM * kLdsRow + so[u]) = rb[u];
+            }
+        }
+        __syncthreads();
+    }
+    const int m0 = m_blk + wm * 64, n0 = n_blk + wn * 128;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = n0 + 32 * j + r;
+        const float bv = bias ? bias[col] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int x = 0; x < 16; ++x) {
+                const int row = m0 + 32 * i + 8 * (x >> 2) + 4 * (lane >> 5) + (x & 3);
+                float v = acc[i][j][x] + bv;
+                if (act == 1) v = v > 0.0f ? v : 0.0f;
+                if (C) C[(int64_t)row * ldc + col] = v;
+                if (Cb) Cb[(int64_t)row * ldcb + col] = f2bf(v);
+            }
+        }
+    }
+}
+
 // Small products (too few 64x64 tiles to fill the chip): a workgroup computes
 // one 32x32 tile of C and its four wavefronts split K in four consecutive
 // ranges; wavefronts 1-3 hand their partial sums to wavefront 0 through LDS,
@@ -350,7 +479,16 @@ int dtb_mlp_gemm_bf16_nt(int32_t M, int32_t N, int32_t K, const void* A, int64_t
     // workgroup, else 64x64, else 32x32 tiles with K split over the wavefronts
     // (the surrogate's own products are that small)
     const int64_t big = (int64_t)((N + 127) / 128) * ((M + 127) / 128);
-    if (big >= 256) {
+    const int64_t lds_tiles = (int64_t)(N / kLdsBN) * (M / kLdsBM);
+    if (M % kLdsBM == 0 && N % kLdsBN == 0 && K % kLdsKT == 0 && lds_tiles >= 64) {
+        // LDS-staged 256x256 tiles, one workgroup per tile (the tile map deals them XCD by XCD)
+        const size_t lds = sizeof(uint16_t) * 2 * (size_t)(kLdsBM + kLdsBN) * kLdsRow;
+        DTB_TRY_HIP(hipFuncSetAttribute((const void*)gemm_nt_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+        const unsigned nwg = (unsigned)((lds_tiles + 7) / 8 * 8);
+        hipLaunchKernelGGL(gemm_nt_lds_kernel, dim3(nwg), dim3(512), lds, (hipStream_t)stream, M, N, K,
+                           (const uint16_t*)A, lda, (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);
+    } else if (big >= 256) {
         dim3 grid((unsigned)((N + 127) / 128), (unsigned)((M + 127) / 128));
         hipLaunchKernelGGL((gemm_nt_kernel<2, 2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
                            (const uint16_t*)A, lda, (const uint16_t*)B, ldb, bias, act, C, ldc, (uint16_t*)Cb, ldcb);

@@ -153,6 +153,7 @@ class SubsetSurrogateDA:
         E = da.ensemble_number
         S = self.judge.S
         forecasts, kinds, errs, times = [], [], [], []
+        phases = []  # per window: seconds in (forecast, simulation, fine-tuning, analysis)
         full_next = judge_next = False
         fails = 0  # consecutive judged windows over the threshold
         for t in range(windows):
@@ -166,6 +167,8 @@ class SubsetSurrogateDA:
             if self.residual is not None:
                 draw = torch.randn(pred.shape, generator=self.gen, dtype=torch.float64) * self.residual
                 fc = pred + draw.to(pred.device)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
             if full_next:
                 kind = "full"
                 da.push_hp()
@@ -196,6 +199,8 @@ class SubsetSurrogateDA:
                 kind = "surrogate"
                 e = None
                 da._hidden_state = torch.cat((hpl.double().to(pred.device), fc), dim=2)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
             if e is not None:
                 errs.append((t, kind, e))
                 bad = e > self.nrmse_max
@@ -205,13 +210,18 @@ class SubsetSurrogateDA:
                 if self.finetune_epochs:
                     # a full window adds E x R rows: a sample of them keeps the update cheap
                     self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t, max_rows=4 * 1024)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
             forecasts.append(da._hidden_state.cpu().numpy())
             kinds.append(kind)
             y = observe(t)
             da.da_filter(torch.as_tensor(y).reshape(1, -1), bold_sigma=bold_sigma, solo_rate=solo_rate)
             torch.cuda.synchronize()
-            times.append(time.perf_counter() - t0)
-        return dict(forecasts=np.stack(forecasts), kinds=kinds, nrmse=errs, times=np.array(times))
+            t4 = time.perf_counter()
+            times.append(t4 - t0)
+            phases.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3))
+        return dict(forecasts=np.stack(forecasts), kinds=kinds, nrmse=errs, times=np.array(times),
+                    phases=np.array(phases))
 
 
 def collect_da_windows(da, windows, steps, rng, low, high, R, segment=8, hp_sigma=0.25, bold_sigma=1e-8,
@@ -370,6 +380,10 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
                gain=float(t_sim.sum() / res["times"].sum()), kinds=kinds,
                window_s_by_kind={k: float(np.mean([x for x, kk in zip(res["times"], kinds) if kk == k]))
                                  for k in sorted(set(kinds))},
+               window_phase_s_by_kind={k: dict(zip(("forecast", "simulation", "finetune", "analysis"),
+                                                   np.round(res["phases"][[kk == k for kk in kinds]].mean(0), 4)
+                                                   .tolist()))
+                                       for k in sorted(set(kinds))},
                nrmse=[(t, k, round(e, 5)) for t, k, e in res["nrmse"]],
                nrmse_median=float(np.median(errs)) if errs else None, nrmse_max=float(np.max(errs)) if errs else None,
                bold_rmse_sim_last=float(rmse(w_sim)[-1]), bold_rmse_sur_last=float(rmse(res["forecasts"])[-1]),

@@ -427,8 +427,12 @@ class GRUSurrogate:
 
 def gemm_kernel_name(M, N, K):
     """The kernel dtb_mlp_gemm_bf16_nt dispatches an M x N x K product to:
-    128x128 tiles while they give at least 256 workgroups, else 64x64 tiles,
-    else 32x32 tiles with K split over the four wavefronts."""
+    LDS-staged 256x256 tiles when M and N are multiples of 256, K of 64 and
+    there are at least 64 such tiles; else 128x128 tiles while they give at
+    least 256 workgroups, else 64x64 tiles, else 32x32 tiles with K split
+    over the four wavefronts."""
+    if M % 256 == 0 and N % 256 == 0 and K % 64 == 0 and (M // 256) * (N // 256) >= 64:
+        return "gemm_nt_lds_kernel (256x256 LDS-staged tiles, bf16 MFMA, FP32 accumulate)"
     if ((N + 127) // 128) * ((M + 127) // 128) >= 256:
         return "gemm_nt_kernel<2,2> (128x128 tiles, bf16 MFMA, FP32 accumulate)"
     if ((N + 63) // 64) * ((M + 63) // 64) >= 256:

@@ -149,12 +149,20 @@ class WindowSurrogate:
             pick = torch.randperm(Xn.shape[0], generator=torch.Generator().manual_seed(seed + 1))[:max_rows]
             Xn, Yn = Xn.index_select(0, pick), Yn.index_select(0, pick)
         if len(self.X) > 1 and replay > 0:
-            Xo = torch.cat(self.X[:-1])
-            Yo = torch.cat(self.Y[:-1])
-            k = min(Xo.shape[0], int(round(replay * Xn.shape[0])))
-            idx = torch.randperm(Xo.shape[0], generator=torch.Generator().manual_seed(seed))[:k]
-            Xn = torch.cat((Xn, Xo.index_select(0, idx)))
-            Yn = torch.cat((Yn, Yo.index_select(0, idx)))
+            # distinct rows drawn from the earlier windows, gathered block by
+            # block (no concatenation of everything stored so far)
+            sizes = np.array([b.shape[0] for b in self.X[:-1]], np.int64)
+            ends = np.cumsum(sizes)
+            k = int(min(ends[-1], round(replay * Xn.shape[0])))
+            rng = np.random.default_rng(seed)
+            g = np.sort(rng.choice(ends[-1], size=k, replace=False)) if k < ends[-1] else np.arange(ends[-1])
+            blk = np.searchsorted(ends, g, side="right")
+            xs, ys = [Xn], [Yn]
+            for b in np.unique(blk):
+                loc = torch.from_numpy(g[blk == b] - (ends[b] - sizes[b]))
+                xs.append(self.X[b].index_select(0, loc))
+                ys.append(self.Y[b].index_select(0, loc))
+            Xn, Yn = torch.cat(xs), torch.cat(ys)
         return self.mlp.fit(Xn, Yn, epochs=epochs, seed=seed)
 
     def residual_std(self, X, Y):

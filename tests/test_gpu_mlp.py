@@ -34,10 +34,13 @@ def gemm(A, B, bias=None, act=0, want_bf16=False, lda=None, ldb=None):
 
 
 # the first four take 32x32 tiles with K split over the wavefronts (K = 16 gives
-# three wavefronts an empty range); 1024x1024 takes 64x64 tiles; the last two
-# take 128x128 tiles (at least 256 of them), with a 32-row and a 64-column edge band
+# three wavefronts an empty range); 1024x1024 takes 64x64 tiles; the next two
+# take 128x128 tiles (at least 256 of them), with a 32-row and a 64-column edge band;
+# the last three take LDS-staged 256x256 tiles (64, 72 and 81 tiles: the tile map
+# deals workgroups eight at a time, so 81 leaves seven idle)
 @pytest.mark.parametrize("M,N,K", [(32, 32, 16), (96, 160, 48), (256, 512, 384), (64, 96, 1040), (1024, 1024, 48),
-                                   (2080, 2080, 64), (2048, 2112, 48)])
+                                   (2080, 2080, 64), (2048, 2112, 48), (2048, 2048, 128), (2304, 2048, 64),
+                                   (2304, 2304, 192)])
 def test_gemm_bit_exact_on_integer_operands(M, N, K):
     g = torch.Generator().manual_seed(M * 7 + N + K)
     A = torch.randint(-4, 5, (M, K), generator=g).to(torch.bfloat16)
@@ -51,15 +54,15 @@ def test_gemm_bit_exact_on_integer_operands(M, N, K):
     assert torch.equal(Cb.cpu().float(), want.clamp_min(0).float().to(torch.bfloat16).float())
 
 
-def test_gemm_strided_operands_and_random_values():
+@pytest.mark.parametrize("M,N,K", [(128, 64, 96), (2048, 2048, 256)])
+def test_gemm_strided_operands_and_random_values(M, N, K):
     g = torch.Generator().manual_seed(5)
-    M, N, K = 128, 64, 96
     Abuf = torch.randn((M, K + 24), generator=g).to(torch.bfloat16).cuda()
     Bbuf = torch.randn((N, K + 8), generator=g).to(torch.bfloat16).cuda()
     A, B = Abuf[:, :K], Bbuf[:, :K]
     C = gemm(A, B)
     want = A.float().cpu() @ B.float().cpu().T
-    torch.testing.assert_close(C.cpu(), want, rtol=1e-5, atol=1e-5)
+    torch.testing.assert_close(C.cpu(), want, rtol=1e-5, atol=1e-5 * K ** 0.5)
 
 
 def test_gemm_rejects_bad_shapes():
