@@ -134,7 +134,12 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
         da._push(hp)
         da.timings = []
 
+    from .region_da import _restore, _snapshot
+
+    # both runs start from this state (block state, properties, hyper-parameters, RNG streams)
     reset_hp()
+    da._hidden_state = None
+    snap = _snapshot(da)
     obs = []
 
     def observe(t):
@@ -145,7 +150,8 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
     w_sim, _, t_sim = da.da_rest_run(observe, "/tmp", observation_times=windows, steps=steps, save=False)
     obs = np.stack(obs)
     say(f"surrogate DA: {windows} simulator-only windows, {t_sim.mean():.3f} s each")
-    reset_hp()
+    _restore(da, snap)
+    del snap
     sda = SurrogateDA(da, ws, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs, adapt=adapt)
     res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
     say(f"surrogate DA: {windows} surrogate-assisted windows, {res['times'].mean():.3f} s each, "
@@ -159,7 +165,8 @@ def run_surrogate_da(E=64, voxels=90, train_windows=40, windows=20, window_ms=80
                sur_window_s_mean=float(res["times"].mean()),
                gain=float(t_sim.sum() / res["times"].sum()), simulated_windows=int(res["simulated"].sum()),
                nrmse=[round(e, 5) for _, e in res["nrmse"]],
-               bold_rmse_sim_last=float(rmse(w_sim)[-1]), bold_rmse_sur_last=float(rmse(res["forecasts"])[-1]))
+               bold_rmse_sim_last=float(rmse(w_sim)[-1]), bold_rmse_sur_last=float(rmse(res["forecasts"])[-1]),
+               bold_rmse_sim_mean=float(rmse(w_sim).mean()), bold_rmse_sur_mean=float(rmse(res["forecasts"]).mean()))
     if log:
         log(f"surrogate DA: {out}")
     return out, dict(w_sim=w_sim, sur=res, obs=obs, hist=hist, truth_hp=truth_hp, da=da)

@@ -125,8 +125,16 @@ class SubsetSurrogateDA:
     its rows plus an equal replay of earlier rows."""
 
     def __init__(self, da, surrogate, judge, every=5, nrmse_max=0.02, finetune_epochs=4, replay=1.0,
-                 residual=None, seed=0, rejudge=0, nrmse_scale="members"):
+                 residual=None, seed=0, rejudge=0, nrmse_scale="members", refull=False):
         self.da = da
+        # refull=False: a full window's NRMSE is recorded, but only judged
+        # windows send the next window to the simulator.  A full window starts
+        # from the surrogate-driven state of the windows before it (that error
+        # is what its NRMSE measures), and it hands the next window a
+        # simulated state, from which the surrogate starts out as accurate as
+        # after any simulated window.  refull=True: a full window over the
+        # threshold is followed by another full window.
+        self.refull = bool(refull)
         # a judged window's NRMSE divides the RMSE over the judged members by the
         # range of the window's BOLD over the whole ensemble ("ensemble": the
         # judged members' simulated values, the others' forecasts), the scale a
@@ -206,7 +214,7 @@ class SubsetSurrogateDA:
                 bad = e > self.nrmse_max
                 fails = fails + 1 if (bad and kind == "judge") else 0
                 judge_next = bad and kind == "judge" and fails <= self.rejudge
-                full_next = bad and not judge_next
+                full_next = bad and not judge_next and (kind == "judge" or self.refull)
                 if self.finetune_epochs:
                     # a full window adds E x R rows: a sample of them keeps the update cheap
                     self.sur.finetune(self.finetune_epochs, replay=self.replay, seed=t, max_rows=4 * 1024)
@@ -287,7 +295,7 @@ def _restore(da, s):
 def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, windows=20, window_ms=800.0, dt=0.1,
                             subset=16, every=5, nrmse_max=None, hidden=(512, 512, 512), epochs=None,
                             finetune_epochs=None, seed=7, pre_ms=50.0, per_voxel=True, sample_residual=False, log=None,
-                            tune=True, rejudge=0, nrmse_scale="members", **kw):
+                            tune=True, rejudge=0, nrmse_scale="members", refull=False, calib_windows=3, **kw):
     """Simulator-only DA against surrogate-assisted DA (subset judge) on E
     whole-brain members, from the same state and hyper-parameters, against
     the same truth observations.  Returns a dict of timings, gain, the
@@ -324,20 +332,26 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     nres = min(X.shape[0], 16 * E)  # the spread of the last training windows around the fit
     residual = ws.residual_std(X[-nres:], Y[-nres:]) if sample_residual else None
     # the simulator's own floor: the judge and the full ensemble simulate the first members' window
-    # from the same state and hyper-parameters (two noise streams)
-    Xc = window_inputs(da)
+    # from the same state and hyper-parameters (two noise streams), over `calib_windows`
+    # consecutive windows; the threshold uses the median
     J = np.arange(subset)
-    tail_j = judge.simulate(da._hp_log.reshape(E, -1)[J], Xc.reshape(E, -1, 9)[J][:, :, 4:8], steps)
-    da.push_hp()
-    da.get_hidden_state(steps)
-    tail_m = da._hidden_state[:, :, -6:].double().cpu()
-    pred_c = ws.predict(Xc)[:, :, -1].cpu().numpy()
-    # judged windows' scale: the range over the whole ensemble's window
-    sc = float(tail_m[:, :, -1].max() - tail_m[:, :, -1].min()) if nrmse_scale == "ensemble" else None
-    floor = nrmse(tail_j[:, :, -1].numpy(), tail_m[J][:, :, -1].numpy(), sc)
-    calib = dict(sim_vs_sim=floor, sur_vs_full=nrmse(pred_c[J], tail_m[J][:, :, -1].numpy(), sc),
-                 sur_vs_judge=nrmse(pred_c[J], tail_j[:, :, -1].numpy(), sc),
-                 sur_vs_full_all_members=nrmse(pred_c, tail_m[:, :, -1].numpy()))
+    cal = []
+    for _ in range(max(1, int(calib_windows))):
+        Xc = window_inputs(da)
+        tail_j = judge.simulate(da._hp_log.reshape(E, -1)[J], Xc.reshape(E, -1, 9)[J][:, :, 4:8], steps)
+        da.push_hp()
+        da.get_hidden_state(steps)
+        tail_m = da._hidden_state[:, :, -6:].double().cpu()
+        pred_c = ws.predict(Xc)[:, :, -1].cpu().numpy()
+        # judged windows' scale: the range over the whole ensemble's window
+        sc = float(tail_m[:, :, -1].max() - tail_m[:, :, -1].min()) if nrmse_scale == "ensemble" else None
+        cal.append(dict(sim_vs_sim=nrmse(tail_j[:, :, -1].numpy(), tail_m[J][:, :, -1].numpy(), sc),
+                        sur_vs_full=nrmse(pred_c[J], tail_m[J][:, :, -1].numpy(), sc),
+                        sur_vs_judge=nrmse(pred_c[J], tail_j[:, :, -1].numpy(), sc),
+                        sur_vs_full_all_members=nrmse(pred_c, tail_m[:, :, -1].numpy())))
+    calib = {k: float(np.median([c[k] for c in cal])) for k in cal[0]}
+    calib["sim_vs_sim_windows"] = [round(c["sim_vs_sim"], 5) for c in cal]
+    floor = calib["sim_vs_sim"]
     if nrmse_max is None:
         nrmse_max = max(0.02, 1.25 * floor)
     say(f"region DA: NRMSE floor (simulation against simulation) {floor:.4f}, surrogate {calib}; threshold "
@@ -361,7 +375,8 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     _restore(da, snap)
     del snap
     sda = SubsetSurrogateDA(da, ws, judge, every=every, nrmse_max=nrmse_max, finetune_epochs=finetune_epochs,
-                            residual=residual, seed=seed, rejudge=rejudge, nrmse_scale=nrmse_scale)
+                            residual=residual, seed=seed, rejudge=rejudge, nrmse_scale=nrmse_scale,
+                            refull=refull)
     res = sda.run(lambda t: torch.from_numpy(obs[t]), windows, steps=steps)
     kinds = res["kinds"]
     say(f"region DA: {windows} surrogate-assisted windows, {res['times'].mean():.3f} s each, "
@@ -369,9 +384,10 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
     rmse = lambda w: np.sqrt(((w[:, :, :, -1].mean(1) - obs) ** 2).mean(1))
     errs = [e for _, _, e in res["nrmse"]]
     out = dict(members=E, regions=R, member_neurons=member_n, windows=windows, steps_per_window=steps,
-               nrmse_threshold=nrmse_max, calibration={k: round(v, 5) for k, v in calib.items()},
+               nrmse_threshold=nrmse_max,
+               calibration={k: (round(v, 5) if isinstance(v, float) else v) for k, v in calib.items()},
                neurons=int(nb.n), synapses=int(nb.nnz), judge_members=subset, judge_every=every, rejudge=rejudge,
-               nrmse_scale=nrmse_scale,
+               nrmse_scale=nrmse_scale, refull=refull,
                build_s=t_build, train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect,
                fit_s=t_fit, fit_epochs=epochs, fit_loss_first=hist[0], fit_loss_last=hist[-1],
                surrogate=(f"per-region MLP {2 * 9} -> " if per_voxel else f"per-member MLP {R * 9} -> ") +

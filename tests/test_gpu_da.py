@@ -234,6 +234,13 @@ def test_surrogate_da_small(adapt):
     if (~sim).any():
         assert times[~sim].mean() < times[sim].mean()
     assert out["fit_loss_last"] < out["fit_loss_first"]
+    # both runs start from the same snapshot: the first window is simulated in
+    # both, from the same state and hyper-parameters (float atomics may round
+    # differently, so not bit for bit)
+    b_sim, b_sur = rec["w_sim"][0][..., -1], rec["sur"]["forecasts"][0][..., -1]
+    assert np.linalg.norm(b_sur - b_sim) <= 0.05 * np.linalg.norm(b_sim) + 1e-9
+    # the assisted run tracks the truth about as well as the simulator-only run
+    assert out["bold_rmse_sur_mean"] <= 2.0 * out["bold_rmse_sim_mean"] + 1e-3, out
 
 
 def test_region_surrogate_da_small():
@@ -259,10 +266,10 @@ def test_region_surrogate_da_small():
     kinds = out["kinds"]
     assert kinds[0] == "judge"
     for t, k in enumerate(kinds):
-        prev = [e for (tt, kk, e) in out["nrmse"] if tt == t - 1]
-        if prev and prev[0] > out["nrmse_threshold"]:
-            assert k == "full", (t, kinds)
-        elif k != "full":
+        prev = [(kk, e) for (tt, kk, e) in out["nrmse"] if tt == t - 1]
+        if prev and prev[0][0] == "judge" and prev[0][1] > out["nrmse_threshold"]:
+            assert k == "full", (t, kinds)  # a judged window over the threshold: the simulator next
+        else:
             assert k == ("judge" if t % 5 == 0 else "surrogate"), (t, kinds)
     if "surrogate" in out["window_s_by_kind"]:
         assert out["window_s_by_kind"]["surrogate"] < 0.5 * out["sim_window_s_mean"]
