@@ -219,6 +219,31 @@ __device__ inline uint4 ld_stream(const uint4* p)
 // PSET: the constants shared by groups of neurons (reversal potentials,
 // synaptic decays, leak, threshold, reset, I_ext) come from a small table in
 // LDS indexed by a 1-byte set id instead of 56 B of per-neuron arrays.
+// One neuron's membrane step (models/block.py:83-111): I_syn from the
+// conductances and J (arrivals added), Euler step of V, refractory hold,
+// threshold and clamp.  Shared by every kernel that updates neurons, so all
+// of them compute bit-identical values.
+__device__ __forceinline__ void lif_core(float V, const float4& J, const float4& g, const float4& vu, float gL,
+                                         float vL, float dtC, float vth, float vrst, float iext, bool not_ref,
+                                         bool& act, float& vout)
+{
+    float isyn = g.x * (vu.x - V) * J.x;
+    isyn += g.y * (vu.y - V) * J.y;
+    isyn += g.z * (vu.z - V) * J.z;
+    isyn += g.w * (vu.w - V) * J.w;
+    const float main_part = -gL * (V - vL);
+    const float vn = V + dtC * (main_part + isyn + iext);
+    const float vi = not_ref ? vn : vrst;
+    act = vi >= vth;
+    vout = vi > vth ? vth : vi;
+}
+
+// J decayed to the next step (models/block.py:71)
+__device__ __forceinline__ float4 lif_decay(const float4& J, const float4& dc)
+{
+    return make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
+}
+
 template <bool PAIR, bool PSET, bool GACC>
 __device__ __forceinline__ void neuron_update(const NeuronArgs& a, int64_t i, int64_t step, double t, int32_t k,
                                               float4 din, const float* s_tab, bool& act, bool& spike, float& vout)
@@ -256,16 +281,7 @@ __device__ __forceinline__ void neuron_update(const NeuronArgs& a, int64_t i, in
         dc = a.decay[i];
         gL = a.gL[i]; vL = a.vL[i]; dtC = a.dtC[i]; vth = a.vth[i]; vrst = a.vrst[i]; iext = a.iext[i];
     }
-    float isyn = g.x * (vu.x - V) * J.x;
-    isyn += g.y * (vu.y - V) * J.y;
-    isyn += g.z * (vu.z - V) * J.z;
-    isyn += g.w * (vu.w - V) * J.w;
-    const float main_part = -gL * (V - vL);
-    const float vn = V + dtC * (main_part + isyn + iext);
-    const bool not_ref = step >= (int64_t)rend;
-    const float vi = not_ref ? vn : vrst;
-    act = vi >= vth;
-    vout = vi > vth ? vth : vi;
+    lif_core(V, J, g, vu, gL, vL, dtC, vth, vrst, iext, step >= (int64_t)rend, act, vout);
     if (ss & 1) {
         // a source's threshold spike comes from the schedule (models/block.py:104-107);
         // a frozen one (an input block's neuron) also keeps its V
@@ -285,7 +301,7 @@ __device__ __forceinline__ void neuron_update(const NeuronArgs& a, int64_t i, in
     }
     const float u = a.noise_mode == DTB_NOISE_UNIFORMS ? un : noise_uniform(a.seed, (uint64_t)step, (uint64_t)i);
     spike = ((u < pr) && !(ss & 2)) || (act && !a.isolated);
-    float4 jn = make_float4(J.x * dc.x, J.y * dc.y, J.z * dc.z, J.w * dc.w);
+    float4 jn = lif_decay(J, dc);
     if (a.ext.n) {
         // external arrivals of this step: part of J at the next step, like this step's deliveries
         const float x = (float)ext_count(a.ext, a.seed, (uint64_t)step, (uint64_t)i) * a.ext.w;
@@ -1573,7 +1589,11 @@ __device__ inline int32_t ld_coherent(const int32_t* p)
 }
 
 // MB blocks of 256 neurons per pass (workgroups of 256*MB threads).
-template <bool PAIR, bool PSET, int MB>
+// RES: every thread owns at most one neuron (the workgroup's blocks fit one
+// pass), whose state (V, J, refractory end, flags) and constants stay in
+// registers for the whole batch; it is read before the first step and written
+// back after the last.  The per-step arithmetic is the neuron kernel's own.
+template <bool PAIR, bool PSET, int MB, bool RES>
 __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
 {
     constexpr int C = PAIR ? 4 : 2;
@@ -1600,6 +1620,35 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
     if (threadIdx.x == 0) s_ok = 1;
     __syncthreads();
     const int32_t E = a.D * kSubLists;
+    // register-resident neuron (RES): block sb = threadIdx / 256 of this workgroup
+    const int32_t rsb = (int32_t)(threadIdx.x >> 8);
+    const int64_t rblk = p + (int64_t)kSubLists * rsb;
+    const int64_t ri = rblk * 256 + (threadIdx.x & 255);
+    const bool rown = RES && rsb < P.m && rblk < P.nblk && ri < a.n;
+    float rV = 0.0f, rgL = 0.0f, rvL = 0.0f, rdtC = 0.0f, rvth = 0.0f, rvrst = 0.0f, riext = 0.0f, rtref = 0.0f;
+    float rpr = 0.0f;
+    float4 rJ = make_float4(0.f, 0.f, 0.f, 0.f), rg = rJ, rvu = rJ, rdc = rJ;
+    int32_t rrend = 0, rpop = -1;
+    uint8_t rflags = 0;
+    if (rown) {
+        rV = a.V[ri];
+        rJ = a.Jp[ri];
+        rrend = a.ref_end[ri];
+        rg = a.g[ri];
+        rpop = a.pop[ri];
+        rtref = a.tref[ri];
+        rpr = a.noise_mode == DTB_NOISE_PER_NEURON ? a.noisep[ri] : a.noise_rate;
+        if (PSET) {
+            const float* T = s_tab + 16 * a.pset[ri];
+            rvu = make_float4(T[0], T[1], T[2], T[3]);
+            rdc = make_float4(T[4], T[5], T[6], T[7]);
+            rgL = T[8]; rvL = T[9]; rdtC = T[10]; rvth = T[11]; rvrst = T[12]; riext = T[13];
+        } else {
+            rvu = a.vu[ri];
+            rdc = a.decay[ri];
+            rgL = a.gL[ri]; rvL = a.vL[ri]; rdtC = a.dtC[ri]; rvth = a.vth[ri]; rvrst = a.vrst[ri]; riext = a.iext[ri];
+        }
+    }
     for (int32_t k = 0; k < P.K; ++k) {
         const int64_t step = a.ctl->nbase + k + 1;
         const double t = a.ttab[k];
@@ -1614,7 +1663,41 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
             bool act = false, spike = false;
             float vout = 0.0f;
             int32_t pp = -1;
-            if (sb < P.m && blk < P.nblk && i < a.n) {
+            if (RES) {
+                if (rown) {
+                    pp = rpop;
+                    float4 d;
+                    if (k == 0) {
+                        if (PAIR) {
+                            d = reinterpret_cast<const float4*>(a.acc)[ri];
+                        } else {
+                            const float2 d2 = reinterpret_cast<const float2*>(a.acc)[ri];
+                            d = make_float4(d2.x, d2.x, d2.y, d2.y);
+                        }
+                    } else if (PAIR) {
+                        d = make_float4((float)s_acc[4 * li], (float)s_acc[4 * li + 1], (float)s_acc[4 * li + 2],
+                                        (float)s_acc[4 * li + 3]);
+                    } else {
+                        const float e = (float)s_acc[2 * li], f = (float)s_acc[2 * li + 1];
+                        d = make_float4(e, e, f, f);
+                    }
+#pragma unroll
+                    for (int c = 0; c < C; ++c) s_acc[C * li + c] = 0;
+                    float4 J = rJ;
+                    J.x += d.x; J.y += d.y; J.z += d.z; J.w += d.w;
+                    lif_core(rV, J, rg, rvu, rgL, rvL, rdtC, rvth, rvrst, riext, step >= (int64_t)rrend, act, vout);
+                    rV = vout;
+                    if (act) {
+                        rrend = refractory_end(step, t, rtref, a.dt);
+                        a.last_step[ri] = (int32_t)step;
+                        a.t_last[ri] = t;
+                    }
+                    const float u = noise_uniform(a.seed, (uint64_t)step, (uint64_t)ri);
+                    spike = (u < rpr) || (act && !a.isolated);
+                    rJ = lif_decay(J, rdc);
+                    rflags = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
+                }
+            } else if (sb < P.m && blk < P.nblk && i < a.n) {
                 pp = a.pop[i];
                 float4 d;
                 if (k == 0) {
@@ -1780,6 +1863,12 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
         if (lane == 0 && ev) atomicAdd((unsigned long long*)&P.rec_events[k], (unsigned long long)ev);
         if (p == 0 && threadIdx.x == 0) P.rec_nspk[k] = cum[kSubLists];
         __syncthreads();
+    }
+    if (rown) {
+        a.V[ri] = rV;
+        a.Jp[ri] = rJ;
+        a.ref_end[ri] = rrend;
+        a.flags[ri] = rflags;
     }
     // the arrivals pending for the next step go to acc, as bucketed delivery leaves them
     for (int32_t li = threadIdx.x; li < P.m * 256; li += NT) {
@@ -2715,12 +2804,18 @@ struct Want {
 template <bool PAIR, bool PSET>
 static void launch_persist_t(const Block* b, const PersistArgs& pa, int mb, size_t lds)
 {
-    if (mb == 4)
-        hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 4>), dim3(b->pG), dim3(1024), lds, b->stream, pa);
-    else if (mb == 2)
-        hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 2>), dim3(b->pG), dim3(512), lds, b->stream, pa);
-    else
-        hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 1>), dim3(b->pG), dim3(256), lds, b->stream, pa);
+    // every thread holds at most one neuron when the workgroup's blocks fit one pass
+    const bool res = b->pm <= mb;
+    if (mb == 4) {
+        if (res) hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 4, true>), dim3(b->pG), dim3(1024), lds, b->stream, pa);
+        else hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 4, false>), dim3(b->pG), dim3(1024), lds, b->stream, pa);
+    } else if (mb == 2) {
+        if (res) hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 2, true>), dim3(b->pG), dim3(512), lds, b->stream, pa);
+        else hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 2, false>), dim3(b->pG), dim3(512), lds, b->stream, pa);
+    } else {
+        if (res) hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 1, true>), dim3(b->pG), dim3(256), lds, b->stream, pa);
+        else hipLaunchKernelGGL((persist_step_kernel<PAIR, PSET, 1, false>), dim3(b->pG), dim3(256), lds, b->stream, pa);
+    }
 }
 
 static void launch_persist(const Block* b, const PersistArgs& pa, int mb, size_t lds)
