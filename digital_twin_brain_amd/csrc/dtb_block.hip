@@ -1574,6 +1574,7 @@ struct PersistArgs {
     int32_t m;                         // blocks per workgroup (the LDS slice holds m*256 neurons)
     int64_t nblk;                      // neuron-kernel blocks: ceil(n / 256)
     int32_t K;                         // steps of the batch
+    int32_t seg_loop;                  // 1: each thread walks its own (short) segment; 0: scan-balanced chunks
     uint32_t* pub;                     // [kSubLists] 64-bit words (step << 32 | spikes), kPubStride apart; abort flag
     uint64_t* __restrict__ rec_events;
     uint32_t* __restrict__ rec_nspk;
@@ -1817,6 +1818,28 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
             }
             const uint32_t len = (uint32_t)(hi - lo);
             ev += len; // every workgroup counts the events it receives
+            if (P.seg_loop) {
+                // short segments: each thread walks its own, four synapse loads
+                // in flight at a time (no scan, no barrier inside the chunk)
+                for (int32_t x0 = lo; x0 < hi; x0 += 4) {
+                    uint32_t cc[4];
+                    int2 vv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (x0 + u >= hi) continue;
+                        cc[u] = P.pcol[x0 + u];
+                        vv[u] = PAIR ? reinterpret_cast<const int2*>(P.pval)[x0 + u] : make_int2(P.pval[x0 + u], 0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (x0 + u >= hi) continue;
+                        const int32_t o = (int32_t)(cc[u] & 0x7fffffffu) * C + (int32_t)(cc[u] >> 31) * (C / 2);
+                        atomicAdd(&s_acc[o], vv[u].x);
+                        if (PAIR) atomicAdd(&s_acc[o + 1], vv[u].y);
+                    }
+                }
+                continue;
+            }
             const uint32_t incl = block_inclusive_scan_n<4 * MB>(len, wtot);
             s_lo[threadIdx.x] = lo;
             s_end[threadIdx.x + 1] = incl;
@@ -2862,6 +2885,8 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         pa.na = na;
         pa.bptr = b->pbptr; pa.pcol = b->pcol; pa.pval = b->pval;
         pa.G = b->pG; pa.m = b->pm; pa.nblk = (b->n + 255) / 256; pa.K = K;
+        // a thread per (row, delay, workgroup) segment when they average at most 8 synapses
+        pa.seg_loop = b->nnz <= 8 * b->n * (int64_t)b->D * b->pG ? 1 : 0;
         pa.pub = b->pbar;
         pa.rec_events = b->rec_events; pa.rec_nspk = b->rec_nspk;
         const size_t lds = sizeof(int) * (size_t)b->pm * 256 * (b->pair ? 4 : 2);

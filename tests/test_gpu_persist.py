@@ -70,8 +70,12 @@ def _run_both(a, b, steps, **kw):
     return recs[0]
 
 
+# the first five have short (row, delay, workgroup) segments (each thread walks
+# its own); the last two long ones (scan-balanced chunks: 3,000 x 400 over 12
+# workgroups averages 33 synapses per segment)
 @pytest.mark.parametrize("n,K,D,pair", [(10_000, 100, 1, False), (2_000, 50, 6, False), (4_000, 80, 4, True),
-                                        (40_000, 60, 3, False), (9_000, 100, 63, False)])
+                                        (40_000, 60, 3, False), (9_000, 100, 63, False), (3_000, 400, 1, False),
+                                        (3_000, 400, 2, True)])
 def test_persistent_equals_bucketed_fixed_point(n, K, D, pair):
     a, b = _pair(n, int(0.8 * n), K, D, pair, seed=n % 97 + D, mode=2)
     assert b.get_launch()["deliver_mode"] == 2
