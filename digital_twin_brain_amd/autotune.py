@@ -87,8 +87,9 @@ def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
     cache = _load() if use_cache else {}
     if ckey in cache:
         best = cache[ckey]
-        nb.set_launch(deliver_mode=min(best["deliver_mode"], 1), deliver_lanes=best["deliver_lanes"],
-                      deliver_grid=best["deliver_grid"], stage_segs=best.get("stage_segs", 256),
+        nb.set_launch(deliver_mode=best.get("shape_mode", min(best["deliver_mode"], 1)),
+                      deliver_lanes=best["deliver_lanes"], deliver_grid=best["deliver_grid"],
+                      stage_segs=best.get("stage_segs", 256),
                       stage_ept=best.get("stage_ept", 32), stage_variant=best.get("stage_variant", 0),
                       accum_occ=best.get("accum_occ", 1))
         if "segment_lists" in best:
@@ -173,7 +174,9 @@ def tune_delivery(nb, run_kwargs, steps=20, use_cache=True, log=None):
         table["best shape step us"] = other
         if persist < other:
             nb.set_launch(deliver_mode=2)
-            best = dict(best, deliver_mode=2, us_per_step=persist)
+            # shape_mode: the mode of the best per-kernel shape, kept for batches the
+            # persistent launch hands back to the per-kernel path
+            best = dict(best, deliver_mode=2, us_per_step=persist, shape_mode=mode)
     if use_cache:
         cache[ckey] = best
         _save(cache)

@@ -606,7 +606,10 @@ def test_autotuner_contract_and_cache(tmp_path, monkeypatch):
     best, table = autotune.tune_delivery(nb, kw, steps=10, use_cache=True)
     finalists = {k[:-1]: v for k, v in table.items() if k.endswith("*")}
     assert 1 <= len(finalists) <= 6
-    key = "/".join(str(best[k]) for k in keys)
+    # when the persistent launch wins, best keeps the winning per-kernel shape and its mode
+    shape = dict(best, deliver_mode=best.get("shape_mode", best["deliver_mode"]))
+    assert best["deliver_mode"] in (0, 1) or ("shape_mode" in best and best["deliver_mode"] == 2)
+    key = "/".join(str(shape[k]) for k in keys)
     assert finalists[key] == min(finalists.values()) == best["us_per_1e6_events"]
     got = nb.get_launch()
     assert all(got[k] == best[k] for k in keys)
