@@ -105,6 +105,7 @@ def lib():
         "dtb_block_bold_state": [_vp, _i32, _i32, _vp],
         "dtb_bold_step_f32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "dtb_bold_step_f64": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        "dtb_bold_run_f64": [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "dtb_enkf_analysis_f64": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
         "dtb_enkf_analysis_f64_noalloc": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
         "dtb_mlp_gemm_bf16_nt": [_i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _i32, _vp, _i64, _vp, _i64, _vp],
@@ -135,7 +136,7 @@ def exported_symbols():
             "dtb_block_set_props", "dtb_block_mul_props_by_pop", "dtb_block_gamma_props_by_pop",
             "dtb_block_replace_props", "dtb_block_set_weights", "dtb_block_set_sources", "dtb_block_set_samples", "dtb_block_attach_bold", "dtb_block_bold_state",
             "dtb_block_set_launch", "dtb_block_get_launch", "dtb_block_set_external",
-            "dtb_bold_step_f32", "dtb_bold_step_f64",
+            "dtb_bold_step_f32", "dtb_bold_step_f64", "dtb_bold_run_f64",
             "dtb_enkf_analysis_f64", "dtb_enkf_analysis_f64_noalloc",
             "dtb_mlp_gemm_bf16_nt", "dtb_mlp_relu_backward", "dtb_mlp_bias_grad", "dtb_mlp_mse_grad",
             "dtb_mlp_adam", "dtb_mlp_transpose_bf16", "dtb_mlp_f32_to_bf16", "dtb_mlp_affine",

@@ -33,6 +33,38 @@ __global__ void bold_kernel(int64_t n, T eps, T ts, T tf, T t0, T div_alpha, T E
     out[i] = V0 * (T(7) * E0 * (T(1) - qn) + T(2) * (T(1) - qn / vn) + (T(2) * E0 - T(0.2)) * (T(1) - vn));
 }
 
+// `steps` consecutive steps of bold_kernel with a constant input per voxel,
+// the state kept in registers: the same per-step arithmetic in the same order.
+__global__ void bold_run_kernel(int64_t n, int32_t steps, double eps, double ts, double tf, double t0,
+                                double div_alpha, double E0, double V0, double dt, double* __restrict__ s,
+                                double* __restrict__ q, double* __restrict__ v, double* __restrict__ f,
+                                const double* __restrict__ u, double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double si = s[i], qi = q[i], vi = v[i], fi = f[i];
+    const double ui = u[i];
+    double o = out[i];
+    for (int32_t k = 0; k < steps; ++k) {
+        const double d_s = eps * ui - si / ts - (fi - 1.0) / tf;
+        const double q_part = fi > 0.0 ? 1.0 - pow(1.0 - E0, 1.0 / fi) : 1.0;
+        const double qn = qi + (fi * q_part / E0 - qi * pow(vi, div_alpha - 1.0)) / t0 * dt;
+        const double vn = vi + (fi - pow(vi, div_alpha)) / t0 * dt;
+        double fn = fi + si * dt;
+        fn = fn > 0.0 ? fn : 0.0;
+        si = si + d_s * dt;
+        qi = qn;
+        vi = vn;
+        fi = fn;
+        o = V0 * (7.0 * E0 * (1.0 - qn) + 2.0 * (1.0 - qn / vn) + (2.0 * E0 - 0.2) * (1.0 - vn));
+    }
+    s[i] = si;
+    q[i] = qi;
+    v[i] = vi;
+    f[i] = fi;
+    out[i] = o;
+}
+
 template <typename T>
 static int bold_step(int64_t n, const double* par, T* s, T* q, T* v, T* f, const T* u, T* out, void* stream)
 {
@@ -248,6 +280,19 @@ int dtb_bold_step_f64(int64_t n, const double* par, double* s, double* q, double
                       double* out, void* stream)
 {
     return dtb::bold_step<double>(n, par, s, q, v, f, u, out, stream);
+}
+
+int dtb_bold_run_f64(int64_t n, int32_t steps, const double* par, double* s, double* q, double* v, double* f,
+                     const double* u, double* out, void* stream)
+{
+    DTB_REQUIRE(n >= 0 && steps >= 0, "n and steps must be >= 0");
+    DTB_REQUIRE(par != nullptr, "par is NULL");
+    if (n == 0 || steps == 0) return DTB_OK;
+    DTB_REQUIRE(s && q && v && f && u && out, "state, input and output arrays are required");
+    hipLaunchKernelGGL(dtb::bold_run_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       steps, par[0], par[1], par[2], par[3], 1.0 / par[4], par[5], par[6], par[7], s, q, v, f, u, out);
+    DTB_TRY_HIP(hipGetLastError());
+    return DTB_OK;
 }
 
 int dtb_enkf_analysis_f64(int32_t E, int32_t V, int32_t S, double* w, double sigma, const double* y, double rho,

@@ -149,9 +149,10 @@ class DataAssimilation(simulation):
         original properties, cuda/src/stage/property.cpp:145-284)."""
         self._push(self._hp)
 
-    def get_hidden_state(self, steps=800, show_info=False):
-        """data_assimilation/DataAssimilation.py:327-351."""
-        out = torch.stack(self.evolve(step=steps, bold_detail=True), dim=0).T  # [voxels, 6]
+    def get_hidden_state(self, steps=800, show_info=False, window_input=False):
+        """data_assimilation/DataAssimilation.py:327-351.  window_input: also keep
+        the window's mean BOLD input per voxel (`last_window_input`)."""
+        out = torch.stack(self.evolve(step=steps, bold_detail=True, window_input=window_input), dim=0).T  # [voxels, 6]
         E = self._ensemble_number
         out = out[: self._da_voxels].reshape(E, self._num_voxel_pblk, 6)[:, self._index_da_voxel_pblk]
         hp = self._hp_log.reshape(E, -1)[:, self.for_hidden_state].reshape(out.shape[0], out.shape[1], -1)

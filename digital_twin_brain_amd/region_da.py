@@ -55,7 +55,7 @@ import torch
 
 from . import networks
 from .data_assimilation import DataAssimilation
-from .surrogate_da import WindowSurrogate, nrmse, window_inputs
+from .surrogate_da import NF as NF_DA, WindowSurrogate, nrmse, window_inputs
 
 HP_COLUMNS = (10, 11, 12, 13)  # g_AMPA, g_NMDA, g_GABAa, g_GABAb of every region
 
@@ -89,6 +89,77 @@ def make_region_da(E, R, member_n, dt=0.1, hp_lo=0.5, hp_hi=2.0, seed=7, truth=T
     return nb, da, truth_hp, (low, high)
 
 
+class RateSurrogate:
+    """Window forecast from a learned window-mean BOLD input.  An MLP maps a
+    region's 4 hyper-parameter logits, its member's mean logits and a one-hot
+    of the region to log of the window's mean BOLD input u (spikes per BOLD
+    step over the region's neurons); the hemodynamics are then integrated
+    from the window-start state (s, q, v, f_in) with that constant input,
+    BOLD step by BOLD step (dtb_bold_run_f64, the Balloon-Windkessel model of
+    the simulation), and act is u per network step.  The hemodynamic state
+    so stays on the model's own trajectories across consecutive surrogate
+    windows; only the rate is learned.  Same interface as WindowSurrogate
+    (add / fit / finetune / predict), the training target being u."""
+
+    def __init__(self, voxels, bold_steps, window_bold_steps, hidden=(256, 256), max_batch=2048, lr=1e-3, seed=0,
+                 device=0):
+        from .simulation import bold_params
+        from .surrogate import MLPSurrogate
+
+        self.V = int(voxels)
+        self.sps = int(bold_steps)           # network steps per BOLD step
+        self.nbs = int(window_bold_steps)    # BOLD steps per window
+        p = bold_params
+        self.par = torch.tensor([p["epsilon"], p["tao_s"], p["tao_f"], p["tao_0"], p["alpha"], p["E_0"], p["V_0"],
+                                 1e-3], dtype=torch.float64, device=torch.device("cuda", device))
+        self.mlp = MLPSurrogate(8 + self.V, 1, hidden=hidden, max_batch=max_batch, lr=lr, seed=seed, device=device)
+        self.X = []
+        self.Y = []
+
+    def _rows(self, X):
+        x = torch.as_tensor(X, dtype=torch.float64).cpu().reshape(-1, self.V, NF_DA)
+        hp = x[..., :4]
+        m = hp.mean(dim=1, keepdim=True).expand(-1, self.V, -1)
+        ids = torch.eye(self.V, dtype=torch.float64).expand(x.shape[0], -1, -1)
+        return torch.cat((hp, m, ids), dim=2).reshape(x.shape[0] * self.V, -1)
+
+    def add(self, X, Y=None, idx=None, u=None):
+        """X [E, 9V] window inputs; u [E, V] the simulated window-mean input."""
+        u = torch.as_tensor(u, dtype=torch.float64).cpu().reshape(-1, 1)
+        self.X.append(self._rows(X))
+        self.Y.append(torch.log(u.clamp_min(1e-6)))
+
+    def fit(self, epochs=None, batch=None, renormalize=True):
+        X = torch.cat(self.X)
+        Y = torch.cat(self.Y)
+        if renormalize:
+            self.mlp.set_normalization(X, Y)
+        return self.mlp.fit(X, Y, epochs=epochs or 20, batch=batch)
+
+    finetune = WindowSurrogate.finetune
+
+    def rate(self, X):
+        """[E, V] predicted window-mean input."""
+        E = torch.as_tensor(X).shape[0]
+        return torch.exp(self.mlp.predict(self._rows(X)).double()).reshape(E, self.V)
+
+    def predict(self, X, idx=None):
+        from . import _native as nat
+
+        X = torch.as_tensor(X, dtype=torch.float64)
+        E = X.shape[0]
+        dev = self.par.device
+        u = self.rate(X).to(dev).clamp_min(1e-5).contiguous().reshape(-1)
+        x = X.reshape(E, self.V, NF_DA)
+        st = [x[..., c].reshape(-1).to(dev).contiguous() for c in (4, 5, 6, 7)]  # s, q, v, f_in
+        out = torch.zeros(E * self.V, dtype=torch.float64, device=dev)
+        nat.check(nat.lib().dtb_bold_run_f64(E * self.V, self.nbs, self.par.data_ptr(), st[0].data_ptr(),
+                                              st[1].data_ptr(), st[2].data_ptr(), st[3].data_ptr(), u.data_ptr(),
+                                              out.data_ptr(), torch.cuda.current_stream(dev).cuda_stream))
+        act = u / self.sps
+        return torch.stack((act, st[0], st[1], st[2], st[3], out), dim=1).reshape(E, self.V, 6)
+
+
 class SubsetJudge:
     """Copies of the first `subset` members on a block of their own (the
     generator draws a member's synapses from its own neuron indices, so the
@@ -103,9 +174,10 @@ class SubsetJudge:
         self.pre_steps = int(round(pre_ms / dt))
         self.pre_steps -= self.pre_steps % self.da.bold_steps
 
-    def simulate(self, hp_log, bold_start, steps):
+    def simulate(self, hp_log, bold_start, steps, with_input=False):
         """hp_log [S, R*4] logits; bold_start [S, R, 4] (s, q, v, f_in) ->
-        the window's hidden-state tail [S, R, 6] (float64, CPU)."""
+        the window's hidden-state tail [S, R, 6] (float64, CPU), and with
+        with_input also its mean BOLD input per region [S, R]."""
         da = self.da
         da._hp_log = torch.as_tensor(hp_log, dtype=torch.float32).reshape(self.S, -1).clone()
         da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
@@ -113,8 +185,11 @@ class SubsetJudge:
         if self.pre_steps:
             da.evolve(self.pre_steps, bold_detail=False)  # the network settles under the new conductances
         da.bold.state_update(torch.as_tensor(bold_start, dtype=torch.float64))
-        da.get_hidden_state(steps)
-        return da._hidden_state[:, :, -6:].double().cpu()
+        da.get_hidden_state(steps, window_input=with_input)
+        tail = da._hidden_state[:, :, -6:].double().cpu()
+        if with_input:
+            return tail, da.last_window_input[: tail.shape[0] * tail.shape[1]].reshape(tail.shape[:2]).cpu()
+        return tail
 
 
 class SubsetSurrogateDA:
@@ -135,6 +210,8 @@ class SubsetSurrogateDA:
         # after any simulated window.  refull=True: a full window over the
         # threshold is followed by another full window.
         self.refull = bool(refull)
+        # a surrogate trained on the window-mean input needs it from every simulated window
+        self.with_input = isinstance(surrogate, RateSurrogate)
         # a judged window's NRMSE divides the RMSE over the judged members by the
         # range of the window's BOLD over the whole ensemble ("ensemble": the
         # judged members' simulated values, the others' forecasts), the scale a
@@ -185,16 +262,21 @@ class SubsetSurrogateDA:
                     # first, then starts from the window's BOLD state, like the judge
                     da.evolve(self.judge.pre_steps, bold_detail=False)
                     _set_member_bold(da, X.reshape(E, -1, 9)[:, :, 4:8])
-                da.get_hidden_state(steps)
+                da.get_hidden_state(steps, window_input=self.with_input)
                 tail = da._hidden_state[:, :, -6:].double().cpu()
                 e = nrmse(pred[:, :, -1].cpu().numpy(), tail[:, :, -1].numpy())
-                self.sur.add(X, tail.reshape(E, -1))
+                u = da.last_window_input[: E * tail.shape[1]].reshape(E, -1).cpu() if self.with_input else None
+                self.sur.add(X, tail.reshape(E, -1), u=u)
             elif t % self.every == 0 or judge_next:
                 kind = "judge"
                 J = np.arange(S)
                 bold0 = X.reshape(E, -1, 9)[J][:, :, 4:8]
-                tail = self.judge.simulate(da._hp_log.reshape(E, -1)[J], bold0, steps)
-                self.sur.add(X[J], tail.reshape(S, -1), idx=J)
+                u = None
+                if self.with_input:
+                    tail, u = self.judge.simulate(da._hp_log.reshape(E, -1)[J], bold0, steps, with_input=True)
+                else:
+                    tail = self.judge.simulate(da._hp_log.reshape(E, -1)[J], bold0, steps)
+                self.sur.add(X[J], tail.reshape(S, -1), idx=J, u=u)
                 full = torch.cat((hpl.double().to(pred.device), fc), dim=2)
                 full[torch.from_numpy(J).to(pred.device), :, -6:] = tail.to(pred.device)
                 da._hidden_state = full
@@ -233,7 +315,7 @@ class SubsetSurrogateDA:
 
 
 def collect_da_windows(da, windows, steps, rng, low, high, R, segment=8, hp_sigma=0.25, bold_sigma=1e-8,
-                       solo_rate=0.8, seed=0, log=None):
+                       solo_rate=0.8, seed=0, log=None, with_input=False):
     """Training windows drawn the way the assimilation moves: segments of
     `segment` windows that start from the reference's logit grid, take the
     assimilation's random-walk step and then an EnKF analysis each window,
@@ -242,9 +324,10 @@ def collect_da_windows(da, windows, steps, rng, low, high, R, segment=8, hp_sigm
     prior range and held fixed (a stand-in truth, never the real one), so
     that the ensemble's hyper-parameters and BOLD states travel as they do
     when assimilating an observation from outside the ensemble.
-    Returns (X [windows*E, 9R], Y [windows*E, 6R])."""
+    Returns (X [windows*E, 9R], Y [windows*E, 6R]), and with with_input also
+    U [windows*E, R], each window's mean BOLD input per region."""
     E = da.ensemble_number
-    Xs, Ys = [], []
+    Xs, Ys, Us = [], [], []
     for w0 in range(0, windows, segment):
         da.hp_random_initialize(low, high, num_da_populations_pblk=R, rng=np.random.default_rng(seed + w0))
         target = int(rng.integers(E))
@@ -257,13 +340,17 @@ def collect_da_windows(da, windows, steps, rng, low, high, R, segment=8, hp_sigm
             da._hp = da.sigmoid_torch(da._hp_log, da._hp_low, da._hp_high)
             X = window_inputs(da)
             da.push_hp()
-            da.get_hidden_state(steps)
+            da.get_hidden_state(steps, window_input=with_input)
             Xs.append(X)
             Ys.append(da._hidden_state[:, :, -6:].double().cpu().reshape(E, -1))
+            if with_input:
+                Us.append(da.last_window_input[: E * R].reshape(E, R).cpu())
             y = da._hidden_state[target, :, -1].reshape(1, -1)
             da.da_filter(y, bold_sigma=bold_sigma, solo_rate=solo_rate)
         if log:
             log(f"region DA: {len(Xs)} of {windows} training windows")
+    if with_input:
+        return torch.cat(Xs), torch.cat(Ys), torch.cat(Us)
     return torch.cat(Xs), torch.cat(Ys)
 
 
@@ -295,7 +382,8 @@ def _restore(da, s):
 def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, windows=20, window_ms=800.0, dt=0.1,
                             subset=16, every=5, nrmse_max=None, hidden=(512, 512, 512), epochs=None,
                             finetune_epochs=None, seed=7, pre_ms=50.0, per_voxel=True, sample_residual=False, log=None,
-                            tune=True, rejudge=0, nrmse_scale="members", refull=False, calib_windows=3, **kw):
+                            tune=True, rejudge=0, nrmse_scale="members", refull=False, calib_windows=3,
+                            surrogate="window", rate_hidden=(256, 256), **kw):
     """Simulator-only DA against surrogate-assisted DA (subset judge) on E
     whole-brain members, from the same state and hyper-parameters, against
     the same truth observations.  Returns a dict of timings, gain, the
@@ -312,13 +400,21 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
         f"warmed in {time.perf_counter() - t0:.1f} s")
     # training windows: assimilation-like segments against pseudo-observations
     t0 = time.perf_counter()
-    X, Y = collect_da_windows(da, train_windows, steps, np.random.default_rng(seed + 3), low, high, R,
-                              seed=seed + 100, log=log)
+    rate = surrogate == "rate"
+    got = collect_da_windows(da, train_windows, steps, np.random.default_rng(seed + 3), low, high, R,
+                             seed=seed + 100, log=log, with_input=rate)
+    X, Y = got[0], got[1]
     t_collect = time.perf_counter() - t0
     say(f"region DA: {train_windows} training windows ({X.shape[0]} samples) in {t_collect:.1f} s")
-    ws = WindowSurrogate(R, hidden=hidden, seed=seed, per_voxel=per_voxel, max_batch=2048 if per_voxel else 256,
-                         use_act=False)
-    ws.add(X, Y)
+    if rate:
+        if sample_residual:
+            raise ValueError("residual draws are for the window surrogate")
+        ws = RateSurrogate(R, da.bold_steps, steps // da.bold_steps, hidden=rate_hidden, seed=seed)
+        ws.add(X, u=got[2])
+    else:
+        ws = WindowSurrogate(R, hidden=hidden, seed=seed, per_voxel=per_voxel, max_batch=2048 if per_voxel else 256,
+                             use_act=False)
+        ws.add(X, Y)
     torch.cuda.synchronize()
     if epochs is None:
         epochs = 5 if per_voxel else 60
@@ -390,8 +486,10 @@ def run_region_surrogate_da(E=256, R=90, member_n=10_000, train_windows=196, win
                nrmse_scale=nrmse_scale, refull=refull,
                build_s=t_build, train_windows=train_windows, train_samples=int(X.shape[0]), collect_s=t_collect,
                fit_s=t_fit, fit_epochs=epochs, fit_loss_first=hist[0], fit_loss_last=hist[-1],
-               surrogate=(f"per-region MLP {2 * 9} -> " if per_voxel else f"per-member MLP {R * 9} -> ") +
-               "x".join(str(h) for h in hidden) + (" -> 6" if per_voxel else f" -> {R * 6}"),
+               surrogate=(f"per-region rate MLP {8 + R} -> " + "x".join(str(h) for h in rate_hidden) +
+                          " -> log mean BOLD input, hemodynamics integrated" if rate else
+                          (f"per-region MLP {2 * 9} -> " if per_voxel else f"per-member MLP {R * 9} -> ") +
+                          "x".join(str(h) for h in hidden) + (" -> 6" if per_voxel else f" -> {R * 6}")),
                sim_window_s_mean=float(t_sim.mean()), sur_window_s_mean=float(res["times"].mean()),
                gain=float(t_sim.sum() / res["times"].sum()), kinds=kinds,
                window_s_by_kind={k: float(np.mean([x for x, kk in zip(res["times"], kinds) if kk == k]))

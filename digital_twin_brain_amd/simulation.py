@@ -151,8 +151,12 @@ class simulation:
                               (0, self.num_voxels * self.populations_per_voxel))
         return torch.from_numpy((act / self.num_neurons_per_voxel_cpu).reshape(-1)).to(self.block_model._dev)
 
-    def evolve(self, step, vmean_option=False, sample_option=False, imean_option=False, bold_detail=False):
-        """simulation/simulation.py:225-299: `step` network steps; BOLD per step on the GPU."""
+    def evolve(self, step, vmean_option=False, sample_option=False, imean_option=False, bold_detail=False,
+               window_input=False):
+        """simulation/simulation.py:225-299: `step` network steps; BOLD per step on the GPU.
+
+        window_input (with bold_detail): also keep, in `last_window_input` [voxels], the
+        window's mean BOLD input (spikes of the voxel per BOLD step over its neurons)."""
         nb = self.block_model.nb
         dev = self.block_model._dev
         S = nb.S
@@ -161,10 +165,31 @@ class simulation:
             # the hidden state needs only the last step's counts: run the window
             # without per-step outputs, then its last BOLD sample with counts
             head = step - self.bold_steps
-            nb.run(head, noise_mode=1, graph=True)
+            tot = None
+            if window_input:
+                # per-population spike totals over the window, batch by batch on the device
+                tot = torch.zeros(S, dtype=torch.int64, device=dev)
+                chunk = 1000 - 1000 % self.bold_steps
+                buf = torch.empty((chunk, S), dtype=torch.int32, device=dev)
+                done = 0
+                while done < head:
+                    k = min(chunk, head - done)
+                    nb.run(k, noise_mode=1, graph=True, freqs=buf[:k])
+                    tot += buf[:k].sum(dim=0, dtype=torch.int64)
+                    done += k
+            else:
+                nb.run(head, noise_mode=1, graph=True)
             f = torch.empty((self.bold_steps, S), dtype=torch.int32, device=dev)
             bold = torch.empty((1, self.num_voxels), dtype=torch.float64, device=dev)
             nb.run(self.bold_steps, noise_mode=1, graph=True, freqs=f, bold=bold)
+            if window_input:
+                tot += f.sum(dim=0, dtype=torch.int64)
+                vop = torch.from_numpy(self._vop.astype(np.int64)).to(dev)
+                ok = vop >= 0
+                cnt = torch.zeros(self.num_voxels, dtype=torch.float64, device=dev)
+                cnt.index_add_(0, vop[ok], tot[ok].double())
+                size = torch.from_numpy(self.num_neurons_per_voxel_cpu.astype(np.float64)).to(dev)
+                self.last_window_input = cnt / size / (step // self.bold_steps)
             b = self.bold
             return self._act(f.index_select(1, sel)[-1]), b.s, b.q, b.v, b.f_in, bold[-1]
         f = torch.empty((step, S), dtype=torch.int32, device=dev)

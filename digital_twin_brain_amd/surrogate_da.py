@@ -122,8 +122,9 @@ class WindowSurrogate:
             parts.append(torch.eye(self.V, dtype=torch.float64).expand(X.shape[0], -1, -1))
         return torch.cat(parts, dim=2).reshape(X.shape[0] * self.V, -1)
 
-    def add(self, X, Y, idx=None):
-        """Rows of X belong to members idx (default: row r is member r mod members)."""
+    def add(self, X, Y, idx=None, u=None):
+        """Rows of X belong to members idx (default: row r is member r mod members);
+        u (a window's mean BOLD input) is not used by this surrogate."""
         X = torch.as_tensor(X, dtype=torch.float64).cpu()
         Y = torch.as_tensor(Y, dtype=torch.float64).reshape(X.shape[0], -1).cpu()
         d = Y - window_base(X, self.V, self.use_act)

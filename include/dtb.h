@@ -335,6 +335,12 @@ int dtb_bold_step_f32(int64_t n, const double* par, float* s, float* q, float* v
                       float* out, void* stream);
 int dtb_bold_step_f64(int64_t n, const double* par, double* s, double* q, double* v, double* f,
                       const double* u, double* out, void* stream);
+/* `steps` calls of dtb_bold_step_f64 with the same input u in one launch (the
+ * state kept in registers; out = the last step's BOLD).  No counterpart in the
+ * reference: the whole-brain DA's rate surrogate integrates a window's
+ * hemodynamics from a predicted window-mean input with it. */
+int dtb_bold_run_f64(int64_t n, int32_t steps, const double* par, double* s, double* q, double* v, double* f,
+                     const double* u, double* out, void* stream);
 
 /* Diffusion EnKF analysis in FP64 on device arrays,
  * data_assimilation/DataAssimilation.py:40-60.  w: [E][V][S] updated in

@@ -243,7 +243,68 @@ def test_surrogate_da_small(adapt):
     assert out["bold_rmse_sur_mean"] <= 2.0 * out["bold_rmse_sim_mean"] + 1e-3, out
 
 
-def test_region_surrogate_da_small():
+def test_bold_run_equals_the_oracle_stepped_with_a_constant_input():
+    """dtb_bold_run_f64: K Balloon-Windkessel steps with a constant input per
+    voxel in one launch, against the FP64 oracle stepped K times."""
+    from digital_twin_brain_amd import _native as nat
+
+    rng = np.random.default_rng(7)
+    n, K = 1000, 800
+    s0, q0, v0, f0 = (rng.uniform(-0.05, 0.05, n), rng.uniform(0.8, 1.1, n), rng.uniform(0.9, 1.1, n),
+                      rng.uniform(0.9, 1.2, n))
+    u = rng.uniform(1e-5, 0.02, n)
+    ob = orc.OracleBOLD(n, s=s0, q=q0, v=v0, f_in=f0)
+    for _ in range(K):
+        want = ob.run(u)
+    dev = torch.device("cuda")
+    st = [torch.from_numpy(x.copy()).to(dev) for x in (s0, q0, v0, f0)]
+    ud = torch.from_numpy(u).to(dev)
+    out = torch.zeros(n, dtype=torch.float64, device=dev)
+    par = torch.from_numpy(ob.par.copy()).to(dev)
+    nat.check(nat.lib().dtb_bold_run_f64(n, K, par.data_ptr(), st[0].data_ptr(), st[1].data_ptr(), st[2].data_ptr(),
+                                          st[3].data_ptr(), ud.data_ptr(), out.data_ptr(), None))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-11, atol=1e-14)
+    for got, w in zip(st, (ob.s, ob.q, ob.v, ob.f_in)):
+        np.testing.assert_allclose(got.cpu().numpy(), w, rtol=1e-11, atol=1e-14)
+
+
+def test_window_input_is_the_mean_bold_input_of_the_window():
+    """simulation.evolve(..., window_input=True): the window's mean BOLD input
+    per voxel equals the mean over its BOLD steps of (the voxel's spikes in
+    the step's 10 network steps) / (its neurons), from the per-step counts of
+    the same window replayed from a snapshot."""
+    from digital_twin_brain_amd import networks
+    from digital_twin_brain_amd.data_assimilation import DataAssimilation
+
+    E, V = 3, 4
+    mem = networks.ensemble_members(E=E, seed=2, n_lo=400, n_hi=1200, k_lo=30, k_hi=60, truth_n=500, truth_k=40)
+    props = networks.ensemble_props(mem, voxels=V)
+    nb = networks.NativeBlock.members(props, mem, max_delay=2, stats_exclude_blocked=True)
+    nb.set_launch(deliver_mode=1, accum_fixed=1)  # integer sums: the replay is exact
+    da = DataAssimilation(network=nb, column=False, ensemble=E, dt=0.1, bold_steps=10, da_populations=E * 2 * V,
+                          da_voxels=E * V, write_path="/tmp")
+    da.evolve(200, bold_detail=True)
+    snap = nb.save_state()
+    steps = 2300  # more than one 1000-step batch, not a multiple of it
+    da.evolve(steps, bold_detail=True, window_input=True)
+    got = da.last_window_input.cpu().numpy()
+    nb.load_state(snap)
+    f = da.evolve(steps)[0].cpu().numpy().astype(np.int64)  # [steps, populations]
+    pops = da.population_id_cpu
+    sizes = da.num_neurons_per_voxel_cpu.astype(np.float64)
+    want = np.zeros(da.num_voxels)
+    for j, p in enumerate(pops):
+        v = da._vop[p]
+        if v >= 0:
+            want[v] += f[:, j].sum()
+    want = want / sizes / (steps // 10)
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+    assert want.sum() > 0
+
+
+@pytest.mark.parametrize("surrogate", ["window", "rate"])
+def test_region_surrogate_da_small(surrogate):
     """Surrogate-assisted DA on whole-brain members (4 g_ui per region, a
     judge simulating copies of the first members every 5th window, full
     simulation after a judged window over the NRMSE threshold) against
@@ -256,7 +317,7 @@ def test_region_surrogate_da_small():
     from digital_twin_brain_amd.region_da import run_region_surrogate_da
 
     out, rec = run_region_surrogate_da(E=32, R=20, member_n=4000, train_windows=32, windows=10, window_ms=200.0,
-                                       subset=4, every=5, seed=3)
+                                       subset=4, every=5, seed=3, surrogate=surrogate)
     cal = out["calibration"]
     # 200 ms windows: the BOLD state at the window end is mostly the start state's, so two
     # simulations agree closely; the surrogate (1,024 training windows) is off by a few percent
