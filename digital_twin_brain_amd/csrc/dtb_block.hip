@@ -1549,9 +1549,10 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 // Persistent delivery (deliver_mode 2) for blocks small enough that every
 // workgroup of one launch is resident: one launch runs all K steps of a
 // batch, so a step costs one grid barrier instead of two or three kernel
-// launches.  Workgroup p (p < G <= kSubLists) owns the neuron-kernel blocks
-// p, p + 32, p + 64, ... (256 neurons each): it updates them, writes their
-// spikes into spike sub-list p (its only writer, so in the standard list
+// launches.  Workgroup p (p < G <= kSubLists) owns the neurons
+// [p * per, (p + 1) * per), per = ceil(n / G), so every workgroup has the
+// same share: it updates them, writes their spikes into spike sub-list p
+// (per never exceeds a sub-list's capacity; its only writer, so in the standard list
 // format without reservation atomics), and after the barrier accumulates
 // every arrival for its own neurons into an int32 LDS slice.  The synapses
 // are a copy of the CSR grouped by owning workgroup inside every (row, delay)
@@ -1573,6 +1574,7 @@ struct PersistArgs {
     int32_t G;                         // workgroups
     int32_t m;                         // blocks per workgroup (the LDS slice holds m*256 neurons)
     int64_t nblk;                      // neuron-kernel blocks: ceil(n / 256)
+    int32_t per;                       // neurons per workgroup: workgroup p owns [p * per, (p + 1) * per)
     int32_t K;                         // steps of the batch
     int32_t seg_loop;                  // 1: each thread walks its own (short) segment; 0: scan-balanced chunks
     uint32_t* pub;                     // [kSubLists] 64-bit words (step << 32 | spikes), kPubStride apart; abort flag
@@ -1622,10 +1624,8 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
     __syncthreads();
     const int32_t E = a.D * kSubLists;
     // register-resident neuron (RES): block sb = threadIdx / 256 of this workgroup
-    const int32_t rsb = (int32_t)(threadIdx.x >> 8);
-    const int64_t rblk = p + (int64_t)kSubLists * rsb;
-    const int64_t ri = rblk * 256 + (threadIdx.x & 255);
-    const bool rown = RES && rsb < P.m && rblk < P.nblk && ri < a.n;
+    const int64_t ri = (int64_t)p * P.per + threadIdx.x;
+    const bool rown = RES && (int32_t)threadIdx.x < P.per && ri < a.n;
     float rV = 0.0f, rgL = 0.0f, rvL = 0.0f, rdtC = 0.0f, rvth = 0.0f, rvrst = 0.0f, riext = 0.0f, rtref = 0.0f;
     float rpr = 0.0f;
     float4 rJ = make_float4(0.f, 0.f, 0.f, 0.f), rg = rJ, rvu = rJ, rdc = rJ;
@@ -1657,10 +1657,8 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
         const int64_t sl = (int64_t)slot * kSubLists + p;
         uint32_t run = 0; // spikes written to sub-list p this step (uniform)
         for (int32_t jb = 0; jb < P.m; jb += MB) {
-            const int32_t sb = jb + (int32_t)(threadIdx.x >> 8);
-            const int64_t blk = p + (int64_t)kSubLists * sb;
-            const int64_t i = blk * 256 + (threadIdx.x & 255);
-            const int li = sb * 256 + (threadIdx.x & 255);
+            const int li = jb * 256 + (int)threadIdx.x; // index in this workgroup's slice
+            const int64_t i = (int64_t)p * P.per + li;
             bool act = false, spike = false;
             float vout = 0.0f;
             int32_t pp = -1;
@@ -1698,7 +1696,7 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
                     rJ = lif_decay(J, rdc);
                     rflags = (uint8_t)((act ? 1 : 0) | (spike ? 2 : 0));
                 }
-            } else if (sb < P.m && blk < P.nblk && i < a.n) {
+            } else if (li < P.per && i < a.n) {
                 pp = a.pop[i];
                 float4 d;
                 if (k == 0) {
@@ -1819,19 +1817,19 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
             const uint32_t len = (uint32_t)(hi - lo);
             ev += len; // every workgroup counts the events it receives
             if (P.seg_loop) {
-                // short segments: each thread walks its own, four synapse loads
+                // short segments: each thread walks its own, eight synapse loads
                 // in flight at a time (no scan, no barrier inside the chunk)
-                for (int32_t x0 = lo; x0 < hi; x0 += 4) {
-                    uint32_t cc[4];
-                    int2 vv[4];
+                for (int32_t x0 = lo; x0 < hi; x0 += 8) {
+                    uint32_t cc[8];
+                    int2 vv[8];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < 8; ++u) {
                         if (x0 + u >= hi) continue;
                         cc[u] = P.pcol[x0 + u];
                         vv[u] = PAIR ? reinterpret_cast<const int2*>(P.pval)[x0 + u] : make_int2(P.pval[x0 + u], 0);
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < 8; ++u) {
                         if (x0 + u >= hi) continue;
                         const int32_t o = (int32_t)(cc[u] & 0x7fffffffu) * C + (int32_t)(cc[u] >> 31) * (C / 2);
                         atomicAdd(&s_acc[o], vv[u].x);
@@ -1894,10 +1892,9 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
         a.flags[ri] = rflags;
     }
     // the arrivals pending for the next step go to acc, as bucketed delivery leaves them
-    for (int32_t li = threadIdx.x; li < P.m * 256; li += NT) {
-        const int64_t blk = p + (int64_t)kSubLists * (li >> 8);
-        const int64_t i = blk * 256 + (li & 255);
-        if (blk >= P.nblk || i >= a.n) continue;
+    for (int32_t li = threadIdx.x; li < P.per; li += NT) {
+        const int64_t i = (int64_t)p * P.per + li;
+        if (i >= a.n) continue;
         if (PAIR)
             reinterpret_cast<float4*>(a.acc)[i] = make_float4((float)s_acc[4 * li], (float)s_acc[4 * li + 1],
                                                               (float)s_acc[4 * li + 2], (float)s_acc[4 * li + 3]);
@@ -2775,7 +2772,7 @@ static int create_from_csr(Block* b, const dtb_block_desc* d)
 
 // ------------------------------------------------------------- the run loop
 static int ensure_persist(Block* b);
-static bool persist_fits(const Block* b, int32_t* G = nullptr, int32_t* m = nullptr);
+static bool persist_fits(const Block* b, int32_t* G = nullptr, int32_t* m = nullptr, int32_t* per = nullptr);
 
 static SegArgs seg_args(const Block* b)
 {
@@ -2884,7 +2881,7 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         PersistArgs pa;
         pa.na = na;
         pa.bptr = b->pbptr; pa.pcol = b->pcol; pa.pval = b->pval;
-        pa.G = b->pG; pa.m = b->pm; pa.nblk = (b->n + 255) / 256; pa.K = K;
+        pa.G = b->pG; pa.m = b->pm; pa.nblk = (b->n + 255) / 256; pa.K = K; pa.per = b->pper;
         // a thread per (row, delay, workgroup) segment when they average at most 8 synapses
         pa.seg_loop = b->nnz <= 8 * b->n * (int64_t)b->D * b->pG ? 1 : 0;
         pa.pub = b->pbar;
@@ -3575,27 +3572,32 @@ namespace dtb {
 // Persistent delivery fits when at most kSubLists workgroups of 256 threads
 // cover the neurons with an LDS slice of at most kPersistMaxLds bytes, the
 // synapse positions fit int32 and the fixed-point sums cannot overflow.
-static bool persist_fits(const Block* b, int32_t* G, int32_t* m)
+static bool persist_fits(const Block* b, int32_t* G, int32_t* m, int32_t* per)
 {
+    // G workgroups own consecutive, equal ranges of ceil(n / G) neurons (at most
+    // a sub-list's capacity), each held in an LDS slice of m 256-neuron blocks
     const int64_t nblk = (b->n + 255) / 256;
-    const int64_t mm = (nblk + kSubLists - 1) / kSubLists;
+    const int64_t g = std::min<int64_t>(kSubLists, nblk);
+    const int64_t pp = g > 0 ? (b->n + g - 1) / g : 0;
+    const int64_t mm = (pp + 255) / 256;
     const int64_t lds = mm * 256 * (b->pair ? 4 : 2) * 4;
-    if (G) *G = (int32_t)std::min<int64_t>(kSubLists, nblk);
+    if (G) *G = (int32_t)g;
     if (m) *m = (int32_t)mm;
+    if (per) *per = (int32_t)pp;
     return b->n > 0 && b->fixed_ok && lds <= kPersistMaxLds && b->nnz < (1ll << 31) &&
            b->n * b->D * (std::min<int64_t>(kSubLists, nblk) + 1) < (1ll << 28);
 }
 
 // (Re)build the grouped copy of the CSR: inside every (row, delay) segment
-// the synapses are ordered by owning workgroup ((post >> 8) mod 32, stable),
-// each stored as its index in that workgroup's LDS slice (the workgroup's
-// blocks back to back) with the kind bit, and its weight rounded to the
+// the synapses are ordered by owning workgroup (post / per, stable), each
+// stored as its index in that workgroup's LDS slice (post - owner * per)
+// with the kind bit, and its weight rounded to the
 // int32 the bucketed accumulation adds (__float2int_rn of the scaled weight).
 static int ensure_persist(Block* b)
 {
     if (!b->persist_dirty && b->pcol) return DTB_OK;
-    int32_t G = 0, m = 0;
-    DTB_REQUIRE(persist_fits(b, &G, &m),
+    int32_t G = 0, m = 0, per = 0;
+    DTB_REQUIRE(persist_fits(b, &G, &m, &per),
                 "persistent delivery needs at most %d x 256 x %d neurons, fixed-point sums and fewer than 2^31 "
                 "synapses", kSubLists, kPersistMaxLds / (256 * 4 * (b->pair ? 4 : 2)), kSubLists);
     DTB_TRY_HIP(hipSetDevice(b->device));
@@ -3624,7 +3626,7 @@ static int ensure_persist(Block* b)
                                                          b->pk.dmask) == d - 1)
                 ++end;
             std::fill(cnt.begin(), cnt.end(), 0);
-            for (int64_t x = e; x < end; ++x) ++cnt[(((col[(size_t)x] & b->pk.idx_mask) >> 8) & (kSubLists - 1)) + 1];
+            for (int64_t x = e; x < end; ++x) ++cnt[(col[(size_t)x] & b->pk.idx_mask) / (uint32_t)per + 1];
             int32_t* bp = &bptr[(size_t)((j * D + d - 1) * (G + 1))];
             bp[0] = (int32_t)e;
             for (int g = 0; g < G; ++g) bp[g + 1] = bp[g] + cnt[g + 1];
@@ -3632,9 +3634,9 @@ static int ensure_persist(Block* b)
             for (int64_t x = e; x < end; ++x) {
                 const uint32_t c = col[(size_t)x];
                 const uint32_t post = c & b->pk.idx_mask;
-                const int g = (int)((post >> 8) & (kSubLists - 1));
+                const int g = (int)(post / (uint32_t)per);
                 const int32_t to = pos[g]++;
-                pcol[(size_t)to] = (((post >> 13) << 8) | (post & 255u)) | (c & 0x80000000u);
+                pcol[(size_t)to] = (post - (uint32_t)g * (uint32_t)per) | (c & 0x80000000u);
                 for (int y = 0; y < W; ++y)
                     pval[(size_t)to * W + y] = (int32_t)std::nearbyint(w[(size_t)x * W + y]);
             }
@@ -3663,6 +3665,7 @@ static int ensure_persist(Block* b)
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     b->pG = G;
     b->pm = m;
+    b->pper = per;
     b->persist_dirty = false;
     for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
     b->graphs.clear();

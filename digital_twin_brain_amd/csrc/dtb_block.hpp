@@ -154,7 +154,8 @@ struct Block {
     uint32_t* pcol = nullptr;   // [nnz] local index | kind << 31
     int32_t* pval = nullptr;    // [nnz] or [nnz][2]: round(scaled weight)
     uint32_t* pbar = nullptr;   // [0] barrier counter, [64] abort flag
-    int32_t pG = 0, pm = 0;     // workgroups, neuron blocks per workgroup
+    int32_t pG = 0, pm = 0;     // workgroups, 256-neuron blocks of LDS slice per workgroup
+    int32_t pper = 0;           // neurons per workgroup (consecutive ranges)
     bool persist_dirty = true;
 
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
