@@ -1573,7 +1573,6 @@ struct PersistArgs {
     const int32_t* __restrict__ pval;  // [nnz] (compact) or [nnz][2] (pair): round(scaled weight)
     int32_t G;                         // workgroups
     int32_t m;                         // blocks per workgroup (the LDS slice holds m*256 neurons)
-    int64_t nblk;                      // neuron-kernel blocks: ceil(n / 256)
     int32_t per;                       // neurons per workgroup: workgroup p owns [p * per, (p + 1) * per)
     int32_t K;                         // steps of the batch
     int32_t seg_loop;                  // 1: each thread walks its own (short) segment; 0: scan-balanced chunks
@@ -2881,7 +2880,7 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
         PersistArgs pa;
         pa.na = na;
         pa.bptr = b->pbptr; pa.pcol = b->pcol; pa.pval = b->pval;
-        pa.G = b->pG; pa.m = b->pm; pa.nblk = (b->n + 255) / 256; pa.K = K; pa.per = b->pper;
+        pa.G = b->pG; pa.m = b->pm; pa.K = K; pa.per = b->pper;
         // a thread per (row, delay, workgroup) segment when they average at most 8 synapses
         pa.seg_loop = b->nnz <= 8 * b->n * (int64_t)b->D * b->pG ? 1 : 0;
         pa.pub = b->pbar;
