@@ -1556,8 +1556,9 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 // format without reservation atomics), and after the barrier accumulates
 // every arrival for its own neurons into an int32 LDS slice.  The synapses
 // are a copy of the CSR grouped by owning workgroup inside every (row, delay)
-// segment, with the weights pre-rounded to the same int32 fixed point as
-// bucketed delivery, so the results equal bucketed delivery bit for bit.
+// segment (one record per synapse: slice index and kind, then the weights
+// pre-rounded to the same int32 fixed point as bucketed delivery, so one
+// load fetches both), so the results equal bucketed delivery bit for bit.
 constexpr int kPersistMaxLds = 64 * 1024; // LDS slice bytes per workgroup
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima), gfx9 encoding
@@ -1569,8 +1570,9 @@ constexpr int kPubWords = kPubAbort + 64;
 struct PersistArgs {
     NeuronArgs na;
     const int32_t* __restrict__ bptr;  // [n*D][G+1]: start of each workgroup's synapses in (row, delay) segments
-    const uint32_t* __restrict__ pcol; // grouped synapses: local index | kind << 31
-    const int32_t* __restrict__ pval;  // [nnz] (compact) or [nnz][2] (pair): round(scaled weight)
+    // grouped synapses, one record each: compact (local index | kind << 31,
+    // round(scaled weight)) as 8 bytes, pair (index | kind, w0, w1, 0) as 16
+    const uint32_t* __restrict__ pcw;
     int32_t G;                         // workgroups
     int32_t m;                         // blocks per workgroup (the LDS slice holds m*256 neurons)
     int32_t per;                       // neurons per workgroup: workgroup p owns [p * per, (p + 1) * per)
@@ -1585,6 +1587,21 @@ struct PersistArgs {
 // with agent-coherent loads (they bypass a stale copy in this XCD's L2);
 // everything else the delivery reads is read-only during the launch or the
 // workgroup's own, and stays cacheable.
+// one grouped synapse: its slice index | kind bit, and its rounded weight(s)
+template <bool PAIR>
+__device__ __forceinline__ void persist_record(const uint32_t* __restrict__ pcw, int64_t x, uint32_t& c, int2& v)
+{
+    if (PAIR) {
+        const uint4 r = reinterpret_cast<const uint4*>(pcw)[x];
+        c = r.x;
+        v = make_int2((int32_t)r.y, (int32_t)r.z);
+    } else {
+        const uint2 r = reinterpret_cast<const uint2*>(pcw)[x];
+        c = r.x;
+        v = make_int2((int32_t)r.y, 0);
+    }
+}
+
 __device__ inline int32_t ld_coherent(const int32_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1824,8 +1841,7 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         if (x0 + u >= hi) continue;
-                        cc[u] = P.pcol[x0 + u];
-                        vv[u] = PAIR ? reinterpret_cast<const int2*>(P.pval)[x0 + u] : make_int2(P.pval[x0 + u], 0);
+                        persist_record<PAIR>(P.pcw, x0 + u, cc[u], vv[u]);
                     }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
@@ -1865,8 +1881,7 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     if (pos[u] < 0) continue;
-                    cc[u] = P.pcol[pos[u]];
-                    vv[u] = PAIR ? reinterpret_cast<const int2*>(P.pval)[pos[u]] : make_int2(P.pval[pos[u]], 0);
+                    persist_record<PAIR>(P.pcw, pos[u], cc[u], vv[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -2879,7 +2894,7 @@ static int enqueue_batch(Block* b, int32_t K, const dtb_run_args* a, const float
     if (persist_batch(b, a, w)) {
         PersistArgs pa;
         pa.na = na;
-        pa.bptr = b->pbptr; pa.pcol = b->pcol; pa.pval = b->pval;
+        pa.bptr = b->pbptr; pa.pcw = b->pcol;
         pa.G = b->pG; pa.m = b->pm; pa.K = K; pa.per = b->pper;
         // a thread per (row, delay, workgroup) segment when they average at most 8 synapses
         pa.seg_loop = b->nnz <= 8 * b->n * (int64_t)b->D * b->pG ? 1 : 0;
@@ -3612,8 +3627,8 @@ static int ensure_persist(Block* b)
                               hipMemcpyDeviceToHost));
     }
     std::vector<int32_t> bptr((size_t)(n * D) * (G + 1));
-    std::vector<uint32_t> pcol((size_t)nnz);
-    std::vector<int32_t> pval(w.size());
+    const int R = b->pair ? 4 : 2; // 32-bit words per record
+    std::vector<uint32_t> rec((size_t)nnz * R, 0u);
     const int W = b->pair ? 2 : 1;
     std::vector<int32_t> cnt(G + 1);
     for (int64_t j = 0; j < n; ++j) {
@@ -3635,9 +3650,9 @@ static int ensure_persist(Block* b)
                 const uint32_t post = c & b->pk.idx_mask;
                 const int g = (int)(post / (uint32_t)per);
                 const int32_t to = pos[g]++;
-                pcol[(size_t)to] = (post - (uint32_t)g * (uint32_t)per) | (c & 0x80000000u);
+                rec[(size_t)to * R] = (post - (uint32_t)g * (uint32_t)per) | (c & 0x80000000u);
                 for (int y = 0; y < W; ++y)
-                    pval[(size_t)to * W + y] = (int32_t)std::nearbyint(w[(size_t)x * W + y]);
+                    rec[(size_t)to * R + 1 + y] = (uint32_t)(int32_t)std::nearbyint(w[(size_t)x * W + y]);
             }
             e = end;
         }
@@ -3645,21 +3660,17 @@ static int ensure_persist(Block* b)
     }
     block_free(b, b->pbptr);
     block_free(b, b->pcol);
-    block_free(b, b->pval);
     b->pbptr = nullptr;
     b->pcol = nullptr;
-    b->pval = nullptr;
     DTB_TRY(alloc_n(b, &b->pbptr, (int64_t)bptr.size()));
-    DTB_TRY(alloc_n(b, &b->pcol, std::max<int64_t>(nnz, 1)));
-    DTB_TRY(alloc_n(b, &b->pval, std::max<int64_t>((int64_t)pval.size(), 1)));
+    DTB_TRY(alloc_n(b, &b->pcol, std::max<int64_t>(nnz * R, R)));
     if (!b->pbar) {
         DTB_TRY(alloc_n(b, &b->pbar, kPubWords));
         DTB_TRY_HIP(hipMemsetAsync(b->pbar, 0, 4 * kPubWords, b->stream));
     }
     DTB_TRY(upload(b, b->pbptr, bptr.data(), 4 * bptr.size()));
     if (nnz > 0) {
-        DTB_TRY(upload(b, b->pcol, pcol.data(), 4 * (size_t)nnz));
-        DTB_TRY(upload(b, b->pval, pval.data(), 4 * pval.size()));
+        DTB_TRY(upload(b, b->pcol, rec.data(), 4 * rec.size()));
     }
     DTB_TRY_HIP(hipStreamSynchronize(b->stream));
     b->pG = G;

@@ -151,8 +151,7 @@ struct Block {
     // persistent delivery (deliver_mode 2): the CSR grouped by owning
     // workgroup inside every (row, delay) segment, rebuilt when the weights change
     int32_t* pbptr = nullptr;   // [n*D][pG+1] start of each workgroup's synapses
-    uint32_t* pcol = nullptr;   // [nnz] local index | kind << 31
-    int32_t* pval = nullptr;    // [nnz] or [nnz][2]: round(scaled weight)
+    uint32_t* pcol = nullptr;   // [nnz] records: local index | kind << 31, round(scaled weight) (x2 pair, + pad)
     uint32_t* pbar = nullptr;   // [0] barrier counter, [64] abort flag
     int32_t pG = 0, pm = 0;     // workgroups, 256-neuron blocks of LDS slice per workgroup
     int32_t pper = 0;           // neurons per workgroup (consecutive ranges)
