@@ -1555,10 +1555,15 @@ __global__ __launch_bounds__(1024) void deliver_accum_kernel(StageArgs s)
 // (per never exceeds a sub-list's capacity; its only writer, so in the standard list
 // format without reservation atomics), and after the barrier accumulates
 // every arrival for its own neurons into an int32 LDS slice.  The synapses
-// are a copy of the CSR grouped by owning workgroup inside every (row, delay)
-// segment (one record per synapse: slice index and kind, then the weights
-// pre-rounded to the same int32 fixed point as bucketed delivery, so one
-// load fetches both), so the results equal bucketed delivery bit for bit.
+// are a copy of the CSR grouped by owning workgroup: each workgroup's
+// synapses are contiguous, in (row, delay) segment order (one record per
+// synapse: slice index and kind, then the weights pre-rounded to the same
+// int32 fixed point as bucketed delivery, so one load fetches both), and so
+// are its segment bounds.  The records and bounds that the workgroups of one
+// XCD read then share no cache line with other workgroups', and that XCD's
+// L2 holds only its own workgroups' share of the block (an eighth of it on
+// 32 workgroups over 8 XCDs, instead of most of its lines).  The results
+// equal bucketed delivery bit for bit.
 constexpr int kPersistMaxLds = 64 * 1024; // LDS slice bytes per workgroup
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima), gfx9 encoding
@@ -1569,7 +1574,7 @@ constexpr int kPubWords = kPubAbort + 64;
 
 struct PersistArgs {
     NeuronArgs na;
-    const int32_t* __restrict__ bptr;  // [n*D][G+1]: start of each workgroup's synapses in (row, delay) segments
+    const int32_t* __restrict__ bptr;  // [G][n*D+1]: start of workgroup g's synapses of each (row, delay) segment
     // grouped synapses, one record each: compact (local index | kind << 31,
     // round(scaled weight)) as 8 bytes, pair (index | kind, w0, w1, 0) as 16
     const uint32_t* __restrict__ pcw;
@@ -1825,9 +1830,9 @@ __global__ __launch_bounds__(256 * MB) void persist_step_kernel(PersistArgs P)
                 const int32_t j = ld_coherent(a.spk + ((int64_t)delay_slot(slot, d, a.L) * kSubLists + e % kSubLists) *
                                                           a.spk_cap + (r - cum[e]));
                 if (j >= 0 && (int64_t)j < a.n) {
-                    const int32_t* bp = P.bptr + ((int64_t)j * a.D + (d - 1)) * (G + 1);
-                    lo = bp[p];
-                    hi = bp[p + 1];
+                    const int32_t* bp = P.bptr + (int64_t)p * (a.n * a.D + 1) + ((int64_t)j * a.D + (d - 1));
+                    lo = bp[0];
+                    hi = bp[1];
                 }
             }
             const uint32_t len = (uint32_t)(hi - lo);
@@ -3602,11 +3607,12 @@ static bool persist_fits(const Block* b, int32_t* G, int32_t* m, int32_t* per)
            b->n * b->D * (std::min<int64_t>(kSubLists, nblk) + 1) < (1ll << 28);
 }
 
-// (Re)build the grouped copy of the CSR: inside every (row, delay) segment
-// the synapses are ordered by owning workgroup (post / per, stable), each
-// stored as its index in that workgroup's LDS slice (post - owner * per)
-// with the kind bit, and its weight rounded to the
-// int32 the bucketed accumulation adds (__float2int_rn of the scaled weight).
+// (Re)build the grouped copy of the CSR: the synapses of each owning
+// workgroup (post / per) back to back, workgroup after workgroup, in
+// (row, delay) segment order inside each (stable), each stored as its index
+// in that workgroup's LDS slice (post - owner * per) with the kind bit, and
+// its weight rounded to the int32 the bucketed accumulation adds
+// (__float2int_rn of the scaled weight).
 static int ensure_persist(Block* b)
 {
     if (!b->persist_dirty && b->pcol) return DTB_OK;
@@ -3626,37 +3632,54 @@ static int ensure_persist(Block* b)
         DTB_TRY_HIP(hipMemcpy(w.data(), b->pair ? (const void*)b->w2 : (const void*)b->w, 4 * w.size(),
                               hipMemcpyDeviceToHost));
     }
-    std::vector<int32_t> bptr((size_t)(n * D) * (G + 1));
-    const int R = b->pair ? 4 : 2; // 32-bit words per record
-    std::vector<uint32_t> rec((size_t)nnz * R, 0u);
-    const int W = b->pair ? 2 : 1;
-    std::vector<int32_t> cnt(G + 1);
+    // segment q = j * D + d - 1 (row j, delay d) is the run of row j whose delay field is d - 1
+    const int64_t NS = n * D;
+    std::vector<int64_t> sb((size_t)NS + 1); // CSR start of every segment
     for (int64_t j = 0; j < n; ++j) {
         int64_t e = rp[(size_t)j];
         for (int64_t d = 1; d <= D; ++d) {
-            // the row is sorted by delay: the segment of delay d is the run whose delay field is d - 1
-            int64_t end = e;
-            while (end < rp[(size_t)j + 1] && (int64_t)(((col[(size_t)end] & 0x7fffffffu) >> b->pk.idx_bits) &
-                                                         b->pk.dmask) == d - 1)
-                ++end;
-            std::fill(cnt.begin(), cnt.end(), 0);
-            for (int64_t x = e; x < end; ++x) ++cnt[(col[(size_t)x] & b->pk.idx_mask) / (uint32_t)per + 1];
-            int32_t* bp = &bptr[(size_t)((j * D + d - 1) * (G + 1))];
-            bp[0] = (int32_t)e;
-            for (int g = 0; g < G; ++g) bp[g + 1] = bp[g] + cnt[g + 1];
-            std::vector<int32_t> pos(bp, bp + G);
-            for (int64_t x = e; x < end; ++x) {
-                const uint32_t c = col[(size_t)x];
-                const uint32_t post = c & b->pk.idx_mask;
-                const int g = (int)(post / (uint32_t)per);
-                const int32_t to = pos[g]++;
-                rec[(size_t)to * R] = (post - (uint32_t)g * (uint32_t)per) | (c & 0x80000000u);
-                for (int y = 0; y < W; ++y)
-                    rec[(size_t)to * R + 1 + y] = (uint32_t)(int32_t)std::nearbyint(w[(size_t)x * W + y]);
-            }
-            e = end;
+            sb[(size_t)(j * D + d - 1)] = e;
+            while (e < rp[(size_t)j + 1] &&
+                   (int64_t)(((col[(size_t)e] & 0x7fffffffu) >> b->pk.idx_bits) & b->pk.dmask) == d - 1)
+                ++e;
         }
         DTB_REQUIRE(e == rp[(size_t)j + 1], "row %lld is not sorted by delay", (long long)j);
+    }
+    sb[(size_t)NS] = rp[(size_t)n];
+    DTB_REQUIRE(rp[(size_t)n] == nnz, "the CSR row pointer does not end at nnz");
+    // bptr[g][q]: first record of workgroup g's part of segment q; bptr[g][NS]
+    // = bptr[g + 1][0].  Pass 1 counts every part into bptr[g][q + 1].
+    const size_t BS = (size_t)NS + 1;
+    std::vector<int32_t> bptr((size_t)G * BS, 0);
+    for (int64_t q = 0; q < NS; ++q)
+        for (int64_t x = sb[(size_t)q]; x < sb[(size_t)q + 1]; ++x)
+            ++bptr[(size_t)((col[(size_t)x] & b->pk.idx_mask) / (uint32_t)per) * BS + (size_t)q + 1];
+    int64_t run = 0;
+    for (int g = 0; g < G; ++g) {
+        int32_t* bp = &bptr[(size_t)g * BS];
+        bp[0] = (int32_t)run;
+        for (size_t q = 0; q < (size_t)NS; ++q) bp[q + 1] += bp[q];
+        run = bp[NS];
+    }
+    DTB_REQUIRE(run == nnz, "grouped synapse count %lld differs from nnz %lld", (long long)run, (long long)nnz);
+    const int R = b->pair ? 4 : 2; // 32-bit words per record
+    std::vector<uint32_t> rec((size_t)nnz * R, 0u);
+    const int W = b->pair ? 2 : 1;
+    std::vector<int32_t> pos((size_t)G);
+    for (int64_t q = 0; q < NS; ++q) {
+        for (int g = 0; g < G; ++g) pos[(size_t)g] = bptr[(size_t)g * BS + (size_t)q];
+        for (int64_t x = sb[(size_t)q]; x < sb[(size_t)q + 1]; ++x) {
+            const uint32_t c = col[(size_t)x];
+            const uint32_t post = c & b->pk.idx_mask;
+            const int g = (int)(post / (uint32_t)per);
+            const int32_t to = pos[(size_t)g]++;
+            rec[(size_t)to * R] = (post - (uint32_t)g * (uint32_t)per) | (c & 0x80000000u);
+            for (int y = 0; y < W; ++y)
+                rec[(size_t)to * R + 1 + y] = (uint32_t)(int32_t)std::nearbyint(w[(size_t)x * W + y]);
+        }
+        for (int g = 0; g < G; ++g)
+            DTB_REQUIRE(pos[(size_t)g] == bptr[(size_t)g * BS + (size_t)q + 1], "grouped segment %lld overruns",
+                        (long long)q);
     }
     block_free(b, b->pbptr);
     block_free(b, b->pcol);

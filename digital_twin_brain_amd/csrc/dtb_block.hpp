@@ -149,8 +149,9 @@ struct Block {
     bool seg_dirty = true;        // the lists must be restarted before the next bucketed batch
 
     // persistent delivery (deliver_mode 2): the CSR grouped by owning
-    // workgroup inside every (row, delay) segment, rebuilt when the weights change
-    int32_t* pbptr = nullptr;   // [n*D][pG+1] start of each workgroup's synapses
+    // workgroup (each workgroup's synapses contiguous, in (row, delay) segment
+    // order), rebuilt when the weights change
+    int32_t* pbptr = nullptr;   // [pG][n*D+1] start of workgroup g's synapses of each segment
     uint32_t* pcol = nullptr;   // [nnz] records: local index | kind << 31, round(scaled weight) (x2 pair, + pad)
     uint32_t* pbar = nullptr;   // [0] barrier counter, [64] abort flag
     int32_t pG = 0, pm = 0;     // workgroups, 256-neuron blocks of LDS slice per workgroup
