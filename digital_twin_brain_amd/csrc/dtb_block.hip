@@ -31,6 +31,7 @@
 #include <tuple>
 
 #include "dtb_block.hpp"
+#include "dtb_group.hpp"
 
 namespace dtb {
 
@@ -3607,11 +3608,11 @@ static bool persist_fits(const Block* b, int32_t* G, int32_t* m, int32_t* per)
            b->n * b->D * (std::min<int64_t>(kSubLists, nblk) + 1) < (1ll << 28);
 }
 
-// (Re)build the grouped copy of the CSR: the synapses of each owning
-// workgroup (post / per) back to back, workgroup after workgroup, in
-// (row, delay) segment order inside each (stable), each stored as its index
-// in that workgroup's LDS slice (post - owner * per) with the kind bit, and
-// its weight rounded to the int32 the bucketed accumulation adds
+// (Re)build the grouped copy of the CSR (group_by_owner): the synapses of
+// each owning workgroup (post / per) back to back, workgroup after
+// workgroup, in (row, delay) segment order inside each, each stored as its
+// index in that workgroup's LDS slice (post - owner * per) with the kind bit,
+// and its weight rounded to the int32 the bucketed accumulation adds
 // (__float2int_rn of the scaled weight).
 static int ensure_persist(Block* b)
 {
@@ -3632,55 +3633,14 @@ static int ensure_persist(Block* b)
         DTB_TRY_HIP(hipMemcpy(w.data(), b->pair ? (const void*)b->w2 : (const void*)b->w, 4 * w.size(),
                               hipMemcpyDeviceToHost));
     }
-    // segment q = j * D + d - 1 (row j, delay d) is the run of row j whose delay field is d - 1
-    const int64_t NS = n * D;
-    std::vector<int64_t> sb((size_t)NS + 1); // CSR start of every segment
-    for (int64_t j = 0; j < n; ++j) {
-        int64_t e = rp[(size_t)j];
-        for (int64_t d = 1; d <= D; ++d) {
-            sb[(size_t)(j * D + d - 1)] = e;
-            while (e < rp[(size_t)j + 1] &&
-                   (int64_t)(((col[(size_t)e] & 0x7fffffffu) >> b->pk.idx_bits) & b->pk.dmask) == d - 1)
-                ++e;
-        }
-        DTB_REQUIRE(e == rp[(size_t)j + 1], "row %lld is not sorted by delay", (long long)j);
-    }
-    sb[(size_t)NS] = rp[(size_t)n];
     DTB_REQUIRE(rp[(size_t)n] == nnz, "the CSR row pointer does not end at nnz");
-    // bptr[g][q]: first record of workgroup g's part of segment q; bptr[g][NS]
-    // = bptr[g + 1][0].  Pass 1 counts every part into bptr[g][q + 1].
-    const size_t BS = (size_t)NS + 1;
-    std::vector<int32_t> bptr((size_t)G * BS, 0);
-    for (int64_t q = 0; q < NS; ++q)
-        for (int64_t x = sb[(size_t)q]; x < sb[(size_t)q + 1]; ++x)
-            ++bptr[(size_t)((col[(size_t)x] & b->pk.idx_mask) / (uint32_t)per) * BS + (size_t)q + 1];
-    int64_t run = 0;
-    for (int g = 0; g < G; ++g) {
-        int32_t* bp = &bptr[(size_t)g * BS];
-        bp[0] = (int32_t)run;
-        for (size_t q = 0; q < (size_t)NS; ++q) bp[q + 1] += bp[q];
-        run = bp[NS];
-    }
-    DTB_REQUIRE(run == nnz, "grouped synapse count %lld differs from nnz %lld", (long long)run, (long long)nnz);
+    GroupedCsr gc;
+    const std::string err = group_by_owner(n, D, G, per, b->pk.idx_bits, b->pk.dmask, b->pk.idx_mask, b->pair,
+                                           rp.data(), col.data(), w.data(), gc);
+    DTB_REQUIRE(err.empty(), "persistent delivery: %s", err.c_str());
+    const std::vector<int32_t>& bptr = gc.bptr;
+    const std::vector<uint32_t>& rec = gc.rec;
     const int R = b->pair ? 4 : 2; // 32-bit words per record
-    std::vector<uint32_t> rec((size_t)nnz * R, 0u);
-    const int W = b->pair ? 2 : 1;
-    std::vector<int32_t> pos((size_t)G);
-    for (int64_t q = 0; q < NS; ++q) {
-        for (int g = 0; g < G; ++g) pos[(size_t)g] = bptr[(size_t)g * BS + (size_t)q];
-        for (int64_t x = sb[(size_t)q]; x < sb[(size_t)q + 1]; ++x) {
-            const uint32_t c = col[(size_t)x];
-            const uint32_t post = c & b->pk.idx_mask;
-            const int g = (int)(post / (uint32_t)per);
-            const int32_t to = pos[(size_t)g]++;
-            rec[(size_t)to * R] = (post - (uint32_t)g * (uint32_t)per) | (c & 0x80000000u);
-            for (int y = 0; y < W; ++y)
-                rec[(size_t)to * R + 1 + y] = (uint32_t)(int32_t)std::nearbyint(w[(size_t)x * W + y]);
-        }
-        for (int g = 0; g < G; ++g)
-            DTB_REQUIRE(pos[(size_t)g] == bptr[(size_t)g * BS + (size_t)q + 1], "grouped segment %lld overruns",
-                        (long long)q);
-    }
     block_free(b, b->pbptr);
     block_free(b, b->pcol);
     b->pbptr = nullptr;
