@@ -62,6 +62,54 @@ class RunArgs(ctypes.Structure):
                 ("sample_v", _vp), ("events", _vp), ("spikes", _vp), ("bold", _vp), ("kernel_ms", _vp)]
 
 
+# argument types of every entry point that returns a status code (tests/test_abi.py
+# checks them against the prototypes in include/dtb.h)
+SIGNATURES = {
+    "dtb_block_create": [_vp, _vp],
+    "dtb_block_create_fixed_indegree": [_vp, _vp, _vp, _f64, _u64, _i32, _i32],
+    "dtb_block_destroy": [_vp],
+    "dtb_block_create_members": [_vp, _vp, _i32, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32, _i32],
+    "dtb_block_create_regions": [_vp, _vp, _i32, _vp, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32,
+                                 _i32],
+    "dtb_block_create_region_replicas": [_vp, _vp, _i32, _i32, _vp, _i32, _f32, _f32, _i32, _u64, _vp, _f64,
+                                         _u64, _i32, _i32],
+    "dtb_block_run": [_vp, _i64, _vp],
+    "dtb_block_get": [_vp, _i32, _vp, _i64, _vp],
+    "dtb_block_info": [_vp, _vp],
+    "dtb_block_time": [_vp, _vp],
+    "dtb_block_state_bytes": [_vp, _vp],
+    "dtb_block_save_state": [_vp, _vp, _i64],
+    "dtb_block_load_state": [_vp, _vp, _i64],
+    "dtb_block_set_props": [_vp, _i64, _vp, _vp, _vp],
+    "dtb_block_mul_props_by_pop": [_vp, _i64, _vp, _vp, _vp],
+    "dtb_block_gamma_props_by_pop": [_vp, _i64, _vp, _vp, _vp, _vp, _u64],
+    "dtb_block_replace_props": [_vp, _vp],
+    "dtb_block_set_weights": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "dtb_block_set_sources": [_vp, _i64, _vp, _vp, _i64, _vp, _vp],
+    "dtb_block_set_samples": [_vp, _i64, _vp],
+    "dtb_block_attach_bold": [_vp, _vp, _i32, _i32, _vp, _vp],
+    "dtb_block_set_launch": [_vp, _i32, _i32],
+    "dtb_block_set_external": [_vp, _i32, _vp, _f32],
+    "dtb_block_get_launch": [_vp, _i32, _vp],
+    "dtb_block_bold_state": [_vp, _i32, _i32, _vp],
+    "dtb_bold_step_f32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "dtb_bold_step_f64": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "dtb_bold_run_f64": [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "dtb_enkf_analysis_f64": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
+    "dtb_enkf_analysis_f64_noalloc": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
+    "dtb_mlp_gemm_bf16_nt": [_i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _i32, _vp, _i64, _vp, _i64, _vp],
+    "dtb_mlp_relu_backward": [_i64, _vp, _vp, _vp, _vp, _vp],
+    "dtb_mlp_bias_grad": [_i32, _i32, _vp, _vp, _vp],
+    "dtb_mlp_mse_grad": [_i64, _vp, _vp, _vp, _f32, _vp, _vp, _vp, _vp],
+    "dtb_mlp_adam": [_i64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _i32, _vp, _vp],
+    "dtb_mlp_transpose_bf16": [_i32, _i32, _vp, _vp, _vp],
+    "dtb_mlp_f32_to_bf16": [_i64, _vp, _vp, _vp],
+    "dtb_mlp_affine": [_i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
+    "dtb_gru_forward": [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
+    "dtb_gru_backward": [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+}
+
+
 _lib = None
 
 
@@ -75,51 +123,7 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     L.dtb_last_error.restype = ctypes.c_char_p
     L.dtb_abi_version.restype = _i32
-    sig = {
-        "dtb_block_create": [_vp, _vp],
-        "dtb_block_create_fixed_indegree": [_vp, _vp, _vp, _f64, _u64, _i32, _i32],
-        "dtb_block_destroy": [_vp],
-        "dtb_block_create_members": [_vp, _vp, _i32, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32, _i32],
-        "dtb_block_create_regions": [_vp, _vp, _i32, _vp, _i32, _f32, _f32, _i32, _u64, _vp, _f64, _u64, _i32,
-                                     _i32],
-        "dtb_block_create_region_replicas": [_vp, _vp, _i32, _i32, _vp, _i32, _f32, _f32, _i32, _u64, _vp, _f64,
-                                             _u64, _i32, _i32],
-        "dtb_block_run": [_vp, _i64, _vp],
-        "dtb_block_get": [_vp, _i32, _vp, _i64, _vp],
-        "dtb_block_info": [_vp, _vp],
-        "dtb_block_time": [_vp, _vp],
-        "dtb_block_state_bytes": [_vp, _vp],
-        "dtb_block_save_state": [_vp, _vp, _i64],
-        "dtb_block_load_state": [_vp, _vp, _i64],
-        "dtb_block_set_props": [_vp, _i64, _vp, _vp, _vp],
-        "dtb_block_mul_props_by_pop": [_vp, _i64, _vp, _vp, _vp],
-        "dtb_block_gamma_props_by_pop": [_vp, _i64, _vp, _vp, _vp, _vp, _u64],
-        "dtb_block_replace_props": [_vp, _vp],
-        "dtb_block_set_weights": [_vp, _i64, _vp, _vp, _vp, _vp],
-        "dtb_block_set_sources": [_vp, _i64, _vp, _vp, _i64, _vp, _vp],
-        "dtb_block_set_samples": [_vp, _i64, _vp],
-        "dtb_block_attach_bold": [_vp, _vp, _i32, _i32, _vp, _vp],
-        "dtb_block_set_launch": [_vp, _i32, _i32],
-        "dtb_block_set_external": [_vp, _i32, _vp, _f32],
-        "dtb_block_get_launch": [_vp, _i32, _vp],
-        "dtb_block_bold_state": [_vp, _i32, _i32, _vp],
-        "dtb_bold_step_f32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
-        "dtb_bold_step_f64": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
-        "dtb_bold_run_f64": [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
-        "dtb_enkf_analysis_f64": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
-        "dtb_enkf_analysis_f64_noalloc": [_i32, _i32, _i32, _vp, _f64, _vp, _f64, _vp, _vp],
-        "dtb_mlp_gemm_bf16_nt": [_i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _i32, _vp, _i64, _vp, _i64, _vp],
-        "dtb_mlp_relu_backward": [_i64, _vp, _vp, _vp, _vp, _vp],
-        "dtb_mlp_bias_grad": [_i32, _i32, _vp, _vp, _vp],
-        "dtb_mlp_mse_grad": [_i64, _vp, _vp, _vp, _f32, _vp, _vp, _vp, _vp],
-        "dtb_mlp_adam": [_i64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _i32, _vp, _vp],
-        "dtb_mlp_transpose_bf16": [_i32, _i32, _vp, _vp, _vp],
-        "dtb_mlp_f32_to_bf16": [_i64, _vp, _vp, _vp],
-        "dtb_mlp_affine": [_i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
-        "dtb_gru_forward": [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
-        "dtb_gru_backward": [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
-    }
-    for name, args in sig.items():
+    for name, args in SIGNATURES.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ctypes.c_int

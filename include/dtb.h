@@ -1,6 +1,7 @@
 /*
  * dtb.h -- C ABI of the MI355X Digital Twin Brain network-step library
- * (libdtb.so, built from digital_twin_brain_amd/csrc/*.hip for gfx950).
+ * (libdtb.so, built from the .hip sources in digital_twin_brain_amd/csrc for
+ * gfx950).
  *
  * Plain C types only: pointers, sizes and POD structs.  Every buffer
  * argument may be host memory or device memory (the library copies with
