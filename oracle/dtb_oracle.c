@@ -6,9 +6,9 @@
  * load it.  The product path (digital_twin_brain_amd/) never calls into it.
  *
  * It restates, in plain C, the reference algorithm of:
- *   - models/block.py:30-150          LIF + conductance block step (a1-a8)
- *   - models/bold_model_pytorch.py:43-78   Balloon-Windkessel Euler step (a12)
- *   - data_assimilation/DataAssimilation.py:40-60  diffusion EnKF (a13),
+ *   - models/block.py:30-150          LIF + conductance block step
+ *   - models/bold_model_pytorch.py:43-78   Balloon-Windkessel Euler step
+ *   - data_assimilation/DataAssimilation.py:40-60  diffusion EnKF,
  *     with the observation noise xi passed in instead of drawn
  *   - Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based
  *     RNG shared by the product and this oracle for noise and network draws
