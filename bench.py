@@ -147,7 +147,9 @@ def cpu_plain(steps, warmup):
         ev += int(outdeg[ob.get("spike").astype(bool)].sum())
     dt = time.perf_counter() - t0
     cores = orc.omp_threads("f32")
-    return dict(value=ev / dt, unit="synaptic events/s", cores=cores, kind="port", cpu=cpu_model(),
+    # cores: the OpenMP threads used; nproc: the CPUs this process may run on (what `nproc` prints)
+    return dict(value=ev / dt, unit="synaptic events/s", cores=cores, nproc=len(os.sched_getaffinity(0)),
+                kind="port", cpu=cpu_model(),
                 sample=(f"the whole plain-block workload: oracle FP32 pull (a full pass over all "
                         f"{int(outdeg.sum())} synapses per step, the torch.sparse.mm algorithm of "
                         f"models/block.py:76-81) on the same 10,000-neuron network, parameters and noise stream as "
